@@ -1,0 +1,122 @@
+/*
+ * pbig_hip.h -- C-ABI of the MI355X (gfx950) pairwise box intersection engine.
+ *
+ * Plain C types only: pointers, sizes and status codes.  Boxes are 2-D float32
+ * records {xmin, xmax, ymin, ymax} (16 bytes, the CoordInfo field order of the
+ * reference, src/pbig/rect.h:6-12); a box's id is its index in the array
+ * (the reference's implicit contract rid == index, src/pbig/pbig.cu:331,737).
+ * Two boxes overlap iff their closed intervals overlap in x and in y
+ * (src/pbig/pbig.cu:562-568, 996-1001).  Each overlapping unordered pair is
+ * reported exactly once, as {lo, hi} (two uint32, lo < hi); pair order in the
+ * output is unspecified (as in the reference, whose order depends on atomics).
+ *
+ * Every entry point returns PBIG_OK (0) or a negative status; the text of the
+ * last error is available from pbig_hip_last_error().  Nothing calls exit().
+ */
+#ifndef PBIG_HIP_H_
+#define PBIG_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PBIG_OK = 0,
+  PBIG_E_ARG = -1,        /* bad argument (null pointer, n too large, bad cell size) */
+  PBIG_E_INVALID_BOX = -2,/* a box has xmin > xmax, ymin > ymax or a NaN */
+  PBIG_E_OVERFLOW = -3,   /* more pairs than the caller's capacity; *n_pairs holds the count */
+  PBIG_E_HIP = -4,        /* HIP runtime error */
+  PBIG_E_NOMEM = -5,      /* device memory exhausted or over the configured budget */
+  PBIG_E_TOO_DENSE = -6   /* one cell holds more boxes than the engine's tile limit */
+};
+
+typedef struct pbig_hip_ctx pbig_hip_ctx;
+
+/* Statistics of the last check on a context. */
+typedef struct pbig_hip_stats {
+  uint32_t cell_size_num;  /* cell side = cell_size_num / cell_size_den */
+  uint32_t cell_size_den;
+  uint32_t cells_per_row;
+  uint32_t reserved0;
+  uint64_t n_boxes;
+  uint64_t n_cells;
+  uint64_t memberships;    /* sum over cells of boxes binned into the cell */
+  uint64_t tile_pairs;     /* work items of the per-cell test */
+  uint64_t n_pairs;        /* overlapping pairs found */
+  uint64_t max_per_cell;
+} pbig_hip_stats;
+
+/* Lifecycle.  Replaces pbig_init (src/pbig/pbig.h:9, pbig.cu:2139-2217) and
+ * pbig_finalize (src/pbig/pbig.h:11, pbig.cu:141): one context per device and
+ * caller thread instead of process-global state.  device_id selects the HIP
+ * device (the reference's deviceID argument, pbig.cu:2176-2177). */
+int pbig_hip_create(pbig_hip_ctx** ctx, int device_id);
+int pbig_hip_destroy(pbig_hip_ctx* ctx);
+const char* pbig_hip_last_error(const pbig_hip_ctx* ctx);
+
+/* Reference cell-size rule (decideCellSize, src/pbig/pbig.cu:1285-1357) on
+ * host boxes: the power-of-two side chosen from the first 1000 boxes. */
+uint32_t pbig_hip_reference_cell_size(const float* h_boxes, uint32_t n, double range);
+
+/* Plain path, device-resident boxes.  Replaces the device part of pbig_check
+ * (src/pbig/pbig.h:10, pbig.cu:1962-1981: plan, map, schedule, match).
+ *   d_boxes     n x {xmin,xmax,ymin,ymax} float32 in device memory (16-byte aligned)
+ *   n           number of boxes, < 2^30
+ *   range       domain side: cells cover [0, range)^2; coordinates outside
+ *               the domain fold into the border cells (any float is accepted)
+ *   cell_size   cell side; 0 = the reference rule on the device boxes' first
+ *               1000 entries (pbig_init's cs = 0, pbig.cu:1371-1374)
+ *   d_pairs     device buffer of pair_capacity x {lo, hi} uint32
+ *   n_pairs     out: number of overlapping pairs (also when > pair_capacity,
+ *               in which case PBIG_E_OVERFLOW is returned and d_pairs holds
+ *               an unspecified subset)
+ *   stream      hipStream_t to run on (NULL = the context's stream)
+ * Synchronous: returns after the pairs are in d_pairs. */
+int pbig_hip_check_device(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
+                          float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
+                          uint64_t* n_pairs, void* stream);
+
+/* Asynchronous form of pbig_hip_check_device: enqueues the whole pipeline on
+ * `stream` and returns without waiting.  cell_size must be > 0.  After the
+ * stream reaches this point, d_result[0] holds the pair count and
+ * d_result[1] an error word (0 = ok; bit 0 invalid box, bit 1 workspace too
+ * small, bit 2 cell too dense).  d_result is 2 x uint64 in device memory.
+ * Workspace is sized from the previous call (or `expect_memberships`); when
+ * bit 1 is set, call again after pbig_hip_reserve(). */
+int pbig_hip_check_device_async(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
+                                float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
+                                uint64_t* d_result, void* stream);
+
+/* Grow the context's workspace for up to n_boxes boxes, n_cells cells and
+ * n_memberships cell memberships (used by the async path). */
+int pbig_hip_reserve(pbig_hip_ctx* ctx, uint64_t n_boxes, uint64_t n_cells, uint64_t n_memberships);
+
+/* Plain path, host boxes in the reference's record layout: n x {xmin, xmax,
+ * ymin, ymax, rid} (20 bytes, RectInfo of src/pbig/rect.h:15-18 with
+ * CType = float).  rid must equal the index.  Copies in, runs, copies the
+ * pairs out to h_pairs (pair_capacity x {lo, hi}).  This is the host half of
+ * pbig_check (pbig.cu:1401 copy-in, 1850-1909 copy-out). */
+int pbig_hip_check_host(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, double range,
+                        float cell_size, uint32_t* h_pairs, uint64_t pair_capacity,
+                        uint64_t* n_pairs);
+
+int pbig_hip_last_stats(const pbig_hip_ctx* ctx, pbig_hip_stats* out);
+
+/* Per-kernel device timing of the asynchronous path (no counterpart in the
+ * reference, whose CUDA-event phase timers, src/pbig/cuda_comm.h:39-45, print
+ * through LOG).  pbig_hip_timing_start arms HIP events around each of the
+ * plain path's kernels for the next max_calls asynchronous calls;
+ * pbig_hip_timing_read waits for them and returns the summed milliseconds per
+ * kernel, in launch order (clear, count, scan_reduce, scan_apply, scatter,
+ * test; n_kernels >= 6), and the number of calls recorded. */
+int pbig_hip_timing_start(pbig_hip_ctx* ctx, int max_calls);
+int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBIG_HIP_H_ */

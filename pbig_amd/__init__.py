@@ -1,0 +1,252 @@
+"""pbig_amd -- MI355X-native pairwise box intersection engine.
+
+Python host binding over the C-ABI of ``include/pbig_hip.h`` (ctypes).  The
+compute path is the HIP library ``libpbig_hip.so`` built by
+``pbig_amd/build.py``; importing this module never falls back to a CPU
+implementation: if the library is missing, every entry point raises.
+
+Names follow the reference API (src/pbig/pbig.h:8-11): ``Engine`` is the
+handle that ``pbig_init``/``pbig_finalize`` manage in the reference, and
+``Engine.check*`` is ``pbig_check``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbig_hip.so")
+
+PBIG_OK = 0
+PBIG_E_ARG = -1
+PBIG_E_INVALID_BOX = -2
+PBIG_E_OVERFLOW = -3
+PBIG_E_HIP = -4
+PBIG_E_NOMEM = -5
+PBIG_E_TOO_DENSE = -6
+
+# The C-ABI entry points declared in include/pbig_hip.h.
+EXPORTS = (
+    "pbig_hip_create",
+    "pbig_hip_destroy",
+    "pbig_hip_last_error",
+    "pbig_hip_reference_cell_size",
+    "pbig_hip_check_device",
+    "pbig_hip_check_device_async",
+    "pbig_hip_reserve",
+    "pbig_hip_check_host",
+    "pbig_hip_last_stats",
+    "pbig_hip_timing_start",
+    "pbig_hip_timing_read",
+)
+
+# Kernels of the plain path in launch order (pbig_hip_timing_read).
+PLAIN_KERNELS = ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test")
+
+# numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)
+RECT_DTYPE = np.dtype([("xmin", "<f4"), ("xmax", "<f4"), ("ymin", "<f4"), ("ymax", "<f4"),
+                       ("rid", "<u4")])
+
+
+class PbigError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pbig status {status}: {message}")
+        self.status = status
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("cell_size_num", ctypes.c_uint32),
+        ("cell_size_den", ctypes.c_uint32),
+        ("cells_per_row", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("n_boxes", ctypes.c_uint64),
+        ("n_cells", ctypes.c_uint64),
+        ("memberships", ctypes.c_uint64),
+        ("tile_pairs", ctypes.c_uint64),
+        ("n_pairs", ctypes.c_uint64),
+        ("max_per_cell", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        d = {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved0"}
+        d["cell_size"] = d["cell_size_num"] / max(d["cell_size_den"], 1)
+        return d
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library(path: str | None = None):
+    """Load libpbig_hip.so (no compute, no GPU needed) and declare signatures."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(p)
+    P, u32, u64, i32, f32, f64 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_double)
+    sig = {
+        "pbig_hip_create": (i32, [ctypes.POINTER(P), i32]),
+        "pbig_hip_destroy": (i32, [P]),
+        "pbig_hip_last_error": (ctypes.c_char_p, [P]),
+        "pbig_hip_reference_cell_size": (u32, [P, u32, f64]),
+        "pbig_hip_check_device": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64), P]),
+        "pbig_hip_check_device_async": (i32, [P, P, u32, f64, f32, P, u64, P, P]),
+        "pbig_hip_reserve": (i32, [P, u64, u64, u64]),
+        "pbig_hip_check_host": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64)]),
+        "pbig_hip_last_stats": (i32, [P, ctypes.POINTER(Stats)]),
+        "pbig_hip_timing_start": (i32, [P, i32]),
+        "pbig_hip_timing_read": (i32, [P, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def reference_cell_size(boxes: np.ndarray, range_: float) -> int:
+    """The reference's decideCellSize (src/pbig/pbig.cu:1285-1357), via the C-ABI."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)
+    return int(load_library().pbig_hip_reference_cell_size(b.ctypes.data, b.shape[0], float(range_)))
+
+
+def _ptr(t) -> int:
+    """Device pointer of a torch tensor (torch is only memory plumbing here)."""
+    return int(t.data_ptr())
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+class Engine:
+    """One HIP context of the engine (the reference's pbig_init .. pbig_finalize)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.pbig_hip_create(ctypes.byref(h), int(device))
+        if rc != PBIG_OK:
+            raise PbigError(rc, f"pbig_hip_create(device={device}) failed")
+        self._h = h
+        self.device = device
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pbig_hip_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _raise(self, rc: int):
+        msg = self._lib.pbig_hip_last_error(self._h)
+        raise PbigError(rc, msg.decode() if msg else "")
+
+    def last_error(self) -> str:
+        msg = self._lib.pbig_hip_last_error(self._h)
+        return msg.decode() if msg else ""
+
+    def stats(self) -> dict:
+        st = Stats()
+        self._lib.pbig_hip_last_stats(self._h, ctypes.byref(st))
+        return st.as_dict()
+
+    def reserve(self, n_boxes: int, n_cells: int, n_memberships: int):
+        rc = self._lib.pbig_hip_reserve(self._h, n_boxes, n_cells, n_memberships)
+        if rc != PBIG_OK:
+            self._raise(rc)
+
+    def timing_start(self, max_calls: int):
+        rc = self._lib.pbig_hip_timing_start(self._h, int(max_calls))
+        if rc != PBIG_OK:
+            self._raise(rc)
+
+    def timing_read(self) -> tuple[dict, int]:
+        """Summed device milliseconds per kernel over the recorded calls."""
+        arr = (ctypes.c_double * 8)()
+        calls = ctypes.c_int(0)
+        rc = self._lib.pbig_hip_timing_read(self._h, arr, 8, ctypes.byref(calls))
+        if rc != PBIG_OK:
+            self._raise(rc)
+        return {k: float(arr[i]) for i, k in enumerate(PLAIN_KERNELS)}, int(calls.value)
+
+    # -- plain path ---------------------------------------------------------
+    def check_device(self, boxes, range_: float, cell_size: float, pairs, stream=None,
+                     allow_overflow: bool = False) -> int:
+        """boxes: (n, 4) float32 device tensor; pairs: (cap, 2) int32/uint32
+        device tensor.  Returns the number of pairs (synchronous)."""
+        n = int(boxes.shape[0])
+        cap = int(pairs.shape[0]) if pairs is not None else 0
+        out = ctypes.c_uint64(0)
+        rc = self._lib.pbig_hip_check_device(self._h, _ptr(boxes) if n else None, n, float(range_),
+                                             float(cell_size), _ptr(pairs) if cap else None, cap,
+                                             ctypes.byref(out), _stream_ptr(stream))
+        if rc == PBIG_E_OVERFLOW and allow_overflow:
+            return int(out.value)
+        if rc != PBIG_OK:
+            self._raise(rc)
+        return int(out.value)
+
+    def check_device_async(self, boxes, range_: float, cell_size: float, pairs, result, stream=None):
+        """Enqueue only.  result: (2,) int64 device tensor <- {n_pairs, error}."""
+        n = int(boxes.shape[0])
+        cap = int(pairs.shape[0])
+        rc = self._lib.pbig_hip_check_device_async(self._h, _ptr(boxes) if n else None, n, float(range_),
+                                                   float(cell_size), _ptr(pairs), cap, _ptr(result),
+                                                   _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._raise(rc)
+
+    def check_host(self, rects, range_: float, cell_size: float = 0.0, capacity: int | None = None):
+        """rects: RectInfo records (RECT_DTYPE) or an (n, 4) float32 array
+        (then rid = index).  Returns an (k, 2) uint32 numpy array of pairs."""
+        rects = as_rects(rects)
+        n = rects.shape[0]
+        cap = int(capacity) if capacity is not None else max(1024, 2 * n)
+        while True:
+            out = np.empty((max(cap, 1), 2), dtype=np.uint32)
+            cnt = ctypes.c_uint64(0)
+            rc = self._lib.pbig_hip_check_host(self._h, rects.ctypes.data if n else None, n, float(range_),
+                                               float(cell_size), out.ctypes.data, cap, ctypes.byref(cnt))
+            if rc == PBIG_E_OVERFLOW and capacity is None:
+                cap = int(cnt.value)
+                continue
+            if rc != PBIG_OK:
+                self._raise(rc)
+            return out[: int(cnt.value)]
+
+
+def as_rects(boxes) -> np.ndarray:
+    """(n, 4) float32 boxes -> RectInfo records with rid = index."""
+    if isinstance(boxes, np.ndarray) and boxes.dtype == RECT_DTYPE:
+        return np.ascontiguousarray(boxes)
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)
+    r = np.empty(b.shape[0], dtype=RECT_DTYPE)
+    r["xmin"], r["xmax"], r["ymin"], r["ymax"] = b[:, 0], b[:, 1], b[:, 2], b[:, 3]
+    r["rid"] = np.arange(b.shape[0], dtype=np.uint32)
+    return r

@@ -1,0 +1,82 @@
+// Reference-compatible C++ API (include/pbig.h, report.h, rect.h) on top of
+// the C-ABI of include/pbig_hip.h.
+//
+//   pbig_init      src/pbig/pbig.cu:2139-2217 (globals, device, pools)
+//   pbig_check     src/pbig/pbig.cu:1962-1981 (plan, map, schedule, match,
+//                  decode into report callbacks, decode.cc:26-138)
+//   pbig_finalize  src/pbig/pbig.cu:141
+//   report         src/pbig/report.cc:6-8
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "pbig.h"
+#include "pbig_hip.h"
+#include "rect.h"
+#include "report.h"
+
+namespace {
+
+pbig_hip_ctx* g_ctx = nullptr;
+unsigned int g_n = 0;
+unsigned int g_range = 0;
+unsigned int g_cs = 0;
+std::vector<unsigned int> g_pairs;
+
+[[noreturn]] void die(const char* what, int rc) {
+  printf("ERROR:%s failed (status %d): %s\n", what, rc, g_ctx ? pbig_hip_last_error(g_ctx) : "no context");
+  if (g_ctx) pbig_hip_destroy(g_ctx);
+  g_ctx = nullptr;
+  exit(-1);
+}
+
+}  // namespace
+
+report::report() : sum(0) {}
+void report::print() { printf("Number of pairs = %llu\n", sum); }
+void report::found(const RectInfo* a, const RectInfo* b) {
+  (void)a;
+  (void)b;
+  sum++;
+}
+
+void pbig_init(unsigned int num, unsigned int range, unsigned int cs, unsigned int deviceID,
+               unsigned int hasProfSteps) {
+  (void)hasProfSteps;
+  if (g_ctx) pbig_finalize();
+  if (cs != 0 && (cs & (cs - 1)) != 0) {
+    printf("ERROR:cell size %u is not a power of two\n", cs);
+    exit(-1);
+  }
+  int rc = pbig_hip_create(&g_ctx, (int)deviceID);
+  if (rc) die("pbig_init", rc);
+  g_n = num;
+  g_range = range;
+  g_cs = cs;
+  rc = pbig_hip_reserve(g_ctx, num, 1, (unsigned long long)num + num / 2);
+  if (rc) die("pbig_init", rc);
+}
+
+void pbig_check(RectInfo* input, report* r) {
+  if (!g_ctx) {
+    printf("ERROR:pbig_check called before pbig_init\n");
+    exit(-1);
+  }
+  unsigned long long cap = g_pairs.size() / 2, np = 0;
+  int rc = pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs,
+                               g_pairs.empty() ? nullptr : g_pairs.data(), cap, (uint64_t*)&np);
+  if (rc == PBIG_E_OVERFLOW) {
+    g_pairs.resize(2 * (np + np / 8 + 1024));
+    cap = g_pairs.size() / 2;
+    rc = pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs, g_pairs.data(), cap,
+                             (uint64_t*)&np);
+  }
+  if (rc) die("pbig_check", rc);
+  for (unsigned long long k = 0; k < np; ++k) (*r)(g_pairs[2 * k], g_pairs[2 * k + 1]);
+}
+
+void pbig_finalize() {
+  if (g_ctx) pbig_hip_destroy(g_ctx);
+  g_ctx = nullptr;
+  std::vector<unsigned int>().swap(g_pairs);
+}

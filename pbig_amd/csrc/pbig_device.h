@@ -1,0 +1,73 @@
+// Shared definitions of the device kernels and their host launcher.
+#ifndef PBIG_DEVICE_H_
+#define PBIG_DEVICE_H_
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace pbig {
+
+constexpr uint32_t kTile = 64;              // boxes per tile: one per lane of a wave64
+constexpr uint32_t kMaxTilesPerCell = 4096;  // 262,144 boxes in one cell
+constexpr int kScanPerThread = 16;
+constexpr uint32_t kScanTile = 256 * kScanPerThread;  // cells per scan block
+constexpr uint32_t kStage = 512;            // pairs staged in LDS per wave
+constexpr uint32_t kFlagX = 0x80000000u;    // box starts in this cell's column
+constexpr uint32_t kFlagY = 0x40000000u;    // box starts in this cell's row
+constexpr uint32_t kIdMask = 0x3FFFFFFFu;
+constexpr uint32_t kMaxBoxes = 0x3FFFFFFFu;
+
+constexpr unsigned long long kErrInvalidBox = 1ull;
+constexpr unsigned long long kErrWorkspace = 2ull;
+constexpr unsigned long long kErrTooDense = 4ull;
+
+struct GridParams {
+  float inv;      // 1 / cell side, rounded once on the host
+  uint32_t nrc;   // cells per row (and per column)
+};
+
+// Device-resident result words of one call.
+struct Meta {
+  unsigned long long n_pairs;
+  unsigned long long error;
+  unsigned long long memberships;
+  unsigned long long items;
+  unsigned int max_per_cell;
+  unsigned int pad[7];
+};
+
+struct ScanPartial {
+  unsigned long long memberships;
+  unsigned long long items;
+  unsigned int max_count;
+  unsigned int pad;
+};
+
+// Device workspace of one context (all sizes are element counts).
+struct Workspace {
+  uint32_t* cnt = nullptr;        // per-cell count, then scatter cursor
+  uint32_t* start = nullptr;      // per-cell exclusive start
+  ScanPartial* partials = nullptr;
+  uint4* items = nullptr;         // {start, count, row tile, column tile}
+  float4* mbox = nullptr;         // cell-ordered box copies
+  uint32_t* mid = nullptr;        // cell-ordered ids | start-cell flags
+  Meta* meta = nullptr;
+  uint64_t cap_cells = 0, cap_partials = 0, cap_items = 0, cap_memb = 0;
+  uint32_t test_blocks = 0;
+};
+
+}  // namespace pbig
+
+namespace pbig {
+// Number of kernels of the plain path, in launch order:
+// clear, count, scan_reduce, scan_apply, scatter, test.
+constexpr int kPlainKernels = 6;
+
+// Optional per-kernel timing: when `events` is non-null, events[k] is
+// recorded before kernel k and events[kPlainKernels] after the last one.
+hipError_t launch_plain(const Workspace& ws, const float4* boxes, uint32_t n, GridParams gp,
+                        uint32_t n_cells, uint2* out, uint64_t cap, hipStream_t s,
+                        hipEvent_t* events = nullptr);
+}  // namespace pbig
+
+#endif  // PBIG_DEVICE_H_

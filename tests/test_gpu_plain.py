@@ -1,0 +1,223 @@
+"""GPU parity of the plain path (HIP kernels through the C-ABI) against the
+CPU oracle.  Pair sets are compared sorted and must be bit-identical."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def eng(torch_cuda):
+    import pbig_amd
+
+    e = pbig_amd.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def manifest():
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)
+
+
+def run_device(torch, eng, boxes, L, cs, cap=None):
+    import oracle
+
+    d = torch.from_numpy(np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)).to("cuda:0")
+    cap = cap if cap is not None else max(1024, 4 * boxes.shape[0])
+    out = torch.empty((cap, 2), dtype=torch.int32, device="cuda:0")
+    n = eng.check_device(d, L, cs, out)
+    return oracle.sort_pairs(out[:n].cpu().numpy().view(np.uint32))
+
+
+def test_c1_matches_golden_and_oracle(torch_cuda, eng, orc, manifest):
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c1_small_uniform()
+    m = manifest["c1_uniform_16k"]
+    assert [float(v) for v in boxes[0]] == m["first_box"]
+    cs = m["reference_cell_size"]
+    got = run_device(torch_cuda, eng, boxes, L, cs)
+    golden = np.load(os.path.join(GOLDEN, "c1_pairs.npy"))
+    assert np.array_equal(got, golden)
+    assert orc.pair_hash(got) == m["sha256"]
+    st = eng.stats()
+    assert st["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
+    assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
+    # cell_size = 0 selects the reference rule on the device boxes
+    got0 = run_device(torch_cuda, eng, boxes, L, 0)
+    assert np.array_equal(got0, golden)
+    assert eng.stats()["cell_size"] == cs
+
+
+@pytest.mark.parametrize("cs", [8, 16, 32, 64, 128, 256, 1024, 12.5])
+def test_c1_any_cell_size(torch_cuda, eng, orc, cs):
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c1_small_uniform()
+    got = run_device(torch_cuda, eng, boxes, L, cs)
+    assert np.array_equal(got, np.load(os.path.join(GOLDEN, "c1_pairs.npy")))
+
+
+def test_c2_full_size(torch_cuda, eng, orc, manifest):
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c2_large_uniform()
+    m = manifest["c2_uniform_4m"]
+    assert [float(v) for v in boxes[0]] == m["first_box"]
+    got = run_device(torch_cuda, eng, boxes, L, m["reference_cell_size"], cap=4 * m["pairs"])
+    assert len(got) == m["pairs"]
+    assert orc.pair_hash(got) == m["sha256"]
+    assert eng.stats()["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
+
+
+def test_c3_full_size(torch_cuda, eng, orc, manifest):
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c3_clustered()
+    m = manifest["c3_clustered_2m"]
+    assert [float(v) for v in boxes[0]] == m["first_box"]
+    got = run_device(torch_cuda, eng, boxes, L, m["reference_cell_size"], cap=4 * m["pairs"])
+    assert len(got) == m["pairs"]
+    assert orc.pair_hash(got) == m["sha256"]
+    st = eng.stats()
+    assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
+
+
+def edge_cases():
+    rng = np.random.default_rng(7)
+    cases = {}
+    cases["empty"] = (np.zeros((0, 4), np.float32), 64.0, 16)
+    cases["single"] = (np.array([[1, 2, 1, 2]], np.float32), 64.0, 16)
+    # closed intervals: edge and corner contact count as overlap
+    cases["touching"] = (np.array([[0, 1, 0, 1], [1, 2, 0, 1], [1, 2, 1, 2], [2, 3, 2, 3],
+                                   [3.0000002, 4, 3, 4]], np.float32), 8.0, 1)
+    # identical boxes: one dense cell, n(n-1)/2 pairs, several 64-box tiles
+    cases["identical_300"] = (np.tile(np.array([[5, 6, 5, 6]], np.float32), (300, 1)), 64.0, 16)
+    # tile boundaries inside one cell
+    for k in (63, 64, 65, 127, 128, 129, 191, 257):
+        c = rng.uniform(1, 15, size=(k, 2)).astype(np.float32)
+        b = np.stack([c[:, 0] - 0.6, c[:, 0] + 0.6, c[:, 1] - 0.6, c[:, 1] + 0.6], 1).astype(np.float32)
+        cases[f"one_cell_{k}"] = (b, 64.0, 16)
+    # coordinates outside [0, range) and negative: fold into border cells
+    c = rng.uniform(-40, 104, size=(3000, 2)).astype(np.float32)
+    w = rng.uniform(0.5, 6, size=(3000, 2)).astype(np.float32)
+    cases["outside_domain"] = (np.stack([c[:, 0], c[:, 0] + w[:, 0], c[:, 1], c[:, 1] + w[:, 1]], 1), 64.0, 8)
+    # large boxes spanning many cells among small ones
+    c = rng.uniform(0, 512, size=(4000, 2)).astype(np.float32)
+    w = np.where(rng.uniform(size=(4000, 2)) < 0.02, rng.uniform(50, 300, size=(4000, 2)),
+                 rng.uniform(0.5, 4, size=(4000, 2))).astype(np.float32)
+    cases["mixed_sizes"] = (np.stack([c[:, 0], c[:, 0] + w[:, 0], c[:, 1], c[:, 1] + w[:, 1]], 1), 512.0, 16)
+    # degenerate (zero-area) boxes and -0.0
+    cases["degenerate"] = (np.array([[0, 0, 0, 0], [-0.0, 0, -0.0, 0], [0, 1, 0, 0], [0, 0, 0, 1],
+                                     [1, 1, 1, 1]], np.float32), 4.0, 1)
+    # subnormal coordinates
+    tiny = np.float32(1e-40)
+    cases["subnormal"] = (np.array([[tiny, 2 * tiny, 0, 1], [2 * tiny, 3 * tiny, 0, 1],
+                                    [3 * tiny, 4 * tiny, 0, 1], [-tiny, 0, 0, 1]], np.float32), 4.0, 1)
+    # infinite extents
+    cases["infinite"] = (np.array([[-np.inf, np.inf, 0, 1], [3, 4, 0.5, 0.7], [10, 11, 5, 6]], np.float32),
+                         16.0, 4)
+    return cases
+
+
+EDGE = edge_cases()
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_edge_cases_vs_brute_force(torch_cuda, eng, orc, name):
+    boxes, L, cs = EDGE[name]
+    want = orc.sort_pairs(orc.brute_pairs(boxes)) if len(boxes) else np.zeros((0, 2), np.uint32)
+    got = run_device(torch_cuda, eng, boxes, L, cs) if len(boxes) else np.zeros((0, 2), np.uint32)
+    if len(boxes):
+        assert np.array_equal(got, want), (name, len(got), len(want))
+
+
+def test_empty_input(torch_cuda, eng):
+    d = torch_cuda.empty((0, 4), dtype=torch_cuda.float32, device="cuda:0")
+    out = torch_cuda.empty((16, 2), dtype=torch_cuda.int32, device="cuda:0")
+    assert eng.check_device(d, 64.0, 16, out) == 0
+
+
+def test_invalid_box_is_rejected(torch_cuda, eng):
+    import pbig_amd
+
+    boxes = np.array([[0, 1, 0, 1], [2, 1, 0, 1]], np.float32)
+    with pytest.raises(pbig_amd.PbigError) as e:
+        run_device(torch_cuda, eng, boxes, 8.0, 1)
+    assert e.value.status == pbig_amd.PBIG_E_INVALID_BOX
+    boxes = np.array([[0, 1, 0, 1], [0, 1, np.nan, 1]], np.float32)
+    with pytest.raises(pbig_amd.PbigError) as e:
+        run_device(torch_cuda, eng, boxes, 8.0, 1)
+    assert e.value.status == pbig_amd.PBIG_E_INVALID_BOX
+    # the engine stays usable after an error
+    assert len(run_device(torch_cuda, eng, np.array([[0, 1, 0, 1], [1, 2, 1, 2]], np.float32), 8.0, 1)) == 1
+
+
+def test_overflow_reports_true_count(torch_cuda, eng):
+    import pbig_amd
+
+    boxes = np.tile(np.array([[5, 6, 5, 6]], np.float32), (100, 1))
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((10, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pytest.raises(pbig_amd.PbigError) as e:
+        eng.check_device(d, 64.0, 16, out)
+    assert e.value.status == pbig_amd.PBIG_E_OVERFLOW
+    assert eng.check_device(d, 64.0, 16, out, allow_overflow=True) == 4950
+
+
+def test_async_matches_sync(torch_cuda, eng, orc):
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c1_small_uniform()
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((65536, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    eng.check_device(d, L, 64, out)  # sizes the workspace
+    for _ in range(3):
+        eng.check_device_async(d, L, 64, out, res)
+    torch_cuda.cuda.synchronize()
+    n, err = [int(v) for v in res.cpu()]
+    assert err == 0
+    got = orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))
+    assert np.array_equal(got, np.load(os.path.join(GOLDEN, "c1_pairs.npy")))
+
+
+def test_host_entry_point_rectinfo(torch_cuda, eng, orc):
+    import pbig_amd
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c1_small_uniform()
+    got = orc.sort_pairs(eng.check_host(pbig_amd.as_rects(boxes), L, 0))
+    assert np.array_equal(got, np.load(os.path.join(GOLDEN, "c1_pairs.npy")))
+    bad = pbig_amd.as_rects(boxes[:10])
+    bad["rid"][3] = 7
+    with pytest.raises(pbig_amd.PbigError) as e:
+        eng.check_host(bad, L, 0)
+    assert e.value.status == pbig_amd.PBIG_E_ARG
+
+
+def test_repeated_calls_shrinking_and_growing(torch_cuda, eng, orc):
+    from pbig_amd import workloads
+
+    for n, L, cs in ((50000, 2048.0, 32), (1000, 256.0, 16), (200000, 4096.0, 64), (10, 16.0, 4)):
+        boxes = workloads.uniform(n, L, 1.0, 8.0, seed=n)
+        got = run_device(torch_cuda, eng, boxes, L, cs)
+        want = orc.sort_pairs(orc.grid_pairs(boxes, L, cs))
+        assert np.array_equal(got, want), n
