@@ -79,6 +79,22 @@ def cpu_baseline(boxes, L, cs, n_pairs):
     }
 
 
+def read_traffic(path_name, cs, bin_target):
+    """HBM bytes per launch by kernel, measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (each scaled by the same counter's reading for a copy of known
+    size) and recorded in profiles/ for this exact configuration."""
+    fn = os.path.join(ROOT, "profiles", f"traffic_plain_uniform_4m_{path_name}_cs{int(cs)}_bt{bin_target}.txt")
+    if not os.path.exists(fn):
+        return None, None
+    out = {}
+    with open(fn) as f:
+        for line in f:
+            if line.startswith("kernel="):
+                kv = dict(tok.split("=", 1) for tok in line.split())
+                out[kv["kernel"]] = float(kv["traffic_bytes"])
+    return out, os.path.relpath(fn, ROOT)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -96,7 +112,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cell-size", type=float, default=0.0, help="0 = the reference rule")
+    ap.add_argument("--cell-size", type=float, default=64.0,
+                    help="fine cell side; 0 = the reference rule (256 on this instance)")
+    ap.add_argument("--path", choices=("auto", "general", "binned"), default="auto")
+    ap.add_argument("--bin-target", type=int, default=3000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (torch) three times, a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -117,20 +139,29 @@ def main():
     d_boxes = torch.from_numpy(boxes).to(dev)
     eng = pbig_amd.Engine(0)
 
-    # Size the workspace and the output, and read the instance's statistics.
     cap = 3 * 1024 * 1024
     d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    # B_alg is priced at the reference rule's cell size (cells C and
+    # memberships M of the single-level grid), whatever cell size runs.
+    eng.configure(pbig_amd.PATH_GENERAL)
+    n_ref = eng.check_device(d_boxes, L, ref_cs, d_pairs)
+    st_ref = eng.stats()
+    C_ref, M_ref = st_ref["n_cells"], st_ref["memberships"]
+    # The measured configuration; this call also sizes the workspace.
+    eng.configure(pbig_amd.PATH_AUTO if args.path == "auto" else
+                  {"general": pbig_amd.PATH_GENERAL, "binned": pbig_amd.PATH_BINNED}[args.path], args.bin_target)
     n_pairs = eng.check_device(d_boxes, L, cs, d_pairs)
+    assert n_pairs == n_ref, (n_pairs, n_ref)
     st = eng.stats()
-    # B_alg is priced at the reference rule's cell size, whatever cs is used.
-    if cs == ref_cs:
-        C_ref, M_ref = st["n_cells"], st["memberships"]
-    else:
-        eng.check_device(d_boxes, L, ref_cs, d_pairs)
-        st_ref = eng.stats()
-        C_ref, M_ref = st_ref["n_cells"], st_ref["memberships"]
-        eng.check_device(d_boxes, L, cs, d_pairs)
     B = alg_bytes(N, C_ref, M_ref, n_pairs)
+
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     stream = torch.cuda.Stream(device=dev)
     result = torch.zeros(2, dtype=torch.int64, device=dev)
@@ -153,16 +184,20 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = start_ev.elapsed_time(end_ev)
-    per_kernel, calls = eng.timing_read()
+    per_kernel, calls, path_name = eng.timing_read()
     res = result.cpu().tolist()
     assert res[1] == 0 and res[0] == n_pairs, res
 
     ms_per_step = gpu_ms / args.steps
     value = n_pairs / (ms_per_step / 1e3)
     avg = {k: v / calls for k, v in per_kernel.items()}
+    # B_alg terms by kernel (the binned kernels do the same four jobs)
     kernel_alg = {"count": B["count"], "scatter": B["scatter"], "test": B["test"],
-                  "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref}
+                  "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref,
+                  "bin_hist": B["count"], "bin_scan": B["scan"], "bin_scatter": B["scatter"],
+                  "bin_test": 20 * M_ref, "bin_compact": 8 * n_pairs}
     dominant = max(avg, key=lambda k: avg[k])
+    traffic, traffic_src = read_traffic(path_name, cs, args.bin_target)
     achieved = kernel_alg[dominant] / (avg[dominant] / 1e3) / 1e9
     out = {
         "metric": "overlaps/s & HBM GB/s (4M boxes); batched vs per-instance; serve p50/p99",
@@ -181,7 +216,8 @@ def main():
             "workload": "plain large instance: 4,194,304 float32 rects, centres U[0,65536)^2, "
                         "sides U[1,32], seed 2, boxes resident in HBM",
             "n_boxes": N, "range": L, "cell_size": cs, "reference_cell_size": ref_cs,
-            "pairs": n_pairs, "memberships": st["memberships"], "cells": st["n_cells"],
+            "pairs": n_pairs, "path": path_name, "cells": st["n_cells"],
+            "bins_per_row": st["bins_per_row"], "cells_per_bin_side": st["cells_per_bin_side"],
         },
         "roofline": {
             "bound": "hbm",
@@ -190,7 +226,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic.get(dominant) if traffic else None,
+            "traffic_source": traffic_src if traffic else None,
             "alg_bytes_per_launch": kernel_alg[dominant],
             "avg_launch_ms": avg[dominant],
         },
@@ -204,7 +241,7 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(boxes, L, cs if cs == ref_cs else ref_cs, n_pairs)
+        out["cpu_baseline"] = cpu_baseline(boxes, L, ref_cs, n_pairs)
     eng.close()
     print(json.dumps(out))
 

@@ -46,8 +46,26 @@ typedef struct pbig_hip_stats {
   uint64_t memberships;    /* sum over cells of boxes binned into the cell */
   uint64_t tile_pairs;     /* work items of the per-cell test */
   uint64_t n_pairs;        /* overlapping pairs found */
-  uint64_t max_per_cell;
+  uint64_t max_per_cell;   /* general path only */
+  int32_t path;            /* PBIG_PATH_GENERAL or PBIG_PATH_BINNED */
+  uint32_t bins_per_row;   /* binned path: coarse bins per row */
+  uint32_t cells_per_bin_side; /* binned path: fine cells per coarse bin side */
+  uint32_t reserved1;
 } pbig_hip_stats;
+
+/* Which implementation of the plain path runs.
+ *   PBIG_PATH_GENERAL  single-level grid in global memory (any input)
+ *   PBIG_PATH_BINNED   two-level grid: coarse bins partitioned in global
+ *                      memory, fine cells binned and tested in LDS; falls
+ *                      back to GENERAL under AUTO when a bin does not fit
+ *   PBIG_PATH_AUTO     BINNED when it applies, else GENERAL (default) */
+enum { PBIG_PATH_AUTO = 0, PBIG_PATH_GENERAL = 1, PBIG_PATH_BINNED = 2 };
+
+typedef struct pbig_hip_config {
+  int32_t path;          /* PBIG_PATH_* */
+  uint32_t bin_target;   /* binned path: boxes per coarse bin to aim for (0 = 3000) */
+  uint32_t reserved0, reserved1;
+} pbig_hip_config;
 
 /* Lifecycle.  Replaces pbig_init (src/pbig/pbig.h:9, pbig.cu:2139-2217) and
  * pbig_finalize (src/pbig/pbig.h:11, pbig.cu:141): one context per device and
@@ -83,9 +101,10 @@ int pbig_hip_check_device(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, d
  * `stream` and returns without waiting.  cell_size must be > 0.  After the
  * stream reaches this point, d_result[0] holds the pair count and
  * d_result[1] an error word (0 = ok; bit 0 invalid box, bit 1 workspace too
- * small, bit 2 cell too dense).  d_result is 2 x uint64 in device memory.
- * Workspace is sized from the previous call (or `expect_memberships`); when
- * bit 1 is set, call again after pbig_hip_reserve(). */
+ * small, bit 2 cell too dense, bit 3 a coarse bin did not fit in LDS).
+ * d_result is 2 x uint64 in device memory.  The path (and the workspace
+ * size) follow the context's last synchronous call; when bit 1 or bit 3 is
+ * set, run the synchronous entry point once and call again. */
 int pbig_hip_check_device_async(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
                                 float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
                                 uint64_t* d_result, void* stream);
@@ -105,15 +124,20 @@ int pbig_hip_check_host(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, doub
 
 int pbig_hip_last_stats(const pbig_hip_ctx* ctx, pbig_hip_stats* out);
 
+/* Choose the implementation and its tuning (no counterpart in the reference,
+ * whose choices are compile-time flags, src/pbig/config.cmake:1-10). */
+int pbig_hip_configure(pbig_hip_ctx* ctx, const pbig_hip_config* cfg);
+
 /* Per-kernel device timing of the asynchronous path (no counterpart in the
  * reference, whose CUDA-event phase timers, src/pbig/cuda_comm.h:39-45, print
  * through LOG).  pbig_hip_timing_start arms HIP events around each of the
  * plain path's kernels for the next max_calls asynchronous calls;
  * pbig_hip_timing_read waits for them and returns the summed milliseconds per
- * kernel, in launch order (clear, count, scan_reduce, scan_apply, scatter,
- * test; n_kernels >= 6), and the number of calls recorded. */
+ * kernel in launch order, the number of calls recorded and the path they
+ * took (n_kernels >= 6).  GENERAL: clear, count, scan_reduce, scan_apply,
+ * scatter, test.  BINNED: hist, scan, scatter, test, compact. */
 int pbig_hip_timing_start(pbig_hip_ctx* ctx, int max_calls);
-int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls);
+int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls, int* path);
 
 #ifdef __cplusplus
 }

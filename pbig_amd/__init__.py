@@ -40,10 +40,16 @@ EXPORTS = (
     "pbig_hip_last_stats",
     "pbig_hip_timing_start",
     "pbig_hip_timing_read",
+    "pbig_hip_configure",
 )
 
-# Kernels of the plain path in launch order (pbig_hip_timing_read).
-PLAIN_KERNELS = ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test")
+PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
+PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned"}
+# Kernels of each path in launch order (pbig_hip_timing_read).
+KERNELS = {
+    PATH_GENERAL: ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test"),
+    PATH_BINNED: ("bin_hist", "bin_scan", "bin_scatter", "bin_test", "bin_compact"),
+}
 
 # numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)
 RECT_DTYPE = np.dtype([("xmin", "<f4"), ("xmax", "<f4"), ("ymin", "<f4"), ("ymax", "<f4"),
@@ -68,12 +74,21 @@ class Stats(ctypes.Structure):
         ("tile_pairs", ctypes.c_uint64),
         ("n_pairs", ctypes.c_uint64),
         ("max_per_cell", ctypes.c_uint64),
+        ("path", ctypes.c_int32),
+        ("bins_per_row", ctypes.c_uint32),
+        ("cells_per_bin_side", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
     ]
 
     def as_dict(self):
-        d = {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved0"}
+        d = {name: int(getattr(self, name)) for name, _ in self._fields_ if not name.startswith("reserved")}
         d["cell_size"] = d["cell_size_num"] / max(d["cell_size_den"], 1)
         return d
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_int32), ("bin_target", ctypes.c_uint32),
+                ("reserved0", ctypes.c_uint32), ("reserved1", ctypes.c_uint32)]
 
 
 _lib = None
@@ -105,7 +120,9 @@ def load_library(path: str | None = None):
         "pbig_hip_check_host": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64)]),
         "pbig_hip_last_stats": (i32, [P, ctypes.POINTER(Stats)]),
         "pbig_hip_timing_start": (i32, [P, i32]),
-        "pbig_hip_timing_read": (i32, [P, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32)]),
+        "pbig_hip_timing_read": (i32, [P, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32),
+                                       ctypes.POINTER(i32)]),
+        "pbig_hip_configure": (i32, [P, ctypes.POINTER(Config)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -186,14 +203,23 @@ class Engine:
         if rc != PBIG_OK:
             self._raise(rc)
 
-    def timing_read(self) -> tuple[dict, int]:
-        """Summed device milliseconds per kernel over the recorded calls."""
+    def timing_read(self) -> tuple[dict, int, str]:
+        """Summed device milliseconds per kernel over the recorded calls, the
+        number of calls and the path they took."""
         arr = (ctypes.c_double * 8)()
         calls = ctypes.c_int(0)
-        rc = self._lib.pbig_hip_timing_read(self._h, arr, 8, ctypes.byref(calls))
+        path = ctypes.c_int(0)
+        rc = self._lib.pbig_hip_timing_read(self._h, arr, 8, ctypes.byref(calls), ctypes.byref(path))
         if rc != PBIG_OK:
             self._raise(rc)
-        return {k: float(arr[i]) for i, k in enumerate(PLAIN_KERNELS)}, int(calls.value)
+        names = KERNELS.get(path.value, ())
+        return {k: float(arr[i]) for i, k in enumerate(names)}, int(calls.value), PATH_NAMES.get(path.value, "")
+
+    def configure(self, path: int = PATH_AUTO, bin_target: int = 0):
+        cfg = Config(int(path), int(bin_target), 0, 0)
+        rc = self._lib.pbig_hip_configure(self._h, ctypes.byref(cfg))
+        if rc != PBIG_OK:
+            self._raise(rc)
 
     # -- plain path ---------------------------------------------------------
     def check_device(self, boxes, range_: float, cell_size: float, pairs, stream=None,

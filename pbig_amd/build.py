@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-HIP_SOURCES = ["pbig_kernels.hip", "pbig_capi.hip"]
+HIP_SOURCES = ["pbig_kernels.hip", "pbig_binned.hip", "pbig_capi.hip"]
 LIB_HIP = os.path.join(HERE, "libpbig_hip.so")
 LIB_API = os.path.join(HERE, "libpbig.so")
 

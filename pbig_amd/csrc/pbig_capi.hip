@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "pbig_binned.h"
 #include "pbig_device.h"
 #include "pbig_hip.h"
 
@@ -20,6 +21,8 @@ using pbig::GridParams;
 using pbig::Meta;
 using pbig::ScanPartial;
 using pbig::Workspace;
+using pbig::BinnedParams;
+using pbig::BinnedWorkspace;
 
 struct pbig_hip_ctx {
   int device = 0;
@@ -34,13 +37,21 @@ struct pbig_hip_ctx {
   uint64_t cap_h_in = 0;
   uint2* d_pairs = nullptr;
   uint64_t cap_pairs = 0;
+  BinnedWorkspace bws;
+  pbig_hip_config cfg{PBIG_PATH_AUTO, 3000, 0, 0};
+  int last_path = 0;
   pbig_hip_stats stats{};
   std::string err;
   // Per-kernel timing (pbig_hip_timing_*): one event set per recorded call.
   std::vector<hipEvent_t> events;
   int timing_cap = 0;     // event sets allocated
   int timing_used = 0;    // event sets recorded since the last read
+  int timing_path = 0;
 };
+
+namespace {
+constexpr int kEventsPerCall = 8;
+}
 
 namespace {
 
@@ -138,12 +149,14 @@ int prepare(pbig_hip_ctx* c, uint32_t n, uint32_t n_cells) {
   return ensure_work(c, memb, items);
 }
 
+hipEvent_t* next_events(pbig_hip_ctx* c, int path);
+
 int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, uint32_t n_cells,
              uint2* d_out, uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
   int rc;
   if ((rc = prepare(c, n, n_cells))) return rc;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    HIP_TRY(c, pbig::launch_plain(c->ws, d_boxes, n, gp, n_cells, d_out, cap, s));
+    HIP_TRY(c, pbig::launch_plain(c->ws, d_boxes, n, gp, n_cells, d_out, cap, s, next_events(c, PBIG_PATH_GENERAL)));
     HIP_TRY(c, hipMemcpyAsync(c->h_meta, c->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     const Meta m = *c->h_meta;
@@ -174,6 +187,161 @@ int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, 
   return fail(c, PBIG_E_HIP, "unreachable");
 }
 
+// Plan of the binned path for n boxes, domain side `range`, fine cell side
+// `cell`: K fine cells per coarse bin side (1..32) such that a coarse bin
+// holds about cfg.bin_target boxes, with at most kMaxBins bins.
+bool plan_binned(const pbig_hip_ctx* c, uint32_t n, double range, float cell, BinnedParams* bp) {
+  const double nf0 = std::max(1.0, std::ceil(range / (double)cell));
+  const double target = std::max<double>(16.0, c->cfg.bin_target);
+  uint32_t k = (uint32_t)std::lround(std::sqrt(target / std::max<double>(n, 1)) * nf0);
+  k = std::min<uint32_t>(pbig::kMaxK, std::max<uint32_t>(1, k));
+  uint32_t nbx = 0;
+  while (true) {
+    const double nbxd = std::ceil(nf0 / k);
+    if (nbxd * nbxd <= pbig::kMaxBins) { nbx = (uint32_t)nbxd; break; }
+    if (k == pbig::kMaxK) return false;
+    ++k;
+  }
+  bp->inv_f = 1.0f / cell;
+  bp->k = k;
+  bp->k_magic = ((1ull << 32) + k - 1) / k;
+  bp->nbx = nbx;
+  bp->nb = nbx * nbx;
+  bp->nf = nbx * k;
+  uint32_t nw = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(8, ((uint64_t)n / 16384 + 7) / 8 * 8));
+  bp->nw = nw;
+  bp->chunk = std::max<uint32_t>(1024, (uint32_t)(((uint64_t)n + nw - 1) / nw));
+  bp->test_blocks = std::max<uint32_t>((uint32_t)c->n_cu, (bp->nb + pbig::kMaxBinsPerGroup - 1) / pbig::kMaxBinsPerGroup);
+  return true;
+}
+
+int ensure_binned(pbig_hip_ctx* c, const BinnedParams& bp, uint64_t bucket) {
+  BinnedWorkspace& w = c->bws;
+  int rc;
+  uint64_t hist = (uint64_t)bp.nw * (bp.nb + 1), off = (uint64_t)bp.nw * bp.nb;
+  if ((rc = grow(c, &w.hist, &w.cap_hist, hist))) return rc;
+  if ((rc = grow(c, &w.off, &w.cap_off, off))) return rc;
+  if (w.cap_bins < bp.nb || !w.total) {
+    for (uint32_t** q : {&w.total, &w.bin_start, &w.overfull})
+      if (*q) HIP_TRY(c, hipFree(*q));
+    w.total = w.bin_start = w.overfull = nullptr;
+    w.cap_bins = 0;
+    uint64_t d1 = 0, d2 = 0, d3 = 0;
+    if ((rc = grow(c, &w.total, &d1, bp.nb))) return rc;
+    if ((rc = grow(c, &w.bin_start, &d2, bp.nb))) return rc;
+    if ((rc = grow(c, &w.overfull, &d3, bp.nb))) return rc;
+    w.cap_bins = bp.nb;
+  }
+  if ((rc = grow(c, &w.wg_count, &w.cap_wgc, bp.test_blocks))) return rc;
+  uint64_t scratch = (uint64_t)bp.test_blocks * w.cap_wg;
+  if (w.cap_scratch < scratch || !w.scratch) {
+    if (w.scratch) HIP_TRY(c, hipFree(w.scratch));
+    w.scratch = nullptr;
+    w.cap_scratch = 0;
+    uint64_t d = 0;
+    if ((rc = grow(c, &w.scratch, &d, scratch))) return rc;
+    w.cap_scratch = scratch;
+  }
+  if (w.cap_bucket < bucket || !w.bbox) {
+    if (w.bbox) HIP_TRY(c, hipFree(w.bbox));
+    if (w.bid) HIP_TRY(c, hipFree(w.bid));
+    w.bbox = nullptr;
+    w.bid = nullptr;
+    w.cap_bucket = 0;
+    uint64_t d1 = 0, d2 = 0;
+    if ((rc = grow(c, &w.bbox, &d1, bucket))) return rc;
+    if ((rc = grow(c, &w.bid, &d2, bucket))) return rc;
+    w.cap_bucket = bucket;
+  }
+  w.meta = c->ws.meta;
+  return PBIG_OK;
+}
+
+int prepare_binned(pbig_hip_ctx* c, uint32_t n, const BinnedParams& bp) {
+  uint64_t bucket = std::max<uint64_t>(c->bws.cap_bucket, (uint64_t)n + n / 4 + 4096);
+  // Pair region per test workgroup: at least the previous need, first guess
+  // one pair per box spread over the workgroups, plus slack.
+  uint64_t per = std::max<uint64_t>(8192, ((uint64_t)n + bp.test_blocks - 1) / bp.test_blocks * 2);
+  c->bws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, std::max<uint64_t>(c->bws.cap_wg, per));
+  return ensure_binned(c, bp, bucket);
+}
+
+hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
+  if (c->timing_used >= c->timing_cap) return nullptr;
+  c->timing_path = path;
+  return &c->events[(size_t)c->timing_used++ * kEventsPerCall];
+}
+
+// Binned path, synchronous.  Returns PBIG_OK, an error, or 1 when a coarse
+// bin did not fit in LDS (the caller then runs the general path).
+int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, const BinnedParams& bp, uint2* d_out,
+                    uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
+  int rc;
+  if ((rc = prepare_binned(c, n, bp))) return rc;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    HIP_TRY(c, pbig::launch_binned(c->bws, d_boxes, n, bp, d_out, cap, s, next_events(c, PBIG_PATH_BINNED)));
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, c->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const Meta m = *c->h_meta;
+    if (m.error & pbig::kErrInvalidBox)
+      return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    if (m.error & pbig::kErrWorkspace) {
+      if (attempt == 1) return fail(c, PBIG_E_NOMEM, "bin workspace did not fit after regrowth");
+      if (m.items > c->bws.cap_wg)
+        c->bws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, m.items + m.items / 4 + 1024);
+      if ((rc = ensure_binned(c, bp, std::max<uint64_t>(c->bws.cap_bucket, m.memberships + m.memberships / 16 + 4096))))
+        return rc;
+      continue;
+    }
+    if (m.error & pbig::kErrOverfull) return 1;
+    c->last_path = PBIG_PATH_BINNED;
+    c->stats.n_boxes = n;
+    c->stats.n_cells = (uint64_t)bp.nf * bp.nf;
+    c->stats.cells_per_row = bp.nf;
+    c->stats.memberships = m.memberships;
+    c->stats.tile_pairs = 0;
+    c->stats.n_pairs = m.n_pairs;
+    c->stats.max_per_cell = 0;
+    c->stats.path = PBIG_PATH_BINNED;
+    c->stats.bins_per_row = bp.nbx;
+    c->stats.cells_per_bin_side = bp.k;
+    *n_pairs = m.n_pairs;
+    if (m.n_pairs > cap)
+      return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu",
+                  (unsigned long long)m.n_pairs, (unsigned long long)cap);
+    return PBIG_OK;
+  }
+  return fail(c, PBIG_E_HIP, "unreachable");
+}
+
+// Either path, synchronous, by the context's configuration.
+int run_any_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, double range, float cell, uint2* d_out,
+                 uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
+  int rc;
+  if (c->cfg.path != PBIG_PATH_GENERAL) {
+    BinnedParams bp;
+    if (plan_binned(c, n, range, cell, &bp)) {
+      rc = run_binned_sync(c, d_boxes, n, bp, d_out, cap, s, n_pairs);
+      if (rc != 1) return rc;
+      if (c->cfg.path == PBIG_PATH_BINNED)
+        return fail(c, PBIG_E_TOO_DENSE, "a coarse bin holds more than %u fine memberships", pbig::kMaxFine);
+    } else if (c->cfg.path == PBIG_PATH_BINNED) {
+      return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
+    }
+  }
+  GridParams gp;
+  uint32_t n_cells;
+  if ((rc = grid_of(c, range, cell, &gp, &n_cells))) return rc;
+  rc = run_sync(c, d_boxes, n, gp, n_cells, d_out, cap, s, n_pairs);
+  if (rc == PBIG_OK || rc == PBIG_E_OVERFLOW) {
+    c->last_path = PBIG_PATH_GENERAL;
+    c->stats.path = PBIG_PATH_GENERAL;
+    c->stats.bins_per_row = 0;
+    c->stats.cells_per_bin_side = 0;
+  }
+  return rc;
+}
+
 void set_cell_stats(pbig_hip_ctx* c, float cell) {
   // Record the cell side as an exact fraction when it is a dyadic rational.
   uint32_t den = 1;
@@ -199,6 +367,7 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device_id) != hipSuccess) { delete c; return PBIG_E_HIP; }
   c->n_cu = prop.multiProcessorCount;
+  if (pbig::binned_init() != hipSuccess) { delete c; return PBIG_E_HIP; }
   c->ws.test_blocks = (uint32_t)c->n_cu * 8;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PBIG_E_HIP; }
   if (hipMalloc((void**)&c->ws.meta, sizeof(Meta)) != hipSuccess) { delete c; return PBIG_E_NOMEM; }
@@ -217,8 +386,11 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Workspace& w = c->ws;
+  const BinnedWorkspace& b = c->bws;
   for (void* p : {(void*)w.cnt, (void*)w.start, (void*)w.partials, (void*)w.items, (void*)w.mbox,
-                  (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs})
+                  (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs, (void*)b.hist, (void*)b.off,
+                  (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bbox, (void*)b.bid,
+                  (void*)b.scratch, (void*)b.wg_count})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->h_meta) (void)hipHostFree(c->h_meta);
@@ -294,8 +466,9 @@ int pbig_hip_check_device(pbig_hip_ctx* c, const float* d_boxes, uint32_t n, dou
   uint32_t n_cells;
   int rc;
   if ((rc = grid_of(c, range, cell_size, &gp, &n_cells))) return rc;
+  rc = run_any_sync(c, (const float4*)d_boxes, n, range, cell_size, (uint2*)d_pairs, cap, s, n_pairs);
   set_cell_stats(c, cell_size);
-  return run_sync(c, (const float4*)d_boxes, n, gp, n_cells, (uint2*)d_pairs, cap, s, n_pairs);
+  return rc;
 }
 
 int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t n, double range,
@@ -312,10 +485,23 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
   uint32_t n_cells;
   int rc;
   if ((rc = grid_of(c, range, cell_size, &gp, &n_cells))) return rc;
-  if ((rc = prepare(c, n, n_cells))) return rc;
-  hipEvent_t* ev = nullptr;
-  if (c->timing_used < c->timing_cap) ev = &c->events[(size_t)c->timing_used++ * (pbig::kPlainKernels + 1)];
-  HIP_TRY(c, pbig::launch_plain(c->ws, (const float4*)d_boxes, n, gp, n_cells, (uint2*)d_pairs, cap, s, ev));
+  BinnedParams bp;
+  bool binned = c->cfg.path == PBIG_PATH_BINNED ||
+                (c->cfg.path == PBIG_PATH_AUTO && c->last_path != PBIG_PATH_GENERAL);
+  if (binned && !plan_binned(c, n, range, cell_size, &bp)) {
+    if (c->cfg.path == PBIG_PATH_BINNED)
+      return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
+    binned = false;
+  }
+  if (binned) {
+    if ((rc = prepare_binned(c, n, bp))) return rc;
+    HIP_TRY(c, pbig::launch_binned(c->bws, (const float4*)d_boxes, n, bp, (uint2*)d_pairs, cap, s,
+                                   next_events(c, PBIG_PATH_BINNED)));
+  } else {
+    if ((rc = prepare(c, n, n_cells))) return rc;
+    HIP_TRY(c, pbig::launch_plain(c->ws, (const float4*)d_boxes, n, gp, n_cells, (uint2*)d_pairs, cap, s,
+                                  next_events(c, PBIG_PATH_GENERAL)));
+  }
   if (d_result)
     HIP_TRY(c, hipMemcpyAsync(d_result, c->ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
   return PBIG_OK;
@@ -357,15 +543,15 @@ int pbig_hip_check_host(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double
   GridParams gp;
   uint32_t n_cells;
   if ((rc = grid_of(c, range, cell_size, &gp, &n_cells))) return rc;
-  set_cell_stats(c, cell_size);
   uint64_t want = std::max<uint64_t>(c->cap_pairs, std::max<uint64_t>(1024, (uint64_t)n));
   if ((rc = grow(c, &c->d_pairs, &c->cap_pairs, want))) return rc;
   uint64_t np = 0;
-  rc = run_sync(c, c->d_in, n, gp, n_cells, c->d_pairs, c->cap_pairs, s, &np);
+  rc = run_any_sync(c, c->d_in, n, range, cell_size, c->d_pairs, c->cap_pairs, s, &np);
   if (rc == PBIG_E_OVERFLOW) {
     if ((rc = grow(c, &c->d_pairs, &c->cap_pairs, np + np / 8 + 1024))) return rc;
-    rc = run_sync(c, c->d_in, n, gp, n_cells, c->d_pairs, c->cap_pairs, s, &np);
+    rc = run_any_sync(c, c->d_in, n, range, cell_size, c->d_pairs, c->cap_pairs, s, &np);
   }
+  set_cell_stats(c, cell_size);
   if (rc) return rc;
   *n_pairs = np;
   uint64_t ncopy = std::min<uint64_t>(np, cap);
@@ -383,29 +569,41 @@ int pbig_hip_timing_start(pbig_hip_ctx* c, int max_calls) {
   if (!c || max_calls < 0) return PBIG_E_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
   for (hipEvent_t e : c->events) HIP_TRY(c, hipEventDestroy(e));
-  c->events.assign((size_t)max_calls * (pbig::kPlainKernels + 1), nullptr);
+  c->events.assign((size_t)max_calls * kEventsPerCall, nullptr);
   for (auto& e : c->events) HIP_TRY(c, hipEventCreate(&e));
   c->timing_cap = max_calls;
   c->timing_used = 0;
+  c->timing_path = 0;
   return PBIG_OK;
 }
 
-int pbig_hip_timing_read(pbig_hip_ctx* c, double* ms_per_kernel, int n_kernels, int* n_calls) {
+int pbig_hip_timing_read(pbig_hip_ctx* c, double* ms_per_kernel, int n_kernels, int* n_calls, int* path) {
   if (!c || !ms_per_kernel || n_kernels < pbig::kPlainKernels || !n_calls) return PBIG_E_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
+  const int nk = c->timing_path == PBIG_PATH_BINNED ? pbig::kBinnedKernels : pbig::kPlainKernels;
   for (int k = 0; k < n_kernels; ++k) ms_per_kernel[k] = 0.0;
   for (int call = 0; call < c->timing_used; ++call) {
-    hipEvent_t* ev = &c->events[(size_t)call * (pbig::kPlainKernels + 1)];
-    HIP_TRY(c, hipEventSynchronize(ev[pbig::kPlainKernels]));
-    for (int k = 0; k < pbig::kPlainKernels; ++k) {
+    hipEvent_t* ev = &c->events[(size_t)call * kEventsPerCall];
+    HIP_TRY(c, hipEventSynchronize(ev[nk]));
+    for (int k = 0; k < nk; ++k) {
       float ms = 0.f;
       HIP_TRY(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       ms_per_kernel[k] += ms;
     }
   }
   *n_calls = c->timing_used;
+  if (path) *path = c->timing_path;
   c->timing_used = 0;
   c->timing_cap = 0;
+  return PBIG_OK;
+}
+
+int pbig_hip_configure(pbig_hip_ctx* c, const pbig_hip_config* cfg) {
+  if (!c || !cfg) return PBIG_E_ARG;
+  if (cfg->path < PBIG_PATH_AUTO || cfg->path > PBIG_PATH_BINNED) return fail(c, PBIG_E_ARG, "unknown path %d", cfg->path);
+  c->cfg = *cfg;
+  if (c->cfg.bin_target == 0) c->cfg.bin_target = 3000;
+  c->last_path = 0;
   return PBIG_OK;
 }
 

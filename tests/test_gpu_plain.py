@@ -21,11 +21,29 @@ def torch_cuda():
     return torch
 
 
-@pytest.fixture(scope="module")
-def eng(torch_cuda):
+PATHS = {"general": 1, "binned": 2, "auto": 0}
+
+
+@pytest.fixture(scope="module", params=["general", "auto"])
+def eng(torch_cuda, request):
+    """An engine per implementation: the single-level general path, and the
+    default (binned, falling back to general when a bin overflows LDS)."""
     import pbig_amd
 
     e = pbig_amd.Engine(0)
+    e.configure(PATHS[request.param])
+    e.path_name = request.param
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def binned(torch_cuda):
+    """An engine forced onto the binned path (errors instead of falling back)."""
+    import pbig_amd
+
+    e = pbig_amd.Engine(0)
+    e.configure(PATHS["binned"])
     yield e
     e.close()
 
@@ -42,7 +60,10 @@ def run_device(torch, eng, boxes, L, cs, cap=None):
     d = torch.from_numpy(np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)).to("cuda:0")
     cap = cap if cap is not None else max(1024, 4 * boxes.shape[0])
     out = torch.empty((cap, 2), dtype=torch.int32, device="cuda:0")
-    n = eng.check_device(d, L, cs, out)
+    n = eng.check_device(d, L, cs, out, allow_overflow=True)
+    if n > cap:  # rerun with room for every pair
+        out = torch.empty((n, 2), dtype=torch.int32, device="cuda:0")
+        assert eng.check_device(d, L, cs, out) == n
     return oracle.sort_pairs(out[:n].cpu().numpy().view(np.uint32))
 
 
@@ -58,8 +79,9 @@ def test_c1_matches_golden_and_oracle(torch_cuda, eng, orc, manifest):
     assert np.array_equal(got, golden)
     assert orc.pair_hash(got) == m["sha256"]
     st = eng.stats()
-    assert st["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
-    assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
+    if st["path"] == 1:
+        assert st["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
+        assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
     # cell_size = 0 selects the reference rule on the device boxes
     got0 = run_device(torch_cuda, eng, boxes, L, 0)
     assert np.array_equal(got0, golden)
@@ -84,7 +106,8 @@ def test_c2_full_size(torch_cuda, eng, orc, manifest):
     got = run_device(torch_cuda, eng, boxes, L, m["reference_cell_size"], cap=4 * m["pairs"])
     assert len(got) == m["pairs"]
     assert orc.pair_hash(got) == m["sha256"]
-    assert eng.stats()["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
+    if eng.stats()["path"] == 1:
+        assert eng.stats()["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
 
 
 def test_c3_full_size(torch_cuda, eng, orc, manifest):
@@ -97,7 +120,8 @@ def test_c3_full_size(torch_cuda, eng, orc, manifest):
     assert len(got) == m["pairs"]
     assert orc.pair_hash(got) == m["sha256"]
     st = eng.stats()
-    assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
+    if st["path"] == 1:
+        assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
 
 
 def edge_cases():
@@ -221,3 +245,68 @@ def test_repeated_calls_shrinking_and_growing(torch_cuda, eng, orc):
         got = run_device(torch_cuda, eng, boxes, L, cs)
         want = orc.sort_pairs(orc.grid_pairs(boxes, L, cs))
         assert np.array_equal(got, want), n
+
+
+# ---------------------------------------------------------------------------
+# Binned path forced on: it must either produce the exact set or refuse.
+
+@pytest.mark.parametrize("cs", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("bin_target", [300, 1500, 3000])
+def test_binned_c1_cell_sizes_and_bin_sizes(torch_cuda, binned, orc, cs, bin_target):
+    import pbig_amd
+    from pbig_amd import workloads
+
+    binned.configure(PATHS["binned"], bin_target)
+    boxes, L = workloads.c1_small_uniform()
+    try:
+        got = run_device(torch_cuda, binned, boxes, L, cs)
+    except pbig_amd.PbigError as e:
+        # too dense for LDS, or more coarse bins than the partition supports
+        assert e.status in (pbig_amd.PBIG_E_TOO_DENSE, pbig_amd.PBIG_E_ARG)
+        return
+    assert binned.stats()["path"] == PATHS["binned"]
+    assert np.array_equal(got, np.load(os.path.join(GOLDEN, "c1_pairs.npy")))
+
+
+@pytest.mark.parametrize("cs", [32, 64, 128])
+def test_binned_c2_full_size(torch_cuda, binned, orc, manifest, cs):
+    from pbig_amd import workloads
+
+    binned.configure(PATHS["binned"], 3000)
+    boxes, L = workloads.c2_large_uniform()
+    m = manifest["c2_uniform_4m"]
+    got = run_device(torch_cuda, binned, boxes, L, cs, cap=4 * m["pairs"])
+    assert binned.stats()["path"] == PATHS["binned"]
+    assert len(got) == m["pairs"]
+    assert orc.pair_hash(got) == m["sha256"]
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_binned_edge_cases(torch_cuda, binned, orc, name):
+    import pbig_amd
+
+    binned.configure(PATHS["binned"], 3000)
+    boxes, L, cs = EDGE[name]
+    if not len(boxes):
+        return
+    want = orc.sort_pairs(orc.brute_pairs(boxes))
+    try:
+        got = run_device(torch_cuda, binned, boxes, L, cs)
+    except pbig_amd.PbigError as e:
+        assert e.status == pbig_amd.PBIG_E_TOO_DENSE, name
+        return
+    assert np.array_equal(got, want), (name, len(got), len(want))
+
+
+def test_auto_falls_back_on_dense_bins(torch_cuda, orc):
+    """6000 boxes in one fine cell overflow the binned path's LDS buffer:
+    AUTO must still return the exact set through the general path."""
+    import pbig_amd
+
+    rng = np.random.default_rng(11)
+    c = rng.uniform(10, 12, size=(6000, 2)).astype(np.float32)
+    boxes = np.stack([c[:, 0], c[:, 0] + 0.5, c[:, 1], c[:, 1] + 0.5], 1).astype(np.float32)
+    with pbig_amd.Engine(0) as e:
+        got = run_device(torch_cuda, e, boxes, 64.0, 16)
+        assert e.stats()["path"] == PATHS["general"]
+    assert np.array_equal(got, orc.sort_pairs(orc.brute_pairs(boxes)))
