@@ -115,7 +115,7 @@ def main():
     ap.add_argument("--cell-size", type=float, default=64.0,
                     help="fine cell side; 0 = the reference rule (256 on this instance)")
     ap.add_argument("--path", choices=("auto", "general", "binned"), default="auto")
-    ap.add_argument("--bin-target", type=int, default=3000)
+    ap.add_argument("--bin-target", type=int, default=2000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (torch) three times, a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")

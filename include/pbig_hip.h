@@ -63,7 +63,7 @@ enum { PBIG_PATH_AUTO = 0, PBIG_PATH_GENERAL = 1, PBIG_PATH_BINNED = 2 };
 
 typedef struct pbig_hip_config {
   int32_t path;          /* PBIG_PATH_* */
-  uint32_t bin_target;   /* binned path: boxes per coarse bin to aim for (0 = 3000) */
+  uint32_t bin_target;   /* binned path: boxes per coarse bin to aim for (0 = 2000) */
   uint32_t reserved0, reserved1;
 } pbig_hip_config;
 
@@ -134,10 +134,72 @@ int pbig_hip_configure(pbig_hip_ctx* ctx, const pbig_hip_config* cfg);
  * plain path's kernels for the next max_calls asynchronous calls;
  * pbig_hip_timing_read waits for them and returns the summed milliseconds per
  * kernel in launch order, the number of calls recorded and the path they
- * took (n_kernels >= 6).  GENERAL: clear, count, scan_reduce, scan_apply,
- * scatter, test.  BINNED: hist, scan, scatter, test, compact. */
+ * took (n_kernels >= 7).  GENERAL: clear, count, scan_reduce, scan_apply,
+ * scatter, test.  BINNED: hist, scan, starts, scatter, test, pair_scan,
+ * compact.  Batch launches are timed the same way (path BINNED). */
 int pbig_hip_timing_start(pbig_hip_ctx* ctx, int max_calls);
 int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls, int* path);
+
+/* ------------------------------------------------------------------------
+ * Batches: many independent instances in one launch sequence.
+ *
+ * The reference checks one instance per pbig_check call (src/pbig/pbig.h:10;
+ * its example driver loops over inputs, package/main.cc:22-40).  A batch plans a
+ * fixed set of instances once; every launch then runs all of them through
+ * the binned path's seven kernels, whatever the number of instances.  The
+ * instances' boxes are concatenated in one device array; instance i's boxes
+ * are d_boxes[box_offset .. box_offset + n_boxes).  Pair ids are indices
+ * inside the instance.  The pairs of instance i occupy
+ * d_pairs[offset_i .. offset_i + count_i), instances in order.
+ * ------------------------------------------------------------------------ */
+typedef struct pbig_hip_instance {
+  uint64_t box_offset;   /* first box of the instance in the batch's box array */
+  uint32_t n_boxes;      /* < 2^30 */
+  float cell_size;       /* fine cell side, > 0 */
+  double range;          /* domain side (as for pbig_hip_check_device) */
+} pbig_hip_instance;
+
+typedef struct pbig_hip_batch pbig_hip_batch;
+
+/* Plan a batch of n_inst instances on ctx's device (tables uploaded, first
+ * workspace allocated).  The batch keeps a pointer to ctx: destroy the batch
+ * first.  Calls on a batch and on its context must not overlap. */
+int pbig_hip_batch_create(pbig_hip_ctx* ctx, const pbig_hip_instance* inst, uint32_t n_inst,
+                          pbig_hip_batch** batch);
+int pbig_hip_batch_destroy(pbig_hip_batch* batch);
+
+/* Synchronous batch check.  d_boxes holds n_boxes_total boxes (every
+ * instance's range must lie inside).  h_inst_pairs (optional, host) receives
+ * n_inst x {offset, count}.  Regrows the workspace when needed; an instance
+ * whose boxes crowd a coarse bin beyond LDS runs on the general path and its
+ * pairs are placed in its slot.  *n_pairs = total; PBIG_E_OVERFLOW when it
+ * exceeds pair_capacity (offsets and counts are still returned). */
+int pbig_hip_batch_check(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
+                         uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
+                         uint64_t* n_pairs, void* stream);
+
+/* Asynchronous batch launch: enqueues the seven kernels and returns; no host
+ * synchronisation, no allocation, no memset, so it may be captured in a HIP
+ * graph.  d_inst_pairs (optional, device) receives n_inst x {offset, count};
+ * d_result (optional, device, 2 x uint64) the total and the error word of
+ * pbig_hip_check_device_async.  When the error word is nonzero, run
+ * pbig_hip_batch_check once (it regrows or falls back) and launch again. */
+int pbig_hip_batch_launch(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
+                          uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* d_inst_pairs,
+                          uint64_t* d_result, void* stream);
+
+/* Capture pbig_hip_batch_launch with these arguments into a HIP graph owned
+ * by the batch (replacing an earlier one), and replay it.  Replay returns
+ * PBIG_E_ARG when the workspace has moved since the capture (after a
+ * pbig_hip_batch_check that regrew it): capture again. */
+int pbig_hip_batch_capture(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
+                           uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* d_inst_pairs,
+                           uint64_t* d_result);
+int pbig_hip_batch_replay(pbig_hip_batch* batch, void* stream);
+
+/* Totals of a batch's plan: chunks, coarse bins, boxes (no counterpart in the
+ * reference). */
+int pbig_hip_batch_info(const pbig_hip_batch* batch, uint64_t* n_boxes, uint32_t* n_chunks, uint32_t* n_bins);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,13 @@ EXPORTS = (
     "pbig_hip_timing_start",
     "pbig_hip_timing_read",
     "pbig_hip_configure",
+    "pbig_hip_batch_create",
+    "pbig_hip_batch_destroy",
+    "pbig_hip_batch_check",
+    "pbig_hip_batch_launch",
+    "pbig_hip_batch_capture",
+    "pbig_hip_batch_replay",
+    "pbig_hip_batch_info",
 )
 
 PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
@@ -48,7 +55,8 @@ PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned"}
 # Kernels of each path in launch order (pbig_hip_timing_read).
 KERNELS = {
     PATH_GENERAL: ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test"),
-    PATH_BINNED: ("bin_hist", "bin_scan", "bin_scatter", "bin_test", "bin_compact"),
+    PATH_BINNED: ("bin_hist", "bin_scan", "bin_starts", "bin_scatter", "bin_test", "bin_pair_scan",
+                  "bin_compact"),
 }
 
 # numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)
@@ -84,6 +92,12 @@ class Stats(ctypes.Structure):
         d = {name: int(getattr(self, name)) for name, _ in self._fields_ if not name.startswith("reserved")}
         d["cell_size"] = d["cell_size_num"] / max(d["cell_size_den"], 1)
         return d
+
+
+class Instance(ctypes.Structure):
+    """pbig_hip_instance: one instance of a batch."""
+    _fields_ = [("box_offset", ctypes.c_uint64), ("n_boxes", ctypes.c_uint32),
+                ("cell_size", ctypes.c_float), ("range", ctypes.c_double)]
 
 
 class Config(ctypes.Structure):
@@ -123,6 +137,13 @@ def load_library(path: str | None = None):
         "pbig_hip_timing_read": (i32, [P, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32),
                                        ctypes.POINTER(i32)]),
         "pbig_hip_configure": (i32, [P, ctypes.POINTER(Config)]),
+        "pbig_hip_batch_create": (i32, [P, ctypes.POINTER(Instance), u32, ctypes.POINTER(P)]),
+        "pbig_hip_batch_destroy": (i32, [P]),
+        "pbig_hip_batch_check": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
+        "pbig_hip_batch_launch": (i32, [P, P, u64, P, u64, P, P, P]),
+        "pbig_hip_batch_capture": (i32, [P, P, u64, P, u64, P, P]),
+        "pbig_hip_batch_replay": (i32, [P, P]),
+        "pbig_hip_batch_info": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -276,3 +297,102 @@ def as_rects(boxes) -> np.ndarray:
     r["xmin"], r["xmax"], r["ymin"], r["ymax"] = b[:, 0], b[:, 1], b[:, 2], b[:, 3]
     r["rid"] = np.arange(b.shape[0], dtype=np.uint32)
     return r
+
+
+class Batch:
+    """A planned set of independent instances checked by one launch sequence
+    (pbig_hip_batch_*).  ``offsets``: (k+1,) box offsets of the instances in
+    the concatenated box array; ``ranges``, ``cell_sizes``: per instance."""
+
+    def __init__(self, engine: Engine, offsets, ranges, cell_sizes):
+        offsets = np.asarray(offsets, dtype=np.int64)
+        self._create(engine, offsets[:-1], np.diff(offsets), ranges, cell_sizes)
+
+    @classmethod
+    def from_instances(cls, engine: Engine, starts, counts, ranges, cell_sizes) -> "Batch":
+        """Instance i = boxes [starts[i], starts[i] + counts[i]) of the box array."""
+        b = cls.__new__(cls)
+        b._create(engine, np.asarray(starts, dtype=np.int64), np.asarray(counts, dtype=np.int64), ranges,
+                  cell_sizes)
+        return b
+
+    def _create(self, engine, starts, counts, ranges, cell_sizes):
+        self._eng = engine
+        self._lib = engine._lib
+        k = starts.shape[0]
+        ranges = np.broadcast_to(np.asarray(ranges, dtype=np.float64), (k,))
+        cells = np.broadcast_to(np.asarray(cell_sizes, dtype=np.float32), (k,))
+        arr = (Instance * max(k, 1))()
+        for i in range(k):
+            arr[i].box_offset = int(starts[i])
+            arr[i].n_boxes = int(counts[i])
+            arr[i].cell_size = float(cells[i])
+            arr[i].range = float(ranges[i])
+        h = ctypes.c_void_p()
+        self._h = None
+        rc = self._lib.pbig_hip_batch_create(engine._h, arr, k, ctypes.byref(h))
+        if rc != PBIG_OK:
+            engine._raise(rc)
+        self._h = h
+        self.n_inst = k
+        self.n_boxes_total = int(np.max(starts + counts)) if k else 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pbig_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        nb, nc, nbin = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._lib.pbig_hip_batch_info(self._h, ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(nbin))
+        return {"n_boxes": int(nb.value), "n_chunks": int(nc.value), "n_bins": int(nbin.value)}
+
+    def check(self, boxes, pairs, allow_overflow: bool = False, stream=None):
+        """Synchronous.  Returns (total pairs, (k, 2) uint64 numpy array of
+        per-instance (offset, count))."""
+        cap = int(pairs.shape[0]) if pairs is not None else 0
+        io = np.zeros((max(self.n_inst, 1), 2), dtype=np.uint64)
+        out = ctypes.c_uint64(0)
+        rc = self._lib.pbig_hip_batch_check(self._h, _ptr(boxes) if self.n_boxes_total else None,
+                                            int(boxes.shape[0]), _ptr(pairs) if cap else None, cap,
+                                            io.ctypes.data, ctypes.byref(out), _stream_ptr(stream))
+        if rc == PBIG_E_OVERFLOW and allow_overflow:
+            return int(out.value), io[: self.n_inst]
+        if rc != PBIG_OK:
+            self._eng._raise(rc)
+        return int(out.value), io[: self.n_inst]
+
+    def launch(self, boxes, pairs, inst_pairs=None, result=None, stream=None):
+        """Enqueue only.  inst_pairs: (k, 2) int64 device tensor; result: (2,)
+        int64 device tensor <- {total, error}."""
+        rc = self._lib.pbig_hip_batch_launch(self._h, _ptr(boxes) if self.n_boxes_total else None,
+                                             int(boxes.shape[0]), _ptr(pairs), int(pairs.shape[0]),
+                                             _ptr(inst_pairs) if inst_pairs is not None else None,
+                                             _ptr(result) if result is not None else None, _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._eng._raise(rc)
+
+    def capture(self, boxes, pairs, inst_pairs=None, result=None):
+        rc = self._lib.pbig_hip_batch_capture(self._h, _ptr(boxes) if self.n_boxes_total else None,
+                                              int(boxes.shape[0]), _ptr(pairs), int(pairs.shape[0]),
+                                              _ptr(inst_pairs) if inst_pairs is not None else None,
+                                              _ptr(result) if result is not None else None)
+        if rc != PBIG_OK:
+            self._eng._raise(rc)
+
+    def replay(self, stream=None):
+        rc = self._lib.pbig_hip_batch_replay(self._h, _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._eng._raise(rc)

@@ -16,44 +16,72 @@ constexpr uint32_t kMaxCellsPerBin = 4096;   // k*k fine cells per coarse bin (k
 constexpr uint32_t kMaxK = 64;
 constexpr uint32_t kBinStage = 128;          // staged pairs per wave
 constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent test workgroup
-constexpr uint32_t kBatch = 8;               // boxes per thread loaded before use (hist, scatter)
+constexpr uint32_t kBatch = 16;              // boxes per thread loaded before use (hist)
+constexpr uint32_t kScatterChunk = 8192;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
 
-struct BinnedParams {
-  float inv_f;          // 1 / fine cell side (rounded once on the host)
-  uint32_t nf;          // fine cells per row = nbx * k
-  uint32_t k;           // fine cells per coarse bin side (1..32)
-  uint32_t nbx;         // coarse bins per row
-  uint32_t nb;          // coarse bins = nbx * nbx
-  uint32_t nw;          // workgroups of the histogram and scatter kernels (multiple of 8)
-  uint32_t chunk;       // boxes per such workgroup
-  uint32_t test_blocks; // persistent workgroups of the test kernel
-  uint64_t k_magic;     // ceil(2^32 / k): fine index -> coarse bin by multiply
+// Grid and work layout of one instance (device table, one entry per instance).
+struct InstDesc {
+  uint64_t box0;     // first box of the instance in the box array
+  uint64_t hist0;    // first entry of its histogram rows (nchunks x (nb + 1))
+  uint64_t off0;     // first entry of its offset rows (nchunks x nb)
+  uint64_t k_magic;  // ceil(2^32 / k): fine index -> coarse bin by multiply
+  uint64_t pairs0;   // pairs of earlier instances found outside the launch
+  uint64_t pairs_extra;  // this instance's pairs found outside the launch
+  uint32_t n;        // boxes
+  uint32_t chunk;    // boxes per histogram/scatter workgroup
+  uint32_t chunk0;   // first global chunk
+  uint32_t nchunks;
+  uint32_t bin0;     // first global coarse bin
+  uint32_t nb;       // coarse bins = nbx * nbx
+  uint32_t nbx;      // coarse bins per row
+  uint32_t k;        // fine cells per coarse bin side (1..kMaxK)
+  uint32_t nf;       // fine cells per row = nbx * k
+  float inv_f;       // 1 / fine cell side (rounded once on the host)
+  uint32_t pad;
+};
+static_assert(sizeof(InstDesc) == 96, "InstDesc layout");
+
+// Totals of a batch (host side).
+struct BatchPlan {
+  uint32_t n_inst = 0, n_chunks = 0, n_bins = 0, test_blocks = 0;
+  uint64_t n_boxes = 0, hist_entries = 0, off_entries = 0;
+  uint64_t extra_total = 0;  // pairs found outside the launch (sum of pairs_extra)
 };
 
 struct BinnedWorkspace {
-  uint32_t* hist = nullptr;       // nw x (nb + 1): per-workgroup bin counts + error word
-  uint32_t* off = nullptr;        // nw x nb: workgroup offset inside each bin
-  uint32_t* total = nullptr;      // nb: memberships per bin
-  uint32_t* bin_start = nullptr;  // nb: exclusive scan of total
-  uint32_t* overfull = nullptr;   // nb: bins that did not fit in LDS
+  InstDesc* inst = nullptr;       // per instance
+  uint32_t* chunk_inst = nullptr; // instance of each chunk
+  uint32_t* bin_inst = nullptr;   // instance of each coarse bin
+  uint32_t* hist = nullptr;       // per chunk: bin counts + error word
+  uint32_t* off = nullptr;        // per chunk: offset inside each bin
+  uint32_t* total = nullptr;      // per bin: memberships
+  uint32_t* bin_start = nullptr;  // per bin: exclusive scan of total
+  uint32_t* overfull = nullptr;   // bins that did not fit in LDS
+  uint2* bin_seg = nullptr;       // per bin: (first pair in its workgroup's region, pairs)
+  uint64_t* bin_out = nullptr;    // per bin + 1: output position of its pairs
   uint2* scratch = nullptr;       // test_blocks x cap_wg pair regions
-  uint32_t* wg_count = nullptr;   // pairs found by each test workgroup
   uint32_t cap_wg = 0;
-  uint64_t cap_scratch = 0, cap_wgc = 0;
+  uint64_t cap_scratch = 0;
   float4* bbox = nullptr;         // bin-ordered box copies
-  uint32_t* bid = nullptr;        // bin-ordered box ids
+  uint32_t* bid = nullptr;        // bin-ordered box ids (inside their instance)
   Meta* meta = nullptr;
   uint64_t cap_hist = 0, cap_off = 0, cap_bins = 0, cap_bucket = 0;
+  uint64_t cap_inst = 0, cap_chunks = 0, cap_bin_inst = 0;
 };
 
-constexpr int kBinnedKernels = 5;  // hist, scan, scatter, test, compact
+// hist, scan, starts, scatter, test, pair_scan, compact
+constexpr int kBinnedKernels = 7;
 
 size_t binned_test_lds_bytes();
 hipError_t binned_init();
 
-hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, uint32_t n, const BinnedParams& p,
-                         uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* events = nullptr);
+// Enqueue the binned pipeline over the batch described by the workspace's
+// tables (uploaded beforehand).  Pairs land in `out` instance by instance;
+// inst_out (optional, device) receives (offset, count) per instance.
+// events (optional): kBinnedKernels + 1.
+hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, uint2* out,
+                         uint64_t cap, uint64_t* inst_out, hipStream_t s, hipEvent_t* events = nullptr);
 
 }  // namespace pbig
 

@@ -1,36 +1,41 @@
-// Binned plain path for gfx950: a two-level grid whose fine cells are binned,
-// tested and compacted inside LDS, one workgroup per coarse bin.
+// Binned path for gfx950: a two-level grid whose fine cells are binned,
+// tested and compacted inside LDS.  One launch sequence serves a batch of
+// independent instances (a single instance is a batch of one): every
+// histogram/scatter chunk and every coarse bin belongs to one instance, and
+// the kernels read that instance's grid from a small device table.
 //
 // The reference bins every box straight into its cells with one global atomic
 // per membership (src/pbig/pbig.cu:174-221 count, 321-427 scatter).  On
 // MI355X a device-wide atomic costs ~37 ps of whole-GPU throughput (about
-// 27 G/s), so two atomics per membership would dominate the path.  Here
-// global memory sees no per-membership atomics:
+// 27 G/s) and same-address atomics serialise at ~12 ns, so here global memory
+// sees no per-membership atomics:
 //
-//   k_bin_hist     each workgroup owns a contiguous chunk of boxes and counts
-//                  its coarse-bin memberships in an LDS histogram, written as
-//                  one row of hist[workgroup][bin]
-//   k_bin_scan     per bin, exclusive scan of the rows (workgroups ordered
-//                  XCD-major, so runs written from one XCD sit side by side)
-//   k_bin_scatter  re-reads its chunk, ranks memberships with LDS atomics and
-//                  writes each coarse bin's boxes as contiguous runs
-//   k_bin_test     one workgroup per coarse bin: fine-cell count, scan and
-//                  scatter in LDS; all-pairs test of every fine cell with the
-//                  reference-cell rule (pbig.cu:562-603); per-wave LDS staging
-//                  of the pairs, one global atomic per flush of 128 pairs
+//   k_bin_hist     each workgroup owns a chunk of one instance's boxes and
+//                  counts its coarse-bin memberships in an LDS histogram,
+//                  written as one row of that instance's hist[chunk][bin]
+//   k_bin_scan     per bin, exclusive scan over the instance's chunks
+//   k_bin_starts   exclusive scan of all bins' totals (bucket offsets)
+//   k_bin_scatter  the chunk is held in registers and emitted bin range by
+//                  bin range through an LDS stage, so stores cover whole runs
+//                  and every XCD writes the same bins at the same time
+//   k_bin_test     persistent workgroups over all bins: fine-cell count, scan
+//                  and scatter in LDS (in strips that fit), all-pairs test of
+//                  every fine cell with the reference-cell rule
+//                  (pbig.cu:562-603); pairs appended through an LDS counter
+//                  to the workgroup's own region
+//   k_bin_compact  concatenates the regions
 //
 // Cell maps.  The fine cell of a coordinate is clamp(floor(v * inv_f), 0,
-// nf-1) with nf = nbx * K fine cells per row and inv_f rounded once on the
-// host; the coarse bin is the fine index >> log2(K).  Both maps are monotone,
-// so the reference-cell rule reduces to per-membership "starts in this
-// column/row" flags exactly as in the single-level kernels.
+// nf-1) with nf = nbx * k fine cells per row and inv_f rounded once on the
+// host; the coarse bin is the fine index / k.  Both maps are monotone, so the
+// reference-cell rule reduces to per-membership "starts in this column/row"
+// flags.
 //
-// Per-cell test.  Cells here hold a handful of boxes, so a lane owns one
-// membership r of a cell with n members and tests it against members
-// (r + k) mod n for k = 1..h(r): h = n/2 (r < n/2) or n/2 - 1 (r >= n/2) for
-// even n, (n-1)/2 for odd n.  Every unordered pair is tested exactly once;
-// this is the reference's BALANCE schedule for the diagonal tile
-// (pbig.cu:746-755, 802-809) applied to whole cells.
+// Per-cell test.  A lane owns one membership r of a cell with n members and
+// tests it against members (r + k) mod n for k = 1..h(r): h = n/2 (r < n/2)
+// or n/2 - 1 (r >= n/2) for even n, (n-1)/2 for odd n.  Every unordered pair
+// is tested exactly once; this is the reference's BALANCE schedule for the
+// diagonal tile (pbig.cu:746-755, 802-809) applied to whole cells.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -56,7 +61,7 @@ __device__ __forceinline__ uint32_t div_k(uint32_t fine, uint64_t magic) {
   return (uint32_t)(((uint64_t)fine * magic) >> 32);
 }
 
-__device__ __forceinline__ BinRange bin_range(const float4& b, const BinnedParams& p) {
+__device__ __forceinline__ BinRange bin_range(const float4& b, const InstDesc& p) {
   BinRange r;
   r.x0 = div_k(fine_of(b.x, p.inv_f, p.nf), p.k_magic);
   r.x1 = div_k(fine_of(b.y, p.inv_f, p.nf), p.k_magic);
@@ -65,82 +70,100 @@ __device__ __forceinline__ BinRange bin_range(const float4& b, const BinnedParam
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ boxes, uint32_t n,
-                                                    BinnedParams p, uint32_t* __restrict__ hist) {
+// Box range [start, end) of global chunk w.
+__device__ __forceinline__ void chunk_range(const InstDesc& I, uint32_t w, uint64_t* start, uint64_t* end) {
+  const uint64_t lc = w - I.chunk0;
+  const uint64_t inst_end = I.box0 + I.n;
+  *start = I.box0 + lc * I.chunk;
+  *end = *start + I.chunk < inst_end ? *start + I.chunk : inst_end;
+}
+
+__global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ boxes,
+                                                    const InstDesc* __restrict__ inst,
+                                                    const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
+                                                    uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[kMaxBins];
   __shared__ uint32_t err;
-  for (uint32_t b = threadIdx.x; b < p.nb; b += blockDim.x) h[b] = 0u;
+  if (blockIdx.x >= n_chunks) return;
+  const InstDesc I = inst[chunk_inst[blockIdx.x]];
+  for (uint32_t b = threadIdx.x; b < I.nb; b += blockDim.x) h[b] = 0u;
   if (threadIdx.x == 0) err = 0u;
   __syncthreads();
-  const uint32_t start = blockIdx.x * p.chunk;
-  const uint32_t end = start + p.chunk < n ? start + p.chunk : n;
+  uint64_t start, end;
+  chunk_range(I, blockIdx.x, &start, &end);
   // Issue kBatch independent loads per thread before using any of them.
-  for (uint32_t i0 = start; i0 < end; i0 += kBatch * blockDim.x) {
+  for (uint64_t i0 = start; i0 < end; i0 += kBatch * blockDim.x) {
     float4 q[kBatch];
 #pragma unroll
     for (uint32_t k = 0; k < kBatch; ++k) {
-      const uint32_t i = i0 + threadIdx.x + k * blockDim.x;
+      const uint64_t i = i0 + threadIdx.x + k * blockDim.x;
       q[k] = i < end ? boxes[i] : make_float4(1.f, 0.f, 1.f, 0.f);
     }
 #pragma unroll
     for (uint32_t k = 0; k < kBatch; ++k) {
-      const uint32_t i = i0 + threadIdx.x + k * blockDim.x;
+      const uint64_t i = i0 + threadIdx.x + k * blockDim.x;
       if (i >= end) continue;
       const float4 b = q[k];
       if (!((b.x <= b.y) && (b.z <= b.w))) {
         err = 1u;
         continue;
       }
-      const BinRange r = bin_range(b, p);
+      const BinRange r = bin_range(b, I);
       for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-        for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * p.nbx + cx], 1u);
+        for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * I.nbx + cx], 1u);
     }
   }
   __syncthreads();
-  uint32_t* row = hist + (size_t)blockIdx.x * (p.nb + 1);
-  for (uint32_t b = threadIdx.x; b < p.nb; b += blockDim.x) row[b] = h[b];
-  if (threadIdx.x == 0) row[p.nb] = err;
+  uint32_t* row = hist + I.hist0 + (uint64_t)(blockIdx.x - I.chunk0) * (I.nb + 1);
+  for (uint32_t b = threadIdx.x; b < I.nb; b += blockDim.x) row[b] = h[b];
+  if (threadIdx.x == 0) row[I.nb] = err;
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup w' in XCD-major order is hardware workgroup w = (w' % q) * 8 + w' / q
-// with q = nw / 8 (workgroup w runs on XCD w % 8).
-__device__ __forceinline__ uint32_t xcd_major(uint32_t wp, uint32_t q) { return (wp % q) * 8u + wp / q; }
-
-__global__ __launch_bounds__(256) void k_bin_scan(const uint32_t* __restrict__ hist, BinnedParams p,
+__global__ __launch_bounds__(256) void k_bin_scan(const uint32_t* __restrict__ hist,
+                                                   const InstDesc* __restrict__ inst,
+                                                   const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
+                                                   const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
                                                    uint32_t* __restrict__ off, uint32_t* __restrict__ total,
                                                    Meta* __restrict__ meta) {
-  // Thread t handles bin b0 + (t % 16) over row group t / 16 (rows in
-  // XCD-major order).
+  // Thread t handles global bin blockIdx.x * 16 + (t % 16) over row group t / 16.
   __shared__ uint32_t part[16][16];
   const uint32_t bl = threadIdx.x & 15, g = threadIdx.x >> 4;
-  const uint32_t bin = blockIdx.x * 16 + bl;
-  const uint32_t q = p.nw / 8u;
-  const uint32_t per = (p.nw + 15) / 16;
-  const uint32_t w0 = g * per, w1 = (w0 + per < p.nw) ? w0 + per : p.nw;
-  uint32_t s = 0;
-  if (bin < p.nb) {
+  const uint32_t gb = blockIdx.x * 16 + bl;
+  uint32_t s = 0, b = 0, nb = 0, w0 = 0, w1 = 0;
+  uint64_t hist0 = 0, off0 = 0;
+  if (gb < n_bins) {
+    const InstDesc I = inst[bin_inst[gb]];
+    b = gb - I.bin0;
+    nb = I.nb;
+    hist0 = I.hist0;
+    off0 = I.off0;
+    const uint32_t per = (I.nchunks + 15) / 16;
+    w0 = g * per;
+    w1 = w0 + per < I.nchunks ? w0 + per : I.nchunks;
 #pragma unroll 8
-    for (uint32_t wp = w0; wp < w1; ++wp) s += hist[(size_t)xcd_major(wp, q) * (p.nb + 1) + bin];
+    for (uint32_t c = w0; c < w1; ++c) s += hist[hist0 + (uint64_t)c * (nb + 1) + b];
   }
   part[g][bl] = s;
   __syncthreads();
   uint32_t base = 0;
   for (uint32_t k = 0; k < g; ++k) base += part[k][bl];
-  if (bin < p.nb) {
+  if (gb < n_bins) {
 #pragma unroll 8
-    for (uint32_t wp = w0; wp < w1; ++wp) {
-      const size_t w = xcd_major(wp, q);
-      const uint32_t v = hist[w * (p.nb + 1) + bin];
-      off[w * p.nb + bin] = base;
+    for (uint32_t c = w0; c < w1; ++c) {
+      const uint32_t v = hist[hist0 + (uint64_t)c * (nb + 1) + b];
+      off[off0 + (uint64_t)c * nb + b] = base;
       base += v;
     }
-    if (g == 15) total[bin] = base;
+    if (g == 15) total[gb] = base;
   }
   if (blockIdx.x == 0) {
     // Result words of the call: pair counter and the boxes' validity.
     uint32_t e = 0;
-    for (uint32_t w = threadIdx.x; w < p.nw; w += blockDim.x) e |= hist[(size_t)w * (p.nb + 1) + p.nb];
+    for (uint32_t w = threadIdx.x; w < n_chunks; w += blockDim.x) {
+      const InstDesc I = inst[chunk_inst[w]];
+      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * (I.nb + 1) + I.nb];
+    }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
     __shared__ uint32_t es[4];
@@ -155,84 +178,218 @@ __global__ __launch_bounds__(256) void k_bin_scan(const uint32_t* __restrict__ h
   }
 }
 
-// ---------------------------------------------------------------------------
-// Exclusive scan of total[0..nb) into s[] by one workgroup (blockDim a
-// multiple of 64, at most 1024 threads); returns the grand total.
-__device__ uint64_t block_exscan_bins(const uint32_t* __restrict__ total, uint32_t nb, uint32_t* s) {
+// Exclusive scan of all bins' totals by one 1024-thread workgroup: bucket
+// offset of every bin, the grand total and the workspace check.
+__global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
+                                                     uint32_t* __restrict__ bin_start, uint64_t cap,
+                                                     Meta* __restrict__ meta) {
   __shared__ uint64_t wsum[16];
-  const uint32_t nt = blockDim.x, nwv = nt >> 6;
-  const uint32_t per = (nb + nt - 1) / nt;
+  const uint32_t per = (n_bins + 1023) / 1024;
   const uint32_t b0 = threadIdx.x * per;
   uint64_t acc = 0;
-  for (uint32_t k = 0; k < per; ++k) {
-    const uint32_t b = b0 + k;
-    if (b < nb) acc += total[b];
-  }
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < n_bins) acc += total[b0 + k];
   uint64_t v = acc;
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    uint64_t o = __shfl_up(v, d, 64);
+    const uint64_t o = __shfl_up(v, d, 64);
     if (lane >= (uint32_t)d) v += o;
   }
   if (lane == 63) wsum[w] = v;
   __syncthreads();
   uint64_t ex = v - acc, grand = 0;
-  for (uint32_t k = 0; k < nwv; ++k) {
+  for (uint32_t k = 0; k < 16; ++k) {
     if (k < w) ex += wsum[k];
     grand += wsum[k];
   }
   for (uint32_t k = 0; k < per; ++k) {
     const uint32_t b = b0 + k;
-    if (b < nb) {
-      s[b] = (uint32_t)ex;
+    if (b < n_bins) {
+      bin_start[b] = (uint32_t)(ex < cap ? ex : cap);
       ex += total[b];
     }
   }
-  __syncthreads();
-  return grand;
+  if (threadIdx.x == 0) {
+    meta->memberships = grand;
+    if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
+  }
 }
 
-__global__ __launch_bounds__(1024) void k_bin_scatter(const float4* __restrict__ boxes, uint32_t n, BinnedParams p,
-                                                       const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ total,
-                                                       uint32_t* __restrict__ bin_start,
-                                                       float4* __restrict__ bbox, uint32_t* __restrict__ bid,
-                                                       uint64_t cap, Meta* __restrict__ meta) {
-  __shared__ uint32_t cur[kMaxBins];
-  const uint64_t grand = block_exscan_bins(total, p.nb, cur);
-  if (blockIdx.x == 0) {
-    for (uint32_t b = threadIdx.x; b < p.nb; b += blockDim.x) bin_start[b] = cur[b];
-    if (threadIdx.x == 0) {
-      meta->memberships = grand;
-      if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
-    }
+// The scatter holds its whole chunk in registers (kScatterPer boxes per
+// thread) and emits the chunk's memberships bin range by bin range.  In each
+// round the memberships of a contiguous range of bins go through an LDS
+// staging buffer in destination order, so a wave's stores cover whole runs;
+// and since every workgroup walks the bins in the same order, the runs that
+// share cache lines (adjacent workgroups on one XCD) are written within the
+// same short window and merge in that XCD's L2.
+constexpr uint32_t kScatterThreads = 1024;
+constexpr uint32_t kScatterPer = 8;
+constexpr uint32_t kMaxRounds = 1024;
+constexpr uint32_t kStageEntries = 2048;
+
+struct ScatterLds {
+  float4 box[kStageEntries];
+  uint32_t id[kStageEntries];
+  uint32_t dst[kStageEntries];
+  uint32_t cur[kMaxBins];          // next destination of each bin (global index)
+  int32_t woff[kMaxBins];          // staging position of a destination = dst + woff - round base
+  uint32_t round_hi[kMaxRounds];   // first bin after round r
+  uint32_t round_n[kMaxRounds];    // memberships of this chunk in round r
+  uint32_t round_base[kMaxRounds]; // memberships of this chunk before round r (bin order)
+  uint32_t wsum_round_end[kMaxRounds];  // first bin after round r
+  uint64_t wsum[16];
+  uint32_t n_rounds;
+};
+
+__device__ __forceinline__ uint32_t pack_range(const BinRange& r) {
+  return r.x0 | (r.x1 << 8) | (r.y0 << 16) | (r.y1 << 24);
+}
+
+__global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* __restrict__ boxes,
+                                                                 const InstDesc* __restrict__ inst,
+                                                                 const uint32_t* __restrict__ chunk_inst,
+                                                                 uint32_t n_chunks,
+                                                                 const uint32_t* __restrict__ hist,
+                                                                 const uint32_t* __restrict__ off,
+                                                                 const uint32_t* __restrict__ bin_start,
+                                                                 float4* __restrict__ bbox,
+                                                                 uint32_t* __restrict__ bid, uint64_t cap,
+                                                                 Meta* __restrict__ meta) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  ScatterLds& L = *reinterpret_cast<ScatterLds*>(lds_raw);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (blockIdx.x >= n_chunks) return;
+  if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
+  const InstDesc I = inst[chunk_inst[blockIdx.x]];
+  const uint32_t nb = I.nb;
+  uint64_t start, end;
+  chunk_range(I, blockIdx.x, &start, &end);
+  // 1. this thread's boxes, all loads in flight at once
+  float4 q[kScatterPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kScatterPer; ++k) {
+    const uint64_t i = start + tid + k * kScatterThreads;
+    q[k] = boxes[i < end ? i : end - 1];
   }
-  if (grand > cap) return;
-  const uint32_t* myoff = off + (size_t)blockIdx.x * p.nb;
-  for (uint32_t b = threadIdx.x; b < p.nb; b += blockDim.x) cur[b] += myoff[b];
-  __syncthreads();
-  const uint32_t start = blockIdx.x * p.chunk;
-  const uint32_t end = start + p.chunk < n ? start + p.chunk : n;
-  for (uint32_t i0 = start; i0 < end; i0 += kBatch * blockDim.x) {
-    float4 q[kBatch];
+  // 2./3. per bin: destination cursor, and the offset from a destination to
+  //    the membership's position among this chunk's memberships in bin order
+  const uint64_t lc = blockIdx.x - I.chunk0;
+  const uint32_t* myoff = off + I.off0 + lc * nb;
+  const uint32_t* myhist = hist + I.hist0 + lc * (nb + 1);
+  const uint32_t* mystart = bin_start + I.bin0;
+  {
+    const uint32_t per = (nb + kScatterThreads - 1) / kScatterThreads;
+    const uint32_t b0 = tid * per;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < per; ++k)
+      if (b0 + k < nb) acc += myhist[b0 + k];
+    uint32_t v = acc;
 #pragma unroll
-    for (uint32_t k = 0; k < kBatch; ++k) {
-      const uint32_t i = i0 + threadIdx.x + k * blockDim.x;
-      q[k] = i < end ? boxes[i] : make_float4(1.f, 0.f, 1.f, 0.f);
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(v, d, 64);
+      if (lane >= (uint32_t)d) v += o;
     }
+    if (lane == 63) L.wsum[w] = v;
+    __syncthreads();
+    uint32_t ex = v - acc;
+    for (uint32_t k = 0; k < w; ++k) ex += (uint32_t)L.wsum[k];
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint32_t b = b0 + k;
+      if (b < nb) {
+        const uint32_t dst0 = mystart[b] + myoff[b];
+        L.cur[b] = dst0;
+        L.woff[b] = (int32_t)(ex - dst0);
+        ex += myhist[b];
+      }
+    }
+    // Rounds: bin b goes to round wpos[b] >> log_h, where wpos is the
+    // chunk's exclusive prefix in bin order and 2^log_h <= kStageEntries / 2,
+    // so a round holds at most 2^log_h plus one bin's memberships (rounds
+    // that still overflow the stage are written directly).
+    if (tid == 0) L.n_rounds = 0;
+    __syncthreads();
+    uint32_t chunk_total = 0;
+    for (uint32_t k = 0; k < kScatterThreads / 64; ++k) chunk_total += (uint32_t)L.wsum[k];
+    uint32_t log_h = 0;
+    while ((1u << (log_h + 1)) <= kStageEntries / 2) ++log_h;
+    while ((chunk_total >> log_h) >= kMaxRounds - 1) ++log_h;
+    // rounds 0 .. (wpos of the last bin) >> log_h; each is started by the first
+    // bin whose round is at least its index, so every entry below is set
+    const uint32_t wp_last = (uint32_t)((int32_t)L.cur[nb - 1] + L.woff[nb - 1]);
+    const uint32_t n_rounds = chunk_total ? (wp_last >> log_h) + 1 : 0;
+    for (uint32_t b = tid; b < nb; b += kScatterThreads) {
+      const uint32_t wp = (uint32_t)((int32_t)L.cur[b] + L.woff[b]);
+      const uint32_t rb = wp >> log_h;
+      const uint32_t rprev = b ? ((uint32_t)((int32_t)L.cur[b - 1] + L.woff[b - 1]) >> log_h) : 0xFFFFFFFFu;
+      // bin b starts rounds rprev+1 .. rb (empty rounds share its start)
+      for (uint32_t rr = rprev + 1; rr <= rb && rr < n_rounds; ++rr) {
+        L.round_hi[rr] = b;  // start of round rr
+        L.round_base[rr] = wp;
+      }
+    }
+    __syncthreads();
+    for (uint32_t rr = tid; rr < n_rounds; rr += kScatterThreads) {
+      const uint32_t nxt_start = rr + 1 < n_rounds ? L.round_hi[rr + 1] : nb;
+      const uint32_t nxt_base = rr + 1 < n_rounds ? L.round_base[rr + 1] : chunk_total;
+      L.round_n[rr] = nxt_base - L.round_base[rr];
+      L.wsum_round_end[rr] = nxt_start;
+    }
+    if (tid == 0) L.n_rounds = n_rounds;
+    __syncthreads();
+  }
+  // 4. bin ranges of this thread's boxes, packed (nbx <= 90 fits 8 bits)
+  uint32_t rng[kScatterPer];
 #pragma unroll
-    for (uint32_t k = 0; k < kBatch; ++k) {
-      const uint32_t i = i0 + threadIdx.x + k * blockDim.x;
-      const float4 b = q[k];
-      if (i >= end || !((b.x <= b.y) && (b.z <= b.w))) continue;
-      const BinRange r = bin_range(b, p);
-      for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-        for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
-          const uint32_t pos = atomicAdd(&cur[cy * p.nbx + cx], 1u);
-          bbox[pos] = b;
-          bid[pos] = i;
+  for (uint32_t k = 0; k < kScatterPer; ++k) {
+    const uint64_t i = start + tid + k * kScatterThreads;
+    const float4 b = q[k];
+    const bool ok = i < end && (b.x <= b.y) && (b.z <= b.w);
+    rng[k] = ok ? pack_range(bin_range(b, I)) : 0xFFFFFFFFu;
+  }
+  // 5. rounds
+  const uint32_t n_rounds = L.n_rounds;
+  for (uint32_t r = 0; r < n_rounds; ++r) {
+    const uint32_t b_lo = L.round_hi[r], b_hi = L.wsum_round_end[r], rn = L.round_n[r];
+    if (rn == 0) continue;  // uniform: every thread reads the same table
+    const bool direct = rn > kStageEntries;  // one bin alone overflows the stage
+    // Fixed before the round starts: cursors of the round's bins move below.
+    const int32_t base = (int32_t)L.round_base[r];
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterPer; ++k) {
+      const uint32_t pr = rng[k];
+      if (pr == 0xFFFFFFFFu) continue;
+      const uint32_t x0 = pr & 0xFF, x1 = (pr >> 8) & 0xFF, y0 = (pr >> 16) & 0xFF, y1 = pr >> 24;
+      if (y1 * I.nbx + x1 < b_lo || y0 * I.nbx + x0 >= b_hi) continue;
+      const uint32_t i = (uint32_t)(start - I.box0) + tid + k * kScatterThreads;  // id inside the instance
+      for (uint32_t cy = y0; cy <= y1; ++cy)
+        for (uint32_t cx = x0; cx <= x1; ++cx) {
+          const uint32_t b = cy * I.nbx + cx;
+          if (b < b_lo || b >= b_hi) continue;
+          const uint32_t d = atomicAdd(&L.cur[b], 1u);
+          const uint32_t at = (uint32_t)((int32_t)d + L.woff[b] - base);
+          if (direct || at >= kStageEntries) {
+            if (d < cap) {
+              bbox[d] = q[k];
+              bid[d] = i;
+            }
+          } else {
+            L.box[at] = q[k];
+            L.id[at] = i;
+            L.dst[at] = d;
+          }
         }
+    }
+    __syncthreads();
+    if (!direct) {
+      for (uint32_t k = tid; k < rn; k += kScatterThreads) {
+        const uint32_t d = L.dst[k];
+        if (d < cap) {
+          bbox[d] = L.box[k];
+          bid[d] = L.id[k];
+        }
+      }
+      __syncthreads();
     }
   }
 }
@@ -272,6 +429,8 @@ struct TestLds {
   uint32_t start[kMaxCellsPerBin + 1];  // exclusive scan of cnt (start[kk] = total)
   uint2 stage[kTestWaves][kBinStage];   // per-wave pair staging
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
+  uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, k, nf) of each bin
+  float bininv[kMaxBinsPerGroup];  // inv_f of each bin's instance
   uint32_t wsum[kTestWaves];
   uint32_t out_n;                  // pairs appended to this workgroup's region
   uint32_t strip_hi;
@@ -282,18 +441,24 @@ __device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn
   const uint32_t lane = threadIdx.x & 63;
   uint32_t g = 0;
   if (lane == 0) g = atomicAdd(&L.out_n, sn);
-  g = __shfl(g, 0, 64);
+  g = __builtin_amdgcn_readfirstlane(g);
   for (uint32_t k = lane; k < sn; k += 64)
     if (g + k < cap_wg) region[g + k] = L.stage[w][k];
   sn = 0;
 }
+
+// Fine grid of one bin's instance.
+struct FineGrid {
+  float inv_f;
+  uint32_t nf, k;
+};
 
 // Fine-cell range of a box clipped to the bin whose first fine cell is (gx, gy).
 struct FineRange {
   uint32_t x0, x1, y0, y1, fx0, fy0;
 };
 
-__device__ __forceinline__ FineRange fine_range(const float4& q, const BinnedParams& p, uint32_t gx, uint32_t gy) {
+__device__ __forceinline__ FineRange fine_range(const float4& q, const FineGrid& p, uint32_t gx, uint32_t gy) {
   FineRange r;
   r.fx0 = fine_of(q.x, p.inv_f, p.nf);
   r.fy0 = fine_of(q.z, p.inv_f, p.nf);
@@ -307,7 +472,7 @@ __device__ __forceinline__ FineRange fine_range(const float4& q, const BinnedPar
 
 // Scatter the fine memberships of one coarse membership that fall in cells
 // [c_lo, c_hi) to their positions in the strip (start[] is the cursor).
-__device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint32_t j, const BinnedParams& p,
+__device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint32_t j, const FineGrid& p,
                                               uint32_t gx, uint32_t gy, uint32_t c_lo, uint32_t c_hi,
                                               uint32_t base) {
   const FineRange r = fine_range(q, p, gx, gy);
@@ -328,48 +493,61 @@ __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint3
 __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restrict__ bbox,
                                                            const uint32_t* __restrict__ bid,
                                                            const uint32_t* __restrict__ bin_start,
-                                                           const uint32_t* __restrict__ total, BinnedParams p,
+                                                           const uint32_t* __restrict__ total,
+                                                           const InstDesc* __restrict__ inst,
+                                                           const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
                                                            uint64_t cap_bucket, uint2* __restrict__ scratch,
-                                                           uint32_t cap_wg, uint32_t* __restrict__ wg_count,
+                                                           uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
                                                            Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
   if (meta->error & (kErrInvalidBox | kErrWorkspace)) {
-    if (tid == 0) wg_count[blockIdx.x] = 0u;
+    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
   }
-  const uint32_t K = p.k, kk = K * K;
   uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
   uint32_t sn = 0;
-
-  const uint32_t my_bins = blockIdx.x < p.nb ? (p.nb - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  for (uint32_t t = tid; t < my_bins; t += kTestThreads) {
-    const uint32_t bb = blockIdx.x + t * gridDim.x;
-    L.binfo[t] = make_uint2(bin_start[bb], total[bb]);
-  }
+  uint32_t seg0 = 0;  // region offset of the current bin's first pair (thread 0)
   if (tid == 0) L.out_n = 0u;
-  __syncthreads();
 
   // Registers: unconditional loads with the index clamped into the bucket.
   const uint64_t last = cap_bucket - 1;
   float4 rb[kRegs];
   uint32_t ri[kRegs];
-  if (my_bins) {
-    const uint64_t s0 = L.binfo[0].x;
-#pragma unroll
-    for (uint32_t k = 0; k < kRegs; ++k) {
-      const uint64_t at = s0 + tid + k * kTestThreads < last ? s0 + tid + k * kTestThreads : last;
-      rb[k] = bbox[at];
-      ri[k] = bid[at];
-    }
-  }
   for (uint32_t t = 0; t < my_bins; ++t) {
+    const uint32_t slot = t % kMaxBinsPerGroup;
+    if (slot == 0) {
+      // The next group of this workgroup's bins: location in the bucket and
+      // their instance's grid.
+      __syncthreads();
+      for (uint32_t u = tid; u < kMaxBinsPerGroup && t + u < my_bins; u += kTestThreads) {
+        const uint32_t gb = blockIdx.x + (t + u) * gridDim.x;
+        const InstDesc I = inst[bin_inst[gb]];
+        const uint32_t lb = gb - I.bin0;
+        L.binfo[u] = make_uint2(bin_start[gb], total[gb]);
+        L.bingrid[u] = make_uint4((lb % I.nbx) * I.k, (lb / I.nbx) * I.k, I.k, I.nf);
+        L.bininv[u] = I.inv_f;
+      }
+      __syncthreads();
+      if (t == 0) {
+        const uint64_t s0 = L.binfo[0].x;
+#pragma unroll
+        for (uint32_t k = 0; k < kRegs; ++k) {
+          const uint64_t at = s0 + tid + k * kTestThreads < last ? s0 + tid + k * kTestThreads : last;
+          rb[k] = bbox[at];
+          ri[k] = bid[at];
+        }
+      }
+    }
     const uint32_t b = blockIdx.x + t * gridDim.x;
-    const uint32_t m = L.binfo[t].y;
-    const uint64_t sN = t + 1 < my_bins ? L.binfo[t + 1].x : 0ull;
-    const uint32_t gx = (b % p.nbx) * K, gy = (b / p.nbx) * K;
+    const uint32_t m = L.binfo[slot].y;
+    const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
+    const uint4 bg = L.bingrid[slot];
+    const uint32_t gx = bg.x, gy = bg.y, K = bg.z, kk = K * K;
+    const FineGrid p{L.bininv[slot], bg.w, K};
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
@@ -469,13 +647,13 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
             cs = L.start[c] - base;
             cn = L.cnt[c];
             const uint32_t r = pp - cs;
-            h = (cn & 1u) ? (cn - 1) >> 1 : ((r < (cn >> 1)) ? (cn >> 1) : (cn >> 1) - 1);
+            // (n-1)/2 partners, plus one for the first half of an even cell
+            h = (cn - 1 + (((cn & 1u) == 0u) & (r < (cn >> 1)))) >> 1;
             a = L.box[pp];
           }
-          const bool afx = (int32_t)aw < 0, afy = (aw & kFlagY) != 0;
           const uint32_t ja = aw & kWordIdxMask;
           const uint32_t wrap = cs + cn;
-          // partners (r + k) mod n for k = 1..h, four at a time; loads first
+          // partners (r + k) mod n for k = 1..h, kUnroll at a time; all loads first
           for (uint32_t k = 1; __ballot(k <= h); k += kUnroll) {
             float4 o[kUnroll];
             uint32_t ow[kUnroll];
@@ -491,9 +669,13 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
-              const bool hit = act[u] & (a.x <= o[u].y) & (o[u].x <= a.y) & (a.z <= o[u].w) & (o[u].z <= a.w) &
-                               (afx | ((int32_t)ow[u] < 0)) & (afy | ((ow[u] & kFlagY) != 0));
-              const uint64_t mask = __ballot(hit);
+              // Overlap, and the reference-cell rule: one of the two boxes
+              // starts in this column and one starts in this row (bits 31,
+              // 30).  Each test is one compare into a wave mask.
+              const uint64_t mask = __ballot(act[u]) & __ballot(a.x <= o[u].y) & __ballot(o[u].x <= a.y) &
+                                    __ballot(a.z <= o[u].w) & __ballot(o[u].z <= a.w) &
+                                    __ballot((aw | ow[u]) >= (kFlagX | kFlagY));
+              const bool hit = (mask >> lane) & 1ull;
               if (mask) {
                 const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -501,12 +683,15 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                   const uint32_t ia = L.id[ja], ib = L.id[ow[u] & kWordIdxMask];
                   L.stage[w][pos] = make_uint2(ia < ib ? ia : ib, ia < ib ? ib : ia);
                 }
-                sn += (uint32_t)__popcll(mask);
+                sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
                 if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
               }
             }
           }
         }
+        // Every strip's pairs are in the region before the barrier, so each
+        // bin's pairs form one run of the region.
+        if (sn) stage_flush(L, w, sn, region, cap_wg);
         __syncthreads();  // the strip buffer is reused
         c_lo = c_hi;
       }
@@ -515,6 +700,13 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
         overfull[atomicAdd(&meta->max_per_cell, 1u)] = b;
         atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrOverfull);
       }
+    }
+    // The bin's pairs were all appended before the last barrier; the next
+    // bin appends only after further barriers.
+    if (tid == 0) {
+      const uint32_t e = L.out_n;
+      bin_seg[b] = make_uint2(seg0, e - seg0);
+      seg0 = e;
     }
     if (!prefetched && t + 1 < my_bins) {
       __syncthreads();
@@ -526,48 +718,79 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
       }
     }
   }
-  if (sn) stage_flush(L, w, sn, region, cap_wg);
-  __syncthreads();
-  if (tid == 0) {
-    wg_count[blockIdx.x] = L.out_n;
-    if (L.out_n > cap_wg) {
-      atomicMax((unsigned long long*)&meta->items, (unsigned long long)L.out_n);
-      atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
-    }
+  if (tid == 0 && seg0 > cap_wg) {
+    atomicMax((unsigned long long*)&meta->items, (unsigned long long)seg0);
+    atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
   }
 }
 
-// Concatenate the per-workgroup pair regions into the caller's buffer.
-// Block (v, part) copies the part-th slice of region v.
-constexpr uint32_t kCompactParts = 8;
-
-__global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ scratch, uint32_t cap_wg,
-                                                      const uint32_t* __restrict__ wg_count, uint32_t nwg,
-                                                      uint2* __restrict__ out, uint64_t cap,
-                                                      Meta* __restrict__ meta) {
-  __shared__ uint64_t part[4];
-  const uint32_t v = blockIdx.x / kCompactParts, slice = blockIdx.x % kCompactParts;
-  uint64_t before = 0;
-  for (uint32_t u = threadIdx.x; u < v; u += 256) before += wg_count[u] < cap_wg ? wg_count[u] : cap_wg;
+// Output position of every bin's pairs: exclusive scan of the bins' pair
+// counts by one 1024-thread workgroup (bin_out[n_bins] = the batch's total).
+// Pairs are laid out in bin order, so each instance's pairs are one run.
+__global__ __launch_bounds__(1024) void k_bin_pair_scan(const uint2* __restrict__ bin_seg, uint32_t n_bins,
+                                                        uint64_t* __restrict__ bin_out, uint64_t extra_total,
+                                                        Meta* __restrict__ meta) {
+  __shared__ uint64_t wsum[16];
+  const uint32_t per = (n_bins + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per;
+  uint64_t acc = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < n_bins) acc += bin_seg[b0 + k].y;
+  uint64_t v = acc;
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) before += __shfl_xor(before, d, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = before;
-  __syncthreads();
-  const uint64_t off = part[0] + part[1] + part[2] + part[3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the true count, also when a region overflowed (the host then retries)
-    uint64_t t = 0;
-    for (uint32_t u = 0; u < nwg; ++u) t += wg_count[u];
-    meta->n_pairs = t;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += o;
   }
-  const uint32_t mine = wg_count[v] < cap_wg ? wg_count[v] : cap_wg;
-  const uint32_t per = (mine + kCompactParts - 1) / kCompactParts;
-  const uint32_t k0 = slice * per, k1 = k0 + per < mine ? k0 + per : mine;
-  const uint2* src = scratch + (size_t)v * cap_wg;
-#pragma unroll 4
-  for (uint32_t k = k0 + threadIdx.x; k < k1; k += 256) {
-    const uint64_t dst = off + k;
-    if (dst < cap) out[dst] = src[k];
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  uint64_t ex = v - acc, grand = 0;
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (k < w) ex += wsum[k];
+    grand += wsum[k];
+  }
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t b = b0 + k;
+    if (b < n_bins) {
+      bin_out[b] = ex;
+      ex += bin_seg[b].y;
+    }
+  }
+  if (threadIdx.x == 0) {
+    bin_out[n_bins] = grand;
+    meta->n_pairs = grand + extra_total;  // the true count, also past the capacity
+  }
+}
+
+// Copy each bin's run of its workgroup's region to the bin's output position
+// (one block per bin), and write every instance's (offset, count).
+__global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ scratch, uint32_t cap_wg,
+                                                      uint32_t nwg, const uint2* __restrict__ bin_seg,
+                                                      const uint64_t* __restrict__ bin_out, uint32_t n_bins,
+                                                      const InstDesc* __restrict__ inst,
+                                                      const uint32_t* __restrict__ bin_inst, uint32_t n_inst,
+                                                      uint2* __restrict__ out, uint64_t cap,
+                                                      uint64_t* __restrict__ inst_out) {
+  if (inst_out) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_inst; i += gridDim.x * 256) {
+      const InstDesc I = inst[i];
+      const uint64_t o0 = bin_out[I.bin0];
+      inst_out[2 * (uint64_t)i] = o0 + I.pairs0;
+      inst_out[2 * (uint64_t)i + 1] = bin_out[I.bin0 + I.nb] - o0 + I.pairs_extra;
+    }
+  }
+  const uint32_t gb = blockIdx.x;
+  if (gb >= n_bins) return;
+  const uint2 seg = bin_seg[gb];
+  const uint64_t dst0 = bin_out[gb] + inst[bin_inst[gb]].pairs0;
+  const uint2* src = scratch + (size_t)(gb % nwg) * cap_wg;
+  // a region that overflowed holds only its first cap_wg pairs (the host
+  // regrows and runs again)
+  const uint32_t n = seg.x < cap_wg ? (seg.y < cap_wg - seg.x ? seg.y : cap_wg - seg.x) : 0u;
+  for (uint32_t k = threadIdx.x; k < n; k += 256) {
+    const uint64_t d = dst0 + k;
+    if (d < cap) out[d] = src[seg.x + k];
   }
 }
 
@@ -575,30 +798,44 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
 size_t binned_test_lds_bytes() { return sizeof(TestLds); }
 
 hipError_t binned_init() {
-  return hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)sizeof(TestLds));
+  hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(TestLds));
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)sizeof(ScatterLds));
 }
 
-hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, uint32_t n, const BinnedParams& p,
-                         uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev) {
+hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, uint2* out,
+                         uint64_t cap, uint64_t* inst_out, hipStream_t s, hipEvent_t* ev) {
   hipError_t e;
+  // Every kernel tolerates an empty batch (no chunks, no bins): grids of at
+  // least one block whose blocks find no work.
+  const uint32_t nc = bp.n_chunks ? bp.n_chunks : 1u;
+  const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;
 #define PBIG_MARK(k) \
   if (ev && (e = hipEventRecord(ev[k], s)) != hipSuccess) return e;
   PBIG_MARK(0)
-  k_bin_hist<<<p.nw, 1024, 0, s>>>(boxes, n, p, ws.hist);
+  k_bin_hist<<<nc, 1024, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
   PBIG_MARK(1)
-  k_bin_scan<<<(p.nb + 15) / 16, 256, 0, s>>>(ws.hist, p, ws.off, ws.total, ws.meta);
+  k_bin_scan<<<(nbins + 15) / 16, 256, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
+                                               bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
-  k_bin_scatter<<<p.nw, 1024, 0, s>>>(boxes, n, p, ws.off, ws.total, ws.bin_start, ws.bbox, ws.bid, ws.cap_bucket,
-                                      ws.meta);
+  k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, ws.meta);
   PBIG_MARK(3)
-  k_bin_test<<<p.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, p,
-                                                                  ws.cap_bucket, ws.scratch, ws.cap_wg,
-                                                                  ws.wg_count, ws.overfull, ws.meta);
+  k_bin_scatter<<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks,
+                                                                ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
+                                                                ws.cap_bucket, ws.meta);
   PBIG_MARK(4)
-  k_bin_compact<<<p.test_blocks * kCompactParts, 256, 0, s>>>(ws.scratch, ws.cap_wg, ws.wg_count, p.test_blocks,
-                                                              out, cap, ws.meta);
+  k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
+                                                                   ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
+                                                                   ws.scratch, ws.cap_wg, ws.bin_seg,
+                                                                   ws.overfull, ws.meta);
   PBIG_MARK(5)
+  k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
+  PBIG_MARK(6)
+  k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
+                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+  PBIG_MARK(7)
 #undef PBIG_MARK
   return hipGetLastError();
 }

@@ -21,8 +21,25 @@ using pbig::GridParams;
 using pbig::Meta;
 using pbig::ScanPartial;
 using pbig::Workspace;
-using pbig::BinnedParams;
+using pbig::BatchPlan;
 using pbig::BinnedWorkspace;
+using pbig::InstDesc;
+
+// Tables and workspace of one binned plan: the context's plain path (a batch
+// of one) or a prepared batch.
+struct BinnedState {
+  BinnedWorkspace ws;
+  std::vector<InstDesc> inst;
+  std::vector<uint32_t> chunk_inst, bin_inst;
+  BatchPlan bp;
+  uint64_t generation = 0;  // bumped whenever a device buffer moves
+  // plain-path plan key; capacities of chunk_inst/bin_inst known to be zero
+  uint64_t zero_chunks = 0, zero_bins = 0;
+  bool key_valid = false;
+  uint32_t key_n = 0, key_target = 0;
+  double key_range = 0.0;
+  float key_cell = 0.0f;
+};
 
 struct pbig_hip_ctx {
   int device = 0;
@@ -37,8 +54,13 @@ struct pbig_hip_ctx {
   uint64_t cap_h_in = 0;
   uint2* d_pairs = nullptr;
   uint64_t cap_pairs = 0;
-  BinnedWorkspace bws;
-  pbig_hip_config cfg{PBIG_PATH_AUTO, 3000, 0, 0};
+  BinnedState plain;
+  // Pinned ring for the plain path's instance record: uploads stay ordered
+  // on the stream without a host synchronisation.
+  InstDesc* h_ring = nullptr;
+  hipEvent_t ring_ev[16] = {};
+  uint32_t ring_next = 0;
+  pbig_hip_config cfg{PBIG_PATH_AUTO, 2000, 0, 0};
   int last_path = 0;
   pbig_hip_stats stats{};
   std::string err;
@@ -150,6 +172,7 @@ int prepare(pbig_hip_ctx* c, uint32_t n, uint32_t n_cells) {
 }
 
 hipEvent_t* next_events(pbig_hip_ctx* c, int path);
+int upload_tables(pbig_hip_ctx* c, BinnedState& st, hipStream_t s);
 
 int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, uint32_t n_cells,
              uint2* d_out, uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
@@ -187,61 +210,134 @@ int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, 
   return fail(c, PBIG_E_HIP, "unreachable");
 }
 
-// Plan of the binned path for n boxes, domain side `range`, fine cell side
-// `cell`: K fine cells per coarse bin side (1..32) such that a coarse bin
-// holds about cfg.bin_target boxes, with at most kMaxBins bins.
-bool plan_binned(const pbig_hip_ctx* c, uint32_t n, double range, float cell, BinnedParams* bp) {
+// Plan of one instance on the binned path: n boxes, domain side `range`,
+// fine cell side `cell`.  K fine cells per coarse bin side (1..kMaxK) such
+// that a coarse bin holds about bin_target boxes, at most kMaxBins bins.
+// Chunks of at most kScatterChunk boxes (the scatter holds its chunk in
+// registers); more than 8 chunks are rounded to a multiple of 8 so that the
+// chunks spread evenly over the 8 XCDs.  False when the grid cannot be
+// planned (the general path then runs).
+bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, InstDesc* d) {
+  *d = InstDesc{};
+  d->n = n;
+  if (n == 0) return true;  // no chunks, no bins
   const double nf0 = std::max(1.0, std::ceil(range / (double)cell));
-  const double target = std::max<double>(16.0, c->cfg.bin_target);
-  uint32_t k = (uint32_t)std::lround(std::sqrt(target / std::max<double>(n, 1)) * nf0);
-  k = std::min<uint32_t>(pbig::kMaxK, std::max<uint32_t>(1, k));
+  const double target = std::max<double>(16.0, bin_target);
+  double kd = std::sqrt(target / std::max<double>(n, 1)) * nf0;
+  kd = std::min<double>(std::max<double>(std::round(kd), 1.0), std::min<double>(pbig::kMaxK, nf0));
+  uint32_t k = (uint32_t)kd;
   uint32_t nbx = 0;
   while (true) {
     const double nbxd = std::ceil(nf0 / k);
     if (nbxd * nbxd <= pbig::kMaxBins) { nbx = (uint32_t)nbxd; break; }
-    if (k == pbig::kMaxK) return false;
+    if (k >= pbig::kMaxK) return false;
     ++k;
   }
-  bp->inv_f = 1.0f / cell;
-  bp->k = k;
-  bp->k_magic = ((1ull << 32) + k - 1) / k;
-  bp->nbx = nbx;
-  bp->nb = nbx * nbx;
-  bp->nf = nbx * k;
-  uint32_t nw = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(8, ((uint64_t)n / 16384 + 7) / 8 * 8));
-  bp->nw = nw;
-  bp->chunk = std::max<uint32_t>(1024, (uint32_t)(((uint64_t)n + nw - 1) / nw));
-  bp->test_blocks = std::max<uint32_t>((uint32_t)c->n_cu, (bp->nb + pbig::kMaxBinsPerGroup - 1) / pbig::kMaxBinsPerGroup);
+  d->inv_f = 1.0f / cell;
+  d->k = k;
+  d->k_magic = ((1ull << 32) + k - 1) / k;
+  d->nbx = nbx;
+  d->nb = nbx * nbx;
+  d->nf = nbx * k;
+  uint64_t nw = ((uint64_t)n + pbig::kScatterChunk - 1) / pbig::kScatterChunk;
+  if (nw > 8) nw = (nw + 7) / 8 * 8;
+  d->chunk = (uint32_t)std::min<uint64_t>(pbig::kScatterChunk, ((uint64_t)n + nw - 1) / nw);
+  d->nchunks = (uint32_t)(((uint64_t)n + d->chunk - 1) / d->chunk);
   return true;
 }
 
-int ensure_binned(pbig_hip_ctx* c, const BinnedParams& bp, uint64_t bucket) {
-  BinnedWorkspace& w = c->bws;
+// Global layout of a planned batch: first chunk, bin, histogram row, offset
+// row and earlier pairs of every instance; the chunk -> instance and bin ->
+// instance tables.  `skip` (optional) marks instances left to the general
+// path: they keep their place in the output (pairs_extra pairs) but get no
+// chunks or bins.
+bool layout_tables(BinnedState& st, uint32_t n_cu, const std::vector<uint64_t>* skip_pairs) {
+  BatchPlan& bp = st.bp;
+  bp = BatchPlan{};
+  bp.n_inst = (uint32_t)st.inst.size();
+  uint64_t chunks = 0, bins = 0, hist = 0, off = 0, pairs0 = 0;
+  for (size_t i = 0; i < st.inst.size(); ++i) {
+    InstDesc& d = st.inst[i];
+    const bool skipped = skip_pairs && (*skip_pairs)[i] != ~0ull;
+    if (skipped) {  // no chunks, no bins: the output step sees an empty bin range
+      d.nchunks = 0;
+      d.nb = 0;
+    }
+    const uint32_t nchunks = d.nchunks, nb = d.n == 0 ? 0u : d.nb;
+    d.chunk0 = (uint32_t)chunks;
+    d.bin0 = (uint32_t)bins;
+    d.hist0 = hist;
+    d.off0 = off;
+    d.pairs0 = pairs0;
+    d.pairs_extra = skipped ? (*skip_pairs)[i] : 0ull;
+    pairs0 += d.pairs_extra;
+    chunks += nchunks;
+    bins += nb;
+    hist += (uint64_t)nchunks * (d.nb + 1);
+    off += (uint64_t)nchunks * d.nb;
+    bp.n_boxes += skipped ? 0u : d.n;
+  }
+  if (chunks > 0x7FFFFFFFull || bins > 0x7FFFFFFFull) return false;
+  st.chunk_inst.assign(chunks, 0u);
+  st.bin_inst.assign(bins, 0u);
+  for (size_t i = 0; i < st.inst.size(); ++i) {
+    const InstDesc& d = st.inst[i];
+    for (uint32_t w = 0; w < d.nchunks; ++w) st.chunk_inst[d.chunk0 + w] = (uint32_t)i;
+    if (d.n)
+      for (uint32_t b = 0; b < d.nb; ++b) st.bin_inst[d.bin0 + b] = (uint32_t)i;
+  }
+  bp.n_chunks = (uint32_t)chunks;
+  bp.n_bins = (uint32_t)bins;
+  bp.hist_entries = hist;
+  bp.off_entries = off;
+  bp.extra_total = pairs0;
+  bp.test_blocks = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)n_cu, (uint32_t)std::max<uint64_t>(bins, 1)));
+  return true;
+}
+
+// Device buffers of a binned state for its current plan: tables, histogram
+// rows, per-bin arrays, pair regions (cap_wg each) and the bucket.
+int ensure_state_(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket);
+
+int ensure_state(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket) {
+  const BinnedWorkspace before = st.ws;
+  const int rc = ensure_state_(c, st, bucket);
+  const BinnedWorkspace& w = st.ws;
+  if (w.inst != before.inst || w.chunk_inst != before.chunk_inst || w.bin_inst != before.bin_inst ||
+      w.hist != before.hist || w.off != before.off || w.total != before.total || w.scratch != before.scratch ||
+      w.bbox != before.bbox || w.cap_scratch != before.cap_scratch)
+    ++st.generation;
+  return rc;
+}
+
+int ensure_state_(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket) {
+  BinnedWorkspace& w = st.ws;
+  const BatchPlan& bp = st.bp;
   int rc;
-  uint64_t hist = (uint64_t)bp.nw * (bp.nb + 1), off = (uint64_t)bp.nw * bp.nb;
-  if ((rc = grow(c, &w.hist, &w.cap_hist, hist))) return rc;
-  if ((rc = grow(c, &w.off, &w.cap_off, off))) return rc;
-  if (w.cap_bins < bp.nb || !w.total) {
-    for (uint32_t** q : {&w.total, &w.bin_start, &w.overfull})
+  if ((rc = grow(c, &w.inst, &w.cap_inst, std::max<uint64_t>(bp.n_inst, 1)))) return rc;
+  if ((rc = grow(c, &w.chunk_inst, &w.cap_chunks, std::max<uint64_t>(bp.n_chunks, 1)))) return rc;
+  if ((rc = grow(c, &w.bin_inst, &w.cap_bin_inst, std::max<uint64_t>(bp.n_bins, 1)))) return rc;
+  if ((rc = grow(c, &w.hist, &w.cap_hist, std::max<uint64_t>(bp.hist_entries, 1)))) return rc;
+  if ((rc = grow(c, &w.off, &w.cap_off, std::max<uint64_t>(bp.off_entries, 1)))) return rc;
+  const uint64_t nb = (uint64_t)bp.n_bins + 1;
+  if (w.cap_bins < nb || !w.total) {
+    for (void** q : {(void**)&w.total, (void**)&w.bin_start, (void**)&w.overfull, (void**)&w.bin_seg,
+                     (void**)&w.bin_out})
       if (*q) HIP_TRY(c, hipFree(*q));
     w.total = w.bin_start = w.overfull = nullptr;
+    w.bin_seg = nullptr;
+    w.bin_out = nullptr;
     w.cap_bins = 0;
-    uint64_t d1 = 0, d2 = 0, d3 = 0;
-    if ((rc = grow(c, &w.total, &d1, bp.nb))) return rc;
-    if ((rc = grow(c, &w.bin_start, &d2, bp.nb))) return rc;
-    if ((rc = grow(c, &w.overfull, &d3, bp.nb))) return rc;
-    w.cap_bins = bp.nb;
+    uint64_t d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
+    if ((rc = grow(c, &w.total, &d1, nb))) return rc;
+    if ((rc = grow(c, &w.bin_start, &d2, nb))) return rc;
+    if ((rc = grow(c, &w.overfull, &d3, nb))) return rc;
+    if ((rc = grow(c, &w.bin_seg, &d4, nb))) return rc;
+    if ((rc = grow(c, &w.bin_out, &d5, nb))) return rc;
+    w.cap_bins = nb;
   }
-  if ((rc = grow(c, &w.wg_count, &w.cap_wgc, bp.test_blocks))) return rc;
-  uint64_t scratch = (uint64_t)bp.test_blocks * w.cap_wg;
-  if (w.cap_scratch < scratch || !w.scratch) {
-    if (w.scratch) HIP_TRY(c, hipFree(w.scratch));
-    w.scratch = nullptr;
-    w.cap_scratch = 0;
-    uint64_t d = 0;
-    if ((rc = grow(c, &w.scratch, &d, scratch))) return rc;
-    w.cap_scratch = scratch;
-  }
+  const uint64_t scratch = (uint64_t)bp.test_blocks * w.cap_wg;
+  if ((rc = grow(c, &w.scratch, &w.cap_scratch, std::max<uint64_t>(scratch, 1)))) return rc;
   if (w.cap_bucket < bucket || !w.bbox) {
     if (w.bbox) HIP_TRY(c, hipFree(w.bbox));
     if (w.bid) HIP_TRY(c, hipFree(w.bid));
@@ -249,21 +345,92 @@ int ensure_binned(pbig_hip_ctx* c, const BinnedParams& bp, uint64_t bucket) {
     w.bid = nullptr;
     w.cap_bucket = 0;
     uint64_t d1 = 0, d2 = 0;
-    if ((rc = grow(c, &w.bbox, &d1, bucket))) return rc;
-    if ((rc = grow(c, &w.bid, &d2, bucket))) return rc;
-    w.cap_bucket = bucket;
+    if ((rc = grow(c, &w.bbox, &d1, std::max<uint64_t>(bucket, 1)))) return rc;
+    if ((rc = grow(c, &w.bid, &d2, std::max<uint64_t>(bucket, 1)))) return rc;
+    w.cap_bucket = std::max<uint64_t>(bucket, 1);
   }
-  w.meta = c->ws.meta;
   return PBIG_OK;
 }
 
-int prepare_binned(pbig_hip_ctx* c, uint32_t n, const BinnedParams& bp) {
-  uint64_t bucket = std::max<uint64_t>(c->bws.cap_bucket, (uint64_t)n + n / 4 + 4096);
-  // Pair region per test workgroup: at least the previous need, first guess
-  // one pair per box spread over the workgroups, plus slack.
-  uint64_t per = std::max<uint64_t>(8192, ((uint64_t)n + bp.test_blocks - 1) / bp.test_blocks * 2);
-  c->bws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, std::max<uint64_t>(c->bws.cap_wg, per));
-  return ensure_binned(c, bp, bucket);
+// Size a state for its plan (at least the previous need; first guesses of
+// 1.25 bucket entries and, per test workgroup, twice its share of one pair
+// per box) and upload its tables.  Synchronises the stream: the host tables
+// are pageable memory.
+int size_state(pbig_hip_ctx* c, BinnedState& st) {
+  const uint64_t n = st.bp.n_boxes;
+  const uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, n + n / 4 + 4096);
+  const uint64_t per = std::max<uint64_t>(8192, (n + st.bp.test_blocks - 1) / st.bp.test_blocks * 2);
+  const uint32_t cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, std::max<uint64_t>(st.ws.cap_wg, per));
+  if (cap_wg != st.ws.cap_wg) ++st.generation;
+  st.ws.cap_wg = cap_wg;
+  return ensure_state(c, st, bucket);
+}
+
+void free_state(BinnedState& st) {
+  const BinnedWorkspace& b = st.ws;
+  for (void* p : {(void*)b.inst, (void*)b.chunk_inst, (void*)b.bin_inst, (void*)b.hist, (void*)b.off,
+                  (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bin_seg, (void*)b.bin_out,
+                  (void*)b.scratch, (void*)b.bbox, (void*)b.bid})
+    if (p) (void)hipFree(p);
+  st.ws = BinnedWorkspace{};
+}
+
+int upload_tables(pbig_hip_ctx* c, BinnedState& st, hipStream_t s) {
+  const BatchPlan& bp = st.bp;
+  if (bp.n_inst)
+    HIP_TRY(c, hipMemcpyAsync(st.ws.inst, st.inst.data(), bp.n_inst * sizeof(InstDesc), hipMemcpyHostToDevice, s));
+  if (bp.n_chunks)
+    HIP_TRY(c, hipMemcpyAsync(st.ws.chunk_inst, st.chunk_inst.data(), bp.n_chunks * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+  if (bp.n_bins)
+    HIP_TRY(c, hipMemcpyAsync(st.ws.bin_inst, st.bin_inst.data(), bp.n_bins * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return PBIG_OK;
+}
+
+// Plain path: the context's batch of one, re-planned only when the instance
+// shape or the configuration changes.
+int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_t s, bool* ok) {
+  BinnedState& st = c->plain;
+  *ok = true;
+  if (st.key_valid && st.key_n == n && st.key_range == range && st.key_cell == cell &&
+      st.key_target == c->cfg.bin_target)
+    return PBIG_OK;
+  st.key_valid = false;
+  InstDesc d;
+  if (n == 0 || !plan_instance(n, range, cell, c->cfg.bin_target, &d)) {
+    *ok = false;
+    return PBIG_OK;
+  }
+  st.inst.assign(1, d);
+  if (!layout_tables(st, (uint32_t)c->n_cu, nullptr)) {
+    *ok = false;
+    return PBIG_OK;
+  }
+  st.ws.meta = c->ws.meta;
+  int rc;
+  if ((rc = size_state(c, st))) return rc;
+  // A batch of one: every chunk and bin belongs to instance 0, so the
+  // chunk/bin tables only have to be zero; the instance record goes through
+  // the pinned ring, ordered on the stream behind earlier launches.
+  if (st.zero_chunks != st.ws.cap_chunks || st.zero_bins != st.ws.cap_bin_inst) {
+    HIP_TRY(c, hipMemsetAsync(st.ws.chunk_inst, 0, st.ws.cap_chunks * sizeof(uint32_t), s));
+    HIP_TRY(c, hipMemsetAsync(st.ws.bin_inst, 0, st.ws.cap_bin_inst * sizeof(uint32_t), s));
+    st.zero_chunks = st.ws.cap_chunks;
+    st.zero_bins = st.ws.cap_bin_inst;
+  }
+  const uint32_t slot = c->ring_next++ % 16u;
+  HIP_TRY(c, hipEventSynchronize(c->ring_ev[slot]));
+  c->h_ring[slot] = st.inst[0];
+  HIP_TRY(c, hipMemcpyAsync(st.ws.inst, &c->h_ring[slot], sizeof(InstDesc), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipEventRecord(c->ring_ev[slot], s));
+  st.key_valid = true;
+  st.key_n = n;
+  st.key_range = range;
+  st.key_cell = cell;
+  st.key_target = c->cfg.bin_target;
+  return PBIG_OK;
 }
 
 hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
@@ -272,46 +439,59 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
   return &c->events[(size_t)c->timing_used++ * kEventsPerCall];
 }
 
-// Binned path, synchronous.  Returns PBIG_OK, an error, or 1 when a coarse
-// bin did not fit in LDS (the caller then runs the general path).
-int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, const BinnedParams& bp, uint2* d_out,
-                    uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
+// Run a binned state synchronously, regrowing the workspace when the bucket
+// or a pair region overflowed.  On success *m holds the result words; the
+// caller handles kErrOverfull.
+int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
+                   uint64_t* d_inst_out, hipStream_t s, Meta* m) {
   int rc;
-  if ((rc = prepare_binned(c, n, bp))) return rc;
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    HIP_TRY(c, pbig::launch_binned(c->bws, d_boxes, n, bp, d_out, cap, s, next_events(c, PBIG_PATH_BINNED)));
-    HIP_TRY(c, hipMemcpyAsync(c->h_meta, c->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, st.bp, d_out, cap, d_inst_out, s,
+                                   next_events(c, PBIG_PATH_BINNED)));
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    const Meta m = *c->h_meta;
-    if (m.error & pbig::kErrInvalidBox)
+    *m = *c->h_meta;
+    if (m->error & pbig::kErrInvalidBox)
       return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
-    if (m.error & pbig::kErrWorkspace) {
-      if (attempt == 1) return fail(c, PBIG_E_NOMEM, "bin workspace did not fit after regrowth");
-      if (m.items > c->bws.cap_wg)
-        c->bws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, m.items + m.items / 4 + 1024);
-      if ((rc = ensure_binned(c, bp, std::max<uint64_t>(c->bws.cap_bucket, m.memberships + m.memberships / 16 + 4096))))
-        return rc;
+    if (m->error & pbig::kErrWorkspace) {
+      if (attempt == 2) return fail(c, PBIG_E_NOMEM, "bin workspace did not fit after regrowth");
+      if (m->items > st.ws.cap_wg)
+        st.ws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, m->items + m->items / 4 + 1024);
+      const uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, m->memberships + m->memberships / 16 + 4096);
+      if ((rc = ensure_state(c, st, bucket))) return rc;
       continue;
     }
-    if (m.error & pbig::kErrOverfull) return 1;
-    c->last_path = PBIG_PATH_BINNED;
-    c->stats.n_boxes = n;
-    c->stats.n_cells = (uint64_t)bp.nf * bp.nf;
-    c->stats.cells_per_row = bp.nf;
-    c->stats.memberships = m.memberships;
-    c->stats.tile_pairs = 0;
-    c->stats.n_pairs = m.n_pairs;
-    c->stats.max_per_cell = 0;
-    c->stats.path = PBIG_PATH_BINNED;
-    c->stats.bins_per_row = bp.nbx;
-    c->stats.cells_per_bin_side = bp.k;
-    *n_pairs = m.n_pairs;
-    if (m.n_pairs > cap)
-      return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu",
-                  (unsigned long long)m.n_pairs, (unsigned long long)cap);
     return PBIG_OK;
   }
   return fail(c, PBIG_E_HIP, "unreachable");
+}
+
+// Binned plain path, synchronous.  Returns PBIG_OK, an error, or 1 when a
+// coarse bin did not fit in LDS (the caller then runs the general path).
+int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d_out, uint64_t cap,
+                    hipStream_t s, uint64_t* n_pairs) {
+  BinnedState& st = c->plain;
+  Meta m;
+  int rc;
+  if ((rc = run_state_sync(c, st, d_boxes, d_out, cap, nullptr, s, &m))) return rc;
+  if (m.error & pbig::kErrOverfull) return 1;
+  const InstDesc& d = st.inst[0];
+  c->last_path = PBIG_PATH_BINNED;
+  c->stats.n_boxes = n;
+  c->stats.n_cells = (uint64_t)d.nf * d.nf;
+  c->stats.cells_per_row = d.nf;
+  c->stats.memberships = m.memberships;
+  c->stats.tile_pairs = 0;
+  c->stats.n_pairs = m.n_pairs;
+  c->stats.max_per_cell = 0;
+  c->stats.path = PBIG_PATH_BINNED;
+  c->stats.bins_per_row = d.nbx;
+  c->stats.cells_per_bin_side = d.k;
+  *n_pairs = m.n_pairs;
+  if (m.n_pairs > cap)
+    return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)m.n_pairs,
+                (unsigned long long)cap);
+  return PBIG_OK;
 }
 
 // Either path, synchronous, by the context's configuration.
@@ -319,9 +499,10 @@ int run_any_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, double rang
                  uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
   int rc;
   if (c->cfg.path != PBIG_PATH_GENERAL) {
-    BinnedParams bp;
-    if (plan_binned(c, n, range, cell, &bp)) {
-      rc = run_binned_sync(c, d_boxes, n, bp, d_out, cap, s, n_pairs);
+    bool ok = false;
+    if ((rc = plan_plain(c, n, range, cell, s, &ok))) return rc;
+    if (ok) {
+      rc = run_binned_sync(c, d_boxes, n, d_out, cap, s, n_pairs);
       if (rc != 1) return rc;
       if (c->cfg.path == PBIG_PATH_BINNED)
         return fail(c, PBIG_E_TOO_DENSE, "a coarse bin holds more than %u fine memberships", pbig::kMaxFine);
@@ -377,6 +558,14 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
     return PBIG_E_NOMEM;
   }
   memset(c->h_meta, 0, sizeof(Meta));
+  bool ok = hipHostMalloc((void**)&c->h_ring, 16 * sizeof(InstDesc), hipHostMallocDefault) == hipSuccess;
+  for (hipEvent_t& e : c->ring_ev)
+    ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess &&
+         hipEventRecord(e, c->stream) == hipSuccess;
+  if (!ok) {
+    pbig_hip_destroy(c);
+    return PBIG_E_HIP;
+  }
   *out = c;
   return PBIG_OK;
 }
@@ -386,13 +575,14 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Workspace& w = c->ws;
-  const BinnedWorkspace& b = c->bws;
   for (void* p : {(void*)w.cnt, (void*)w.start, (void*)w.partials, (void*)w.items, (void*)w.mbox,
-                  (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs, (void*)b.hist, (void*)b.off,
-                  (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bbox, (void*)b.bid,
-                  (void*)b.scratch, (void*)b.wg_count})
+                  (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs})
     if (p) (void)hipFree(p);
+  free_state(c->plain);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->ring_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_ring) (void)hipHostFree(c->h_ring);
   if (c->h_meta) (void)hipHostFree(c->h_meta);
   if (c->h_in) (void)hipHostFree(c->h_in);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -485,18 +675,18 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
   uint32_t n_cells;
   int rc;
   if ((rc = grid_of(c, range, cell_size, &gp, &n_cells))) return rc;
-  BinnedParams bp;
   bool binned = c->cfg.path == PBIG_PATH_BINNED ||
                 (c->cfg.path == PBIG_PATH_AUTO && c->last_path != PBIG_PATH_GENERAL);
-  if (binned && !plan_binned(c, n, range, cell_size, &bp)) {
-    if (c->cfg.path == PBIG_PATH_BINNED)
+  if (binned) {
+    bool ok = false;
+    if ((rc = plan_plain(c, n, range, cell_size, s, &ok))) return rc;
+    if (!ok && c->cfg.path == PBIG_PATH_BINNED)
       return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
-    binned = false;
+    binned = ok;
   }
   if (binned) {
-    if ((rc = prepare_binned(c, n, bp))) return rc;
-    HIP_TRY(c, pbig::launch_binned(c->bws, (const float4*)d_boxes, n, bp, (uint2*)d_pairs, cap, s,
-                                   next_events(c, PBIG_PATH_BINNED)));
+    HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, c->plain.bp, (uint2*)d_pairs, cap,
+                                   nullptr, s, next_events(c, PBIG_PATH_BINNED)));
   } else {
     if ((rc = prepare(c, n, n_cells))) return rc;
     HIP_TRY(c, pbig::launch_plain(c->ws, (const float4*)d_boxes, n, gp, n_cells, (uint2*)d_pairs, cap, s,
@@ -602,7 +792,7 @@ int pbig_hip_configure(pbig_hip_ctx* c, const pbig_hip_config* cfg) {
   if (!c || !cfg) return PBIG_E_ARG;
   if (cfg->path < PBIG_PATH_AUTO || cfg->path > PBIG_PATH_BINNED) return fail(c, PBIG_E_ARG, "unknown path %d", cfg->path);
   c->cfg = *cfg;
-  if (c->cfg.bin_target == 0) c->cfg.bin_target = 3000;
+  if (c->cfg.bin_target == 0) c->cfg.bin_target = 2000;
   c->last_path = 0;
   return PBIG_OK;
 }
@@ -610,6 +800,297 @@ int pbig_hip_configure(pbig_hip_ctx* c, const pbig_hip_config* cfg) {
 int pbig_hip_last_stats(const pbig_hip_ctx* c, pbig_hip_stats* out) {
   if (!c || !out) return PBIG_E_ARG;
   *out = c->stats;
+  return PBIG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches.
+struct pbig_hip_batch {
+  pbig_hip_ctx* ctx = nullptr;
+  std::vector<pbig_hip_instance> desc;
+  std::vector<uint8_t> general;     // instances the binned path cannot plan
+  BinnedState st;
+  uint64_t max_box_end = 0;         // largest box_offset + n_boxes
+  uint2* d_tmp = nullptr;           // fallback pairs
+  uint64_t cap_tmp = 0;
+  uint64_t* d_inst_out = nullptr;   // n_inst x {offset, count}
+  uint64_t cap_inst_out = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_generation = 0;
+  hipStream_t capture_stream = nullptr;
+};
+
+namespace {
+
+int batch_fail(pbig_hip_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  b->ctx->err = buf;
+  return code;
+}
+
+int batch_check_args(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total) {
+  if (b->max_box_end > n_boxes_total)
+    return batch_fail(b, PBIG_E_ARG, "an instance reaches box %llu beyond the %llu boxes given",
+                      (unsigned long long)b->max_box_end, (unsigned long long)n_boxes_total);
+  if (b->max_box_end && !d_boxes) return batch_fail(b, PBIG_E_ARG, "d_boxes is null");
+  if (((uintptr_t)d_boxes & 15) != 0) return batch_fail(b, PBIG_E_ARG, "d_boxes must be 16-byte aligned");
+  return PBIG_OK;
+}
+
+void destroy_graph(pbig_hip_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Instance i through the general path into d_out (capacity cap); the count
+// also when it exceeds cap.
+int run_general_instance(pbig_hip_batch* b, const float4* d_boxes, uint32_t i, uint2* d_out, uint64_t cap,
+                         hipStream_t s, uint64_t* count) {
+  pbig_hip_ctx* c = b->ctx;
+  const pbig_hip_instance& in = b->desc[i];
+  GridParams gp;
+  uint32_t n_cells;
+  int rc;
+  if ((rc = grid_of(c, in.range, in.cell_size, &gp, &n_cells))) return rc;
+  rc = run_sync(c, d_boxes + in.box_offset, in.n_boxes, gp, n_cells, d_out, cap, s, count);
+  return rc == PBIG_E_OVERFLOW ? PBIG_OK : rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbig_hip_batch_create(pbig_hip_ctx* c, const pbig_hip_instance* inst, uint32_t n_inst, pbig_hip_batch** out) {
+  if (!c || !out) return PBIG_E_ARG;
+  *out = nullptr;
+  if (n_inst && !inst) return fail(c, PBIG_E_ARG, "inst is null");
+  HIP_TRY(c, hipSetDevice(c->device));
+  pbig_hip_batch* b = new pbig_hip_batch();
+  b->ctx = c;
+  b->desc.assign(inst, inst + n_inst);
+  b->general.assign(n_inst, 0);
+  b->st.inst.resize(n_inst);
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    const pbig_hip_instance& in = inst[i];
+    GridParams gp;
+    uint32_t n_cells;
+    int rc;
+    if (in.n_boxes > pbig::kMaxBoxes) rc = fail(c, PBIG_E_ARG, "instance %u: n = %u exceeds 2^30-1 boxes", i, in.n_boxes);
+    else rc = grid_of(c, in.range, in.cell_size, &gp, &n_cells);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    b->max_box_end = std::max<uint64_t>(b->max_box_end, in.box_offset + in.n_boxes);
+    if (!plan_instance(in.n_boxes, in.range, in.cell_size, c->cfg.bin_target, &b->st.inst[i])) {
+      b->general[i] = 1;
+      b->st.inst[i] = InstDesc{};
+      b->st.inst[i].n = in.n_boxes;
+    }
+    b->st.inst[i].box0 = in.box_offset;
+  }
+  std::vector<uint64_t> skip(n_inst, ~0ull);
+  bool any_general = false;
+  for (uint32_t i = 0; i < n_inst; ++i)
+    if (b->general[i]) { skip[i] = 0; any_general = true; }
+  if (!layout_tables(b->st, (uint32_t)c->n_cu, any_general ? &skip : nullptr)) {
+    delete b;
+    return fail(c, PBIG_E_ARG, "batch too large (more than 2^31 chunks or bins)");
+  }
+  int rc;
+  if (hipMalloc((void**)&b->st.ws.meta, sizeof(Meta)) != hipSuccess) {
+    delete b;
+    return fail(c, PBIG_E_NOMEM, "hipMalloc of the batch's result words failed");
+  }
+  if ((rc = size_state(c, b->st)) || (rc = upload_tables(c, b->st, c->stream)) ||
+      (rc = grow(c, &b->d_inst_out, &b->cap_inst_out, 2 * (uint64_t)std::max<uint32_t>(n_inst, 1)))) {
+    pbig_hip_batch_destroy(b);
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    pbig_hip_batch_destroy(b);
+    return fail(c, PBIG_E_HIP, "stream creation failed");
+  }
+  *out = b;
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_destroy(pbig_hip_batch* b) {
+  if (!b) return PBIG_E_ARG;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipDeviceSynchronize();
+  destroy_graph(b);
+  free_state(b->st);
+  for (void* p : {(void*)b->st.ws.meta, (void*)b->d_tmp, (void*)b->d_inst_out})
+    if (p) (void)hipFree(p);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  delete b;
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_info(const pbig_hip_batch* b, uint64_t* n_boxes, uint32_t* n_chunks, uint32_t* n_bins) {
+  if (!b) return PBIG_E_ARG;
+  if (n_boxes) *n_boxes = b->st.bp.n_boxes;
+  if (n_chunks) *n_chunks = b->st.bp.n_chunks;
+  if (n_bins) *n_bins = b->st.bp.n_bins;
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                          uint64_t cap, uint64_t* d_inst_pairs, uint64_t* d_result, void* stream) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  int rc;
+  if ((rc = batch_check_args(b, d_boxes, n_boxes_total))) return rc;
+  if (cap && !d_pairs) return fail(c, PBIG_E_ARG, "d_pairs is null with a nonzero capacity");
+  for (uint8_t g : b->general)
+    if (g) return fail(c, PBIG_E_ARG, "the batch has instances only the synchronous entry point can run");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
+                                 next_events(c, PBIG_PATH_BINNED)));
+  if (d_result)
+    HIP_TRY(c, hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                           uint64_t cap, uint64_t* d_inst_pairs, uint64_t* d_result) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  int rc;
+  if ((rc = batch_check_args(b, d_boxes, n_boxes_total))) return rc;
+  for (uint8_t g : b->general)
+    if (g) return fail(c, PBIG_E_ARG, "the batch has instances only the synchronous entry point can run");
+  HIP_TRY(c, hipSetDevice(c->device));
+  destroy_graph(b);
+  hipStream_t s = b->capture_stream;
+  HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap,
+                                     d_inst_pairs, s, nullptr);
+  if (e == hipSuccess && d_result)
+    e = hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    HIP_TRY(c, e != hipSuccess ? e : e2);
+  }
+  b->graph = g;
+  HIP_TRY(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_generation = b->st.generation;
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_replay(pbig_hip_batch* b, void* stream) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  if (!b->exec) return fail(c, PBIG_E_ARG, "no graph captured");
+  if (b->graph_generation != b->st.generation)
+    return fail(c, PBIG_E_ARG, "the batch's workspace moved since the capture; capture again");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : c->stream));
+  return PBIG_OK;
+}
+
+int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                         uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
+  if (cap && !d_pairs) return fail(c, PBIG_E_ARG, "d_pairs is null with a nonzero capacity");
+  int rc;
+  if ((rc = batch_check_args(b, d_boxes, n_boxes_total))) return rc;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const float4* boxes = (const float4*)d_boxes;
+  const uint32_t n_inst = (uint32_t)b->desc.size();
+  BinnedState& st = b->st;
+  // Instances for the general path: those the binned path cannot plan, and
+  // those with a coarse bin or fine cell beyond LDS (found by a first run).
+  std::vector<uint8_t> gen = b->general;
+  bool any_gen = false;
+  for (uint8_t g : gen) any_gen = any_gen || g;
+  Meta m{};
+  if (!any_gen) {
+    if ((rc = run_state_sync(c, st, boxes, (uint2*)d_pairs, cap, b->d_inst_out, s, &m))) return rc;
+    if (m.error & pbig::kErrOverfull) {
+      const uint32_t n_over = (uint32_t)std::min<uint64_t>(m.max_per_cell, st.bp.n_bins);
+      std::vector<uint32_t> over(n_over);
+      if (n_over)
+        HIP_TRY(c, hipMemcpy(over.data(), st.ws.overfull, n_over * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t gb : over)
+        if (gb < st.bin_inst.size()) gen[st.bin_inst[gb]] = 1;
+      any_gen = true;
+    }
+  }
+  if (any_gen) {
+    // General path per flagged instance: counts first, then the pairs into
+    // one temporary buffer, instance after instance.
+    std::vector<uint64_t> cnt(n_inst, ~0ull), at(n_inst, 0);
+    uint64_t tmp_total = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+      if (!gen[i]) continue;
+      uint64_t k = 0;
+      if (b->desc[i].n_boxes && (rc = run_general_instance(b, boxes, i, nullptr, 0, s, &k))) return rc;
+      cnt[i] = k;
+      at[i] = tmp_total;
+      tmp_total += k;
+    }
+    if ((rc = grow(c, &b->d_tmp, &b->cap_tmp, std::max<uint64_t>(tmp_total, 1)))) return rc;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+      if (!gen[i] || cnt[i] == 0) continue;
+      uint64_t k = 0;
+      if ((rc = run_general_instance(b, boxes, i, b->d_tmp + at[i], cnt[i], s, &k))) return rc;
+    }
+    // The binned launch without them, their slots reserved.
+    const std::vector<InstDesc> saved = st.inst;
+    if (!layout_tables(st, (uint32_t)c->n_cu, &cnt)) return fail(c, PBIG_E_ARG, "batch layout failed");
+    if ((rc = upload_tables(c, st, s))) return rc;
+    rc = run_state_sync(c, st, boxes, (uint2*)d_pairs, cap, b->d_inst_out, s, &m);
+    // restore the full plan for the asynchronous entry points
+    st.inst = saved;
+    std::vector<uint64_t> keep(n_inst, ~0ull);
+    for (uint32_t i = 0; i < n_inst; ++i)
+      if (b->general[i]) keep[i] = 0;
+    layout_tables(st, (uint32_t)c->n_cu, &keep);
+    int rc2 = upload_tables(c, st, s);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (m.error & pbig::kErrOverfull) return fail(c, PBIG_E_HIP, "a coarse bin overflowed twice");
+    // Place the general path's pairs in their slots.
+    std::vector<uint64_t> io(2 * (size_t)n_inst);
+    if (n_inst)
+      HIP_TRY(c, hipMemcpy(io.data(), b->d_inst_out, io.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n_inst; ++i) {
+      if (!gen[i] || cnt[i] == 0) continue;
+      const uint64_t off = io[2 * (size_t)i];
+      if (off >= cap) continue;
+      const uint64_t k = std::min<uint64_t>(cnt[i], cap - off);
+      HIP_TRY(c, hipMemcpyAsync(d_pairs + 2 * off, b->d_tmp + at[i], k * sizeof(uint2), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(c, hipStreamSynchronize(s));
+  }
+  if (h_inst_pairs && n_inst)
+    HIP_TRY(c, hipMemcpy(h_inst_pairs, b->d_inst_out, 2 * (size_t)n_inst * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  c->stats = pbig_hip_stats{};
+  c->stats.n_boxes = st.bp.n_boxes;
+  c->stats.memberships = m.memberships;
+  c->stats.n_pairs = m.n_pairs;
+  c->stats.path = PBIG_PATH_BINNED;
+  *n_pairs = m.n_pairs;
+  if (m.n_pairs > cap)
+    return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)m.n_pairs,
+                (unsigned long long)cap);
   return PBIG_OK;
 }
 

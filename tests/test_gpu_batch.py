@@ -1,0 +1,253 @@
+"""GPU parity of batches: many instances in one launch sequence, each
+instance's pairs compared (sorted, bit-identical) with the CPU oracle run on
+that instance alone.  Covers the synchronous entry point, the asynchronous
+launch, HIP-graph capture and replay, the general-path fallback for a crowded
+instance inside a batch, empty and one-box instances and pair overflow."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def eng(torch_cuda):
+    import pbig_amd
+
+    e = pbig_amd.Engine(0)
+    yield e
+    e.close()
+
+
+def concat(instances):
+    """[(boxes, L)] -> (boxes, offsets, ranges)."""
+    n = [b.shape[0] for b, _ in instances]
+    offsets = np.zeros(len(n) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(n)
+    boxes = np.concatenate([b for b, _ in instances]).astype(np.float32) if offsets[-1] else \
+        np.zeros((0, 4), np.float32)
+    return boxes, offsets, np.array([L for _, L in instances], dtype=np.float64)
+
+
+def expected(boxes, offsets):
+    import oracle
+
+    return [oracle.sort_pairs(oracle.sweep_pairs(boxes[offsets[i]:offsets[i + 1]]))
+            for i in range(len(offsets) - 1)]
+
+
+def split(pairs, io):
+    import oracle
+
+    return [oracle.sort_pairs(pairs[int(o):int(o) + int(c)]) for o, c in io]
+
+
+def to_dev(torch, boxes):
+    return torch.from_numpy(np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)).to("cuda:0")
+
+
+def check_all(got, want):
+    assert len(got) == len(want)
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g, w), f"instance {i}: {g.shape[0]} pairs vs {w.shape[0]} expected"
+
+
+def small_batch(seed=41, k=48, nmax=20000):
+    from pbig_amd import workloads
+
+    boxes, offsets, ranges = workloads.batched(k, seed, nmin=64, nmax=nmax)
+    return boxes, offsets, ranges.astype(np.float64)
+
+
+@pytest.mark.parametrize("cell", [8.0, 32.0, 100.0])
+def test_batch_check_matches_oracle(torch_cuda, eng, cell):
+    import pbig_amd
+
+    boxes, offsets, ranges = small_batch()
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, cell) as b:
+        n, io = b.check(d, pairs)
+        assert n == total
+        assert int(io[-1, 0] + io[-1, 1]) == total
+        assert np.array_equal(io[1:, 0], io[:-1, 0] + io[:-1, 1])  # instances in order, back to back
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+
+
+def test_batch_alone_equals_in_mix(torch_cuda, eng):
+    """An instance gives the same sorted pairs alone (plain path) and inside a batch."""
+    import oracle
+    import pbig_amd
+
+    boxes, offsets, ranges = small_batch(seed=43, k=16)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((4 * boxes.shape[0], 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, 32.0) as b:
+        n, io = b.check(d, pairs)
+        in_mix = split(pairs[:n].cpu().numpy().view(np.uint32), io)
+    eng.configure(pbig_amd.PATH_BINNED)
+    try:
+        for i in range(16):
+            sub = to_dev(torch_cuda, boxes[offsets[i]:offsets[i + 1]])
+            out = torch_cuda.empty((4 * sub.shape[0] + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+            m = eng.check_device(sub, float(ranges[i]), 32.0, out)
+            alone = oracle.sort_pairs(out[:m].cpu().numpy().view(np.uint32))
+            assert np.array_equal(alone, in_mix[i]), i
+    finally:
+        eng.configure(pbig_amd.PATH_AUTO)
+
+
+def test_batch_launch_and_graph_replay(torch_cuda, eng):
+    import pbig_amd
+
+    torch = torch_cuda
+    boxes, offsets, ranges = small_batch(seed=47, k=64)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch, boxes)
+    cap = total + 4096
+    with pbig_amd.Batch(eng, offsets, ranges, 32.0) as b:
+        pairs = torch.empty((cap, 2), dtype=torch.int32, device="cuda:0")
+        n, io = b.check(d, pairs)  # sizes the workspace
+        assert n == total
+        io_dev = torch.zeros((len(want), 2), dtype=torch.int64, device="cuda:0")
+        res = torch.zeros(2, dtype=torch.int64, device="cuda:0")
+        pairs.fill_(-1)
+        b.launch(d, pairs, io_dev, res)
+        torch.cuda.synchronize()
+        assert res.cpu().tolist() == [total, 0]
+        assert np.array_equal(io_dev.cpu().numpy().astype(np.uint64), io)
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+        # graph: replayed several times, eager work and checks between replays
+        b.capture(d, pairs, io_dev, res)
+        for r in range(4):
+            pairs.fill_(-1)
+            io_dev.zero_()
+            res.fill_(-7)
+            torch.cuda.synchronize()
+            b.replay()
+            torch.cuda.synchronize()
+            assert res.cpu().tolist() == [total, 0], r
+            io_r = io_dev.cpu().numpy().astype(np.uint64)
+            assert np.array_equal(io_r, io), r
+            check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io_r), want)
+
+
+def test_batch_edge_instances(torch_cuda, eng):
+    """Empty, one-box, two identical, touching-edge and all-overlapping instances."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    inst = [
+        (np.zeros((0, 4), np.float32), 64.0),
+        (np.array([[1, 2, 1, 2]], np.float32), 64.0),
+        (np.array([[5, 6, 5, 6], [5, 6, 5, 6]], np.float32), 64.0),
+        (np.array([[0, 1, 0, 1], [1, 2, 1, 2], [2, 3, 0, 1]], np.float32), 4.0),  # shared edges/corners
+        (workloads.uniform(2000, 256.0, 1.0, 8.0, 9), 256.0),
+        (np.zeros((0, 4), np.float32), 10.0),
+        (np.tile(np.array([[10, 300, 10, 300]], np.float32), (300, 1)), 512.0),  # 44,850 pairs
+        (workloads.uniform(5000, 1024.0, 1.0, 64.0, 10), 1024.0),
+    ]
+    boxes, offsets, ranges = concat(inst)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total + 16, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, 16.0) as b:
+        n, io = b.check(d, pairs)
+        assert n == total
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+
+
+def test_batch_crowded_instance_falls_back(torch_cuda, eng):
+    """A 6000-box stack in one fine cell overflows LDS: that instance runs on
+    the general path and its pairs land in its slot between the others."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    stack = np.tile(np.array([[3, 4, 3, 4]], np.float32), (6000, 1))  # 17,997,000 pairs
+    inst = [(workloads.uniform(3000, 512.0, 1.0, 8.0, 11), 512.0), (stack, 64.0),
+            (workloads.uniform(4000, 512.0, 1.0, 8.0, 12), 512.0)]
+    boxes, offsets, ranges = concat(inst)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, 16.0) as b:
+        n, io = b.check(d, pairs)
+        assert n == total
+        assert [int(c) for _, c in io] == [w.shape[0] for w in want]
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+        # the asynchronous launch reports the overflow instead
+        res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+        b.launch(d, pairs, None, res)
+        torch_cuda.cuda.synchronize()
+        assert res[1].item() & 8
+
+
+def test_batch_overflow_reports_counts(torch_cuda, eng):
+    import pbig_amd
+
+    boxes, offsets, ranges = small_batch(seed=53, k=8)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total // 2, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, 32.0) as b:
+        n, io = b.check(d, pairs, allow_overflow=True)
+        assert n == total
+        assert [int(c) for _, c in io] == [w.shape[0] for w in want]
+        with pytest.raises(pbig_amd.PbigError) as e:
+            b.check(d, pairs)
+        assert e.value.status == pbig_amd.PBIG_E_OVERFLOW
+
+
+def test_batch_rejects_bad_arguments(torch_cuda, eng):
+    import pbig_amd
+
+    boxes, offsets, ranges = small_batch(seed=59, k=4)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((16, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pytest.raises(pbig_amd.PbigError):
+        pbig_amd.Batch(eng, offsets, ranges, 0.0)  # cell size must be > 0
+    with pbig_amd.Batch(eng, offsets, ranges, 32.0) as b:
+        with pytest.raises(pbig_amd.PbigError):
+            b.check(d[:-1], pairs)  # fewer boxes than the instances cover
+    bad = boxes.copy()
+    bad[offsets[2] + 3] = [5, 4, 0, 1]
+    with pbig_amd.Batch(eng, offsets, ranges, 32.0) as b:
+        with pytest.raises(pbig_amd.PbigError) as e:
+            b.check(to_dev(torch_cuda, bad), pairs)
+        assert e.value.status == pbig_amd.PBIG_E_INVALID_BOX
+
+
+@pytest.mark.slow
+def test_batch_c4_full(torch_cuda, eng):
+    """The 4,096-instance batch (129M boxes): every instance against the
+    oracle's grid restatement at the same cell size."""
+    import oracle
+    import pbig_amd
+    from pbig_amd import workloads
+
+    boxes, offsets, ranges = workloads.batched(4096, workloads.SEEDS["c4"])
+    d = to_dev(torch_cuda, boxes)
+    cap = int(0.2 * boxes.shape[0])
+    pairs = torch_cuda.empty((cap, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges.astype(np.float64), 32.0) as b:
+        n, io = b.check(d, pairs)
+    got = pairs[:n].cpu().numpy().view(np.uint32)
+    for i in range(4096):
+        sub = boxes[offsets[i]:offsets[i + 1]]
+        w = oracle.sort_pairs(oracle.grid_pairs(sub, float(ranges[i]), 32, threads=16))
+        g = oracle.sort_pairs(got[int(io[i, 0]):int(io[i, 0] + io[i, 1])])
+        assert np.array_equal(g, w), i

@@ -272,7 +272,7 @@ def test_binned_c1_cell_sizes_and_bin_sizes(torch_cuda, binned, orc, cs, bin_tar
 def test_binned_c2_full_size(torch_cuda, binned, orc, manifest, cs):
     from pbig_amd import workloads
 
-    binned.configure(PATHS["binned"], 3000)
+    binned.configure(PATHS["binned"], 2000)
     boxes, L = workloads.c2_large_uniform()
     m = manifest["c2_uniform_4m"]
     got = run_device(torch_cuda, binned, boxes, L, cs, cap=4 * m["pairs"])
@@ -285,7 +285,7 @@ def test_binned_c2_full_size(torch_cuda, binned, orc, manifest, cs):
 def test_binned_edge_cases(torch_cuda, binned, orc, name):
     import pbig_amd
 
-    binned.configure(PATHS["binned"], 3000)
+    binned.configure(PATHS["binned"], 2000)
     boxes, L, cs = EDGE[name]
     if not len(boxes):
         return
