@@ -95,6 +95,109 @@ def read_traffic(path_name, cs, bin_target):
     return out, os.path.relpath(fn, ROOT)
 
 
+def batched_leg(torch, steps, warmup):
+    """The 4,096-instance workload (n log-uniform in [64, 262144], L = 16
+    sqrt(n), sides U[1,8], seed 4): one HIP-graph replay of the batch per
+    step, against per-instance calls of the plain path on the same instances
+    (synchronous calls, as pbig_check is, and asynchronous calls queued back
+    to back)."""
+    import numpy as np
+
+    import pbig_amd
+    from pbig_amd import autotune, workloads
+
+    boxes, offsets, ranges = workloads.batched(4096, workloads.SEEDS["c4"])
+    k = offsets.shape[0] - 1
+    N = int(boxes.shape[0])
+    table = autotune.load_table()
+    cells = autotune.cell_sizes(offsets, ranges, table)
+    dev = torch.device("cuda:0")
+    eng = pbig_amd.Engine(0)
+    eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
+    d = torch.from_numpy(boxes).to(dev)
+    del boxes
+    cap = N // 4
+    pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    b = pbig_amd.Batch(eng, offsets, ranges.astype(np.float64), cells)
+    total, io = b.check(d, pairs)
+    info = b.info()
+    res = torch.zeros(2, dtype=torch.int64, device=dev)
+    io_dev = torch.zeros((k, 2), dtype=torch.int64, device=dev)
+    b.capture(d, pairs, io_dev, res)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        b.replay(stream)
+    torch.cuda.synchronize()
+    s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s_ev.record(stream)
+    for _ in range(steps):
+        b.replay(stream)
+    e_ev.record(stream)
+    torch.cuda.synchronize()
+    wall_batch = (time.perf_counter() - t0) / steps
+    ms_batch = s_ev.elapsed_time(e_ev) / steps
+    r = res.cpu().tolist()
+    assert r == [total, 0], (r, total)
+    assert np.array_equal(io_dev.cpu().numpy().astype(np.uint64), io)
+
+    # Per-instance plain path on the same device array and cell sizes.
+    views = [d[int(offsets[i]):int(offsets[i + 1])] for i in range(k)]
+    L = [float(x) for x in ranges]
+    cs = [float(x) for x in cells]
+
+    def sync_loop():
+        tot = 0
+        for i in range(k):
+            tot += eng.check_device(views[i], L[i], cs[i], pairs)
+        return tot
+
+    sync_loop()  # warm-up: plans and workspace
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tot_sync = sync_loop()
+    wall_sync = time.perf_counter() - t0
+    assert tot_sync == total, (tot_sync, total)
+
+    res1 = torch.zeros(2, dtype=torch.int64, device=dev)
+
+    def async_loop():
+        for i in range(k):
+            eng.check_device_async(views[i], L[i], cs[i], pairs, res1, stream=stream)
+
+    async_loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s_ev.record(stream)
+    async_loop()
+    e_ev.record(stream)
+    torch.cuda.synchronize()
+    wall_async = time.perf_counter() - t0
+    ms_async = s_ev.elapsed_time(e_ev)
+    assert res1.cpu().tolist() == [int(io[-1, 1]), 0]
+    b.close()
+    eng.close()
+    del d, pairs, views
+    torch.cuda.empty_cache()
+    batch_s = ms_batch / 1e3
+    return {
+        "workload": "4,096 instances, n log-uniform [64, 262144], L = 16 sqrt(n), sides U[1,8], seed 4, "
+                    "boxes resident in HBM",
+        "instances": k, "boxes": N, "pairs": total, "chunks": info["n_chunks"], "bins": info["n_bins"],
+        "cell_sizes": "autotuner table (" + str(table.get("source", "default")) + ")",
+        "bin_target": int(table["bin_target"]),
+        "graph_replay_ms": ms_batch,
+        "graph_replay_host_wall_ms": wall_batch * 1e3,
+        "boxes_per_s": N / batch_s,
+        "pairs_per_s": total / batch_s,
+        "per_instance_sync_ms": wall_sync * 1e3,
+        "per_instance_async_ms": ms_async,
+        "per_instance_async_host_wall_ms": wall_async * 1e3,
+        "speedup_vs_per_instance_sync": wall_sync / batch_s,
+        "speedup_vs_per_instance_async": (max(ms_async, wall_async * 1e3) / 1e3) / batch_s,
+    }
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -116,6 +219,8 @@ def main():
                     help="fine cell side; 0 = the reference rule (256 on this instance)")
     ap.add_argument("--path", choices=("auto", "general", "binned"), default="auto")
     ap.add_argument("--bin-target", type=int, default=2000)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-instance batch leg")
+    ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (torch) three times, a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -192,10 +297,11 @@ def main():
     value = n_pairs / (ms_per_step / 1e3)
     avg = {k: v / calls for k, v in per_kernel.items()}
     # B_alg terms by kernel (the binned kernels do the same four jobs)
+    # (bin_starts and bin_pair_scan move a few KB: counted in bin_scan)
     kernel_alg = {"count": B["count"], "scatter": B["scatter"], "test": B["test"],
                   "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref,
-                  "bin_hist": B["count"], "bin_scan": B["scan"], "bin_scatter": B["scatter"],
-                  "bin_test": 20 * M_ref, "bin_compact": 8 * n_pairs}
+                  "bin_hist": B["count"], "bin_scan": B["scan"], "bin_starts": 0.0, "bin_scatter": B["scatter"],
+                  "bin_test": 20 * M_ref, "bin_pair_scan": 0.0, "bin_compact": 8 * n_pairs}
     dominant = max(avg, key=lambda k: avg[k])
     traffic, traffic_src = read_traffic(path_name, cs, args.bin_target)
     achieved = kernel_alg[dominant] / (avg[dominant] / 1e3) / 1e9
@@ -240,9 +346,13 @@ def main():
             "host_wall_ms_per_step": wall * 1e3 / args.steps,
         },
     }
+    eng.close()
+    del d_boxes, d_pairs
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        out["batched"] = batched_leg(torch, args.batched_steps, 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(boxes, L, ref_cs, n_pairs)
-    eng.close()
     print(json.dumps(out))
 
 

@@ -35,6 +35,10 @@ enum {
 
 typedef struct pbig_hip_ctx pbig_hip_ctx;
 
+/* Every `stream` argument is a hipStream_t; NULL selects the context's own
+ * stream and PBIG_HIP_NULL_STREAM selects HIP's null (default) stream. */
+#define PBIG_HIP_NULL_STREAM ((void*)~(uintptr_t)0)
+
 /* Statistics of the last check on a context. */
 typedef struct pbig_hip_stats {
   uint32_t cell_size_num;  /* cell side = cell_size_num / cell_size_den */

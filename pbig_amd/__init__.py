@@ -165,10 +165,16 @@ def _ptr(t) -> int:
     return int(t.data_ptr())
 
 
+NULL_STREAM = (1 << 64) - 1  # PBIG_HIP_NULL_STREAM
+
+
 def _stream_ptr(stream) -> int | None:
+    """None -> the context's stream; a torch stream (or raw handle) -> that
+    stream, where handle 0 (torch's default stream) is HIP's null stream."""
     if stream is None:
         return None
-    return int(getattr(stream, "cuda_stream", stream))
+    h = int(getattr(stream, "cuda_stream", stream))
+    return h if h else NULL_STREAM
 
 
 class Engine:

@@ -103,13 +103,14 @@ def time_batch(engine, d_boxes, starts, counts, ranges, cells, reps: int = 5) ->
             b.check(d_boxes, pairs)
         res = torch.zeros(2, dtype=torch.int64, device=d_boxes.device)
         b.capture(d_boxes, pairs, None, res)
-        b.replay()
+        stream = torch.cuda.current_stream()
+        b.replay(stream)
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record()
+        s.record(stream)
         for _ in range(reps):
-            b.replay()
-        e.record()
+            b.replay(stream)
+        e.record(stream)
         torch.cuda.synchronize()
         r = res.cpu().tolist()
         if r[1] != 0 or r[0] != n:
@@ -166,7 +167,10 @@ def main():
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--out", default=TABLE_PATH)
     args = ap.parse_args()
+    import torch
+
     boxes, offsets, ranges = workloads.batched(args.instances, workloads.SEEDS["c4"])
+    torch.zeros(1, device="cuda:0")  # torch's HIP runtime first, then the engine's context
     eng = pbig_amd.Engine(0)
     t = tune(eng, boxes, offsets, ranges)
     print("wrote", save_table(t, args.out))

@@ -77,6 +77,13 @@ constexpr int kEventsPerCall = 8;
 
 namespace {
 
+// NULL: the context's stream; PBIG_HIP_NULL_STREAM: HIP's null stream.
+hipStream_t pick_stream(const pbig_hip_ctx* c, void* stream) {
+  if (!stream) return c->stream;
+  if (stream == PBIG_HIP_NULL_STREAM) return (hipStream_t)0;
+  return (hipStream_t)stream;
+}
+
 int fail(pbig_hip_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -643,7 +650,7 @@ int pbig_hip_check_device(pbig_hip_ctx* c, const float* d_boxes, uint32_t n, dou
   if (cap && !d_pairs) return fail(c, PBIG_E_ARG, "d_pairs is null with a nonzero capacity");
   if (((uintptr_t)d_boxes & 15) != 0) return fail(c, PBIG_E_ARG, "d_boxes must be 16-byte aligned");
   HIP_TRY(c, hipSetDevice(c->device));
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = pick_stream(c, stream);
   if (cell_size == 0.0f) {
     std::vector<float> head(4 * (size_t)std::min<uint32_t>(n, 1000));
     if (!head.empty()) {
@@ -670,7 +677,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
   if (((uintptr_t)d_boxes & 15) != 0) return fail(c, PBIG_E_ARG, "d_boxes must be 16-byte aligned");
   if (!(cell_size > 0.0f)) return fail(c, PBIG_E_ARG, "the asynchronous path needs cell_size > 0");
   HIP_TRY(c, hipSetDevice(c->device));
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = pick_stream(c, stream);
   GridParams gp;
   uint32_t n_cells;
   int rc;
@@ -955,7 +962,7 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   for (uint8_t g : b->general)
     if (g) return fail(c, PBIG_E_ARG, "the batch has instances only the synchronous entry point can run");
   HIP_TRY(c, hipSetDevice(c->device));
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = pick_stream(c, stream);
   HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
   if (d_result)
@@ -998,7 +1005,7 @@ int pbig_hip_batch_replay(pbig_hip_batch* b, void* stream) {
   if (b->graph_generation != b->st.generation)
     return fail(c, PBIG_E_ARG, "the batch's workspace moved since the capture; capture again");
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : c->stream));
+  HIP_TRY(c, hipGraphLaunch(b->exec, pick_stream(c, stream)));
   return PBIG_OK;
 }
 
@@ -1011,7 +1018,7 @@ int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_box
   int rc;
   if ((rc = batch_check_args(b, d_boxes, n_boxes_total))) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = pick_stream(c, stream);
   const float4* boxes = (const float4*)d_boxes;
   const uint32_t n_inst = (uint32_t)b->desc.size();
   BinnedState& st = b->st;
