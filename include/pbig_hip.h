@@ -172,6 +172,12 @@ int pbig_hip_batch_create(pbig_hip_ctx* ctx, const pbig_hip_instance* inst, uint
                           pbig_hip_batch** batch);
 int pbig_hip_batch_destroy(pbig_hip_batch* batch);
 
+/* Re-plan a batch for a new set of instances, keeping (and only growing) its
+ * device workspace: the serving loop's way to reuse a batch slot.  The
+ * tables are uploaded on `stream`, which this call synchronises; the batch's
+ * earlier work must be complete.  Drops a captured graph. */
+int pbig_hip_batch_update(pbig_hip_batch* batch, const pbig_hip_instance* inst, uint32_t n_inst, void* stream);
+
 /* Synchronous batch check.  d_boxes holds n_boxes_total boxes (every
  * instance's range must lie inside).  h_inst_pairs (optional, host) receives
  * n_inst x {offset, count}.  Regrows the workspace when needed; an instance

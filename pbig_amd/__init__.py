@@ -43,6 +43,7 @@ EXPORTS = (
     "pbig_hip_configure",
     "pbig_hip_batch_create",
     "pbig_hip_batch_destroy",
+    "pbig_hip_batch_update",
     "pbig_hip_batch_check",
     "pbig_hip_batch_launch",
     "pbig_hip_batch_capture",
@@ -139,6 +140,7 @@ def load_library(path: str | None = None):
         "pbig_hip_configure": (i32, [P, ctypes.POINTER(Config)]),
         "pbig_hip_batch_create": (i32, [P, ctypes.POINTER(Instance), u32, ctypes.POINTER(P)]),
         "pbig_hip_batch_destroy": (i32, [P]),
+        "pbig_hip_batch_update": (i32, [P, ctypes.POINTER(Instance), u32, P]),
         "pbig_hip_batch_check": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
         "pbig_hip_batch_launch": (i32, [P, P, u64, P, u64, P, P, P]),
         "pbig_hip_batch_capture": (i32, [P, P, u64, P, u64, P, P]),
@@ -322,9 +324,10 @@ class Batch:
                   cell_sizes)
         return b
 
-    def _create(self, engine, starts, counts, ranges, cell_sizes):
-        self._eng = engine
-        self._lib = engine._lib
+    @staticmethod
+    def _table(starts, counts, ranges, cell_sizes):
+        starts = np.asarray(starts, dtype=np.int64)
+        counts = np.asarray(counts, dtype=np.int64)
         k = starts.shape[0]
         ranges = np.broadcast_to(np.asarray(ranges, dtype=np.float64), (k,))
         cells = np.broadcast_to(np.asarray(cell_sizes, dtype=np.float32), (k,))
@@ -334,6 +337,12 @@ class Batch:
             arr[i].n_boxes = int(counts[i])
             arr[i].cell_size = float(cells[i])
             arr[i].range = float(ranges[i])
+        return arr, k, (int(np.max(starts + counts)) if k else 0)
+
+    def _create(self, engine, starts, counts, ranges, cell_sizes):
+        self._eng = engine
+        self._lib = engine._lib
+        arr, k, end = self._table(starts, counts, ranges, cell_sizes)
         h = ctypes.c_void_p()
         self._h = None
         rc = self._lib.pbig_hip_batch_create(engine._h, arr, k, ctypes.byref(h))
@@ -341,7 +350,16 @@ class Batch:
             engine._raise(rc)
         self._h = h
         self.n_inst = k
-        self.n_boxes_total = int(np.max(starts + counts)) if k else 0
+        self.n_boxes_total = end
+
+    def update(self, starts, counts, ranges, cell_sizes, stream=None):
+        """Re-plan for new instances, reusing the device workspace."""
+        arr, k, end = self._table(starts, counts, ranges, cell_sizes)
+        rc = self._lib.pbig_hip_batch_update(self._h, arr, k, _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._eng._raise(rc)
+        self.n_inst = k
+        self.n_boxes_total = end
 
     def close(self):
         if getattr(self, "_h", None):

@@ -872,6 +872,50 @@ int run_general_instance(pbig_hip_batch* b, const float4* d_boxes, uint32_t i, u
   return rc == PBIG_E_OVERFLOW ? PBIG_OK : rc;
 }
 
+// (Re)plan a batch: validate and plan every instance, lay out the tables,
+// grow the workspace (it is never shrunk) and upload the tables on stream s
+// (which it synchronises).  A captured graph no longer matches and is dropped.
+int plan_batch(pbig_hip_batch* b, const pbig_hip_instance* inst, uint32_t n_inst, hipStream_t s) {
+  pbig_hip_ctx* c = b->ctx;
+  if (n_inst && !inst) return fail(c, PBIG_E_ARG, "inst is null");
+  destroy_graph(b);
+  std::vector<InstDesc> plan(n_inst);
+  std::vector<uint8_t> general(n_inst, 0);
+  uint64_t max_end = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    const pbig_hip_instance& in = inst[i];
+    GridParams gp;
+    uint32_t n_cells;
+    int rc;
+    if (in.n_boxes > pbig::kMaxBoxes) return fail(c, PBIG_E_ARG, "instance %u: n = %u exceeds 2^30-1 boxes", i, in.n_boxes);
+    if ((rc = grid_of(c, in.range, in.cell_size, &gp, &n_cells))) return rc;
+    max_end = std::max<uint64_t>(max_end, in.box_offset + in.n_boxes);
+    if (!plan_instance(in.n_boxes, in.range, in.cell_size, c->cfg.bin_target, &plan[i])) {
+      general[i] = 1;
+      plan[i] = InstDesc{};
+      plan[i].n = in.n_boxes;
+    }
+    plan[i].box0 = in.box_offset;
+  }
+  b->desc.assign(inst, inst + n_inst);
+  b->general = general;
+  b->max_box_end = max_end;
+  b->st.inst = plan;
+  std::vector<uint64_t> skip(n_inst, ~0ull);
+  bool any_general = false;
+  for (uint32_t i = 0; i < n_inst; ++i)
+    if (general[i]) {
+      skip[i] = 0;
+      any_general = true;
+    }
+  if (!layout_tables(b->st, (uint32_t)c->n_cu, any_general ? &skip : nullptr))
+    return fail(c, PBIG_E_ARG, "batch too large (more than 2^31 chunks or bins)");
+  int rc;
+  if ((rc = size_state(c, b->st))) return rc;
+  if ((rc = grow(c, &b->d_inst_out, &b->cap_inst_out, 2 * (uint64_t)std::max<uint32_t>(n_inst, 1)))) return rc;
+  return upload_tables(c, b->st, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -879,56 +923,31 @@ extern "C" {
 int pbig_hip_batch_create(pbig_hip_ctx* c, const pbig_hip_instance* inst, uint32_t n_inst, pbig_hip_batch** out) {
   if (!c || !out) return PBIG_E_ARG;
   *out = nullptr;
-  if (n_inst && !inst) return fail(c, PBIG_E_ARG, "inst is null");
   HIP_TRY(c, hipSetDevice(c->device));
   pbig_hip_batch* b = new pbig_hip_batch();
   b->ctx = c;
-  b->desc.assign(inst, inst + n_inst);
-  b->general.assign(n_inst, 0);
-  b->st.inst.resize(n_inst);
-  for (uint32_t i = 0; i < n_inst; ++i) {
-    const pbig_hip_instance& in = inst[i];
-    GridParams gp;
-    uint32_t n_cells;
-    int rc;
-    if (in.n_boxes > pbig::kMaxBoxes) rc = fail(c, PBIG_E_ARG, "instance %u: n = %u exceeds 2^30-1 boxes", i, in.n_boxes);
-    else rc = grid_of(c, in.range, in.cell_size, &gp, &n_cells);
-    if (rc) {
-      delete b;
-      return rc;
-    }
-    b->max_box_end = std::max<uint64_t>(b->max_box_end, in.box_offset + in.n_boxes);
-    if (!plan_instance(in.n_boxes, in.range, in.cell_size, c->cfg.bin_target, &b->st.inst[i])) {
-      b->general[i] = 1;
-      b->st.inst[i] = InstDesc{};
-      b->st.inst[i].n = in.n_boxes;
-    }
-    b->st.inst[i].box0 = in.box_offset;
-  }
-  std::vector<uint64_t> skip(n_inst, ~0ull);
-  bool any_general = false;
-  for (uint32_t i = 0; i < n_inst; ++i)
-    if (b->general[i]) { skip[i] = 0; any_general = true; }
-  if (!layout_tables(b->st, (uint32_t)c->n_cu, any_general ? &skip : nullptr)) {
-    delete b;
-    return fail(c, PBIG_E_ARG, "batch too large (more than 2^31 chunks or bins)");
-  }
-  int rc;
   if (hipMalloc((void**)&b->st.ws.meta, sizeof(Meta)) != hipSuccess) {
     delete b;
     return fail(c, PBIG_E_NOMEM, "hipMalloc of the batch's result words failed");
-  }
-  if ((rc = size_state(c, b->st)) || (rc = upload_tables(c, b->st, c->stream)) ||
-      (rc = grow(c, &b->d_inst_out, &b->cap_inst_out, 2 * (uint64_t)std::max<uint32_t>(n_inst, 1)))) {
-    pbig_hip_batch_destroy(b);
-    return rc;
   }
   if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) {
     pbig_hip_batch_destroy(b);
     return fail(c, PBIG_E_HIP, "stream creation failed");
   }
+  int rc;
+  if ((rc = plan_batch(b, inst, n_inst, c->stream))) {
+    pbig_hip_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return PBIG_OK;
+}
+
+int pbig_hip_batch_update(pbig_hip_batch* b, const pbig_hip_instance* inst, uint32_t n_inst, void* stream) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  HIP_TRY(c, hipSetDevice(c->device));
+  return plan_batch(b, inst, n_inst, pick_stream(c, stream));
 }
 
 int pbig_hip_batch_destroy(pbig_hip_batch* b) {

@@ -1,0 +1,359 @@
+"""Serving loop: continuous batching of box-intersection requests on one GPU.
+
+A request is one instance (boxes, domain side).  The reference serves one
+instance per blocking pbig_check call (src/pbig/pbig.h:10, copy-in at
+pbig.cu:1401, per-call grid allocation at 1385 and free at 1979-1980); here a
+server keeps request box sets resident in an HBM pool and runs many requests
+per launch:
+
+* Admission.  A request is admitted when its boxes fit in the pool (first-fit
+  over a free list) and in the HBM budget; its boxes are then copied into the
+  pool on a copy stream.  Requests are admitted in arrival order.
+* Continuous batching.  Each of ``slots`` batch slots owns a pbig_hip_batch
+  (workspace reused across batches), a stream and pinned result buffers.
+  Whenever a slot is free, every admitted request whose upload has been
+  issued forms the slot's next batch: the slot's stream waits for the
+  uploads, the batch is re-planned and launched, and the result words, the
+  per-request (offset, count) table and then the pairs come back by
+  asynchronous copies.  Requests that arrive while batches run are admitted
+  as pool space frees up and go into the next free slot, so uploads, kernels
+  of several batches and result copies overlap on the device.
+* Completion.  A request's pairs are delivered when its batch's copies are
+  done; its pool space is freed at once.  Latency = delivery - arrival.
+
+Each request's pair set does not depend on what else is in its batch: the
+instance is planned and tested alone inside the batch (its own grid, bins and
+output run), which the tests check against the request run alone.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import pbig_amd
+from pbig_amd import autotune
+
+
+@dataclass
+class Request:
+    """One instance: boxes (n, 4) float32 {xmin,xmax,ymin,ymax}, domain side L.
+    ``host`` may be a pinned torch tensor of the boxes (faster uploads)."""
+    rid: int
+    boxes: np.ndarray
+    L: float
+    arrival: float = 0.0
+    cell_size: float = 0.0           # 0 = the autotuner's table
+    host: object = None
+    # filled by the server
+    pairs: np.ndarray | None = None
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    batch_id: int = -1
+    _off: int = -1
+
+    @property
+    def n(self) -> int:
+        return int(self.boxes.shape[0])
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+class Pool:
+    """First-fit allocator over a device array of boxes (16 bytes each)."""
+
+    def __init__(self, capacity_boxes: int):
+        self.capacity = int(capacity_boxes)
+        self.free = [(0, self.capacity)]  # sorted (start, length)
+        self.used = 0
+
+    def alloc(self, n: int) -> int:
+        n = max(int(n), 1)
+        for k, (s, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (s + n, ln - n)
+                self.used += n
+                return s
+        return -1
+
+    def release(self, s: int, n: int):
+        n = max(int(n), 1)
+        self.used -= n
+        self.free.append((s, n))
+        self.free.sort()
+        merged = []
+        for a, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((a, ln))
+        self.free = merged
+
+
+@dataclass
+class _Slot:
+    stream: object
+    batch: object = None
+    pairs: object = None             # device (cap, 2) int32
+    io: object = None                # device (k, 2) int64
+    res: object = None               # device (2,) int64
+    h_res: object = None             # pinned
+    h_io: object = None              # pinned
+    h_pairs: object = None           # pinned
+    ev: object = None
+    state: str = "free"              # free | running | copying
+    reqs: list = field(default_factory=list)
+    bid: int = -1
+    total: int = 0
+
+
+class Server:
+    """Continuous-batching server on one GPU (see the module docstring).
+
+    hbm_budget: bytes of device memory the server may hold: the box pool
+    (pool_frac of the budget) plus every slot's workspace and pair buffers.
+    """
+
+    def __init__(self, device: int = 0, hbm_budget: int = 8 << 30, slots: int = 2, pool_frac: float = 0.5,
+                 table: dict | None = None, bytes_per_box: int = 96):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device(f"cuda:{device}")
+        torch.zeros(1, device=self.dev)
+        self.table = table or autotune.load_table()
+        self.eng = pbig_amd.Engine(device)
+        self.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
+        pool_boxes = int(hbm_budget * pool_frac) // 16
+        self.pool = Pool(pool_boxes)
+        self.d_pool = torch.empty((pool_boxes, 4), dtype=torch.float32, device=self.dev)
+        # per-slot budget -> largest batch (boxes): bucket, regions, pairs
+        self.slot_budget = int(hbm_budget * (1 - pool_frac)) // max(slots, 1)
+        self.max_batch_boxes = max(self.slot_budget // bytes_per_box, 1)
+        self.copy_stream = torch.cuda.Stream(device=self.dev)
+        self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(slots)]
+        self.queue: list[Request] = []       # arrived, not admitted
+        self.admitted: list[Request] = []    # uploaded (or upload issued), not yet batched
+        self.upload_ev = None
+        self.done: list[Request] = []
+        self.n_batches = 0
+        self.batch_sizes: list[int] = []
+
+    # -- client side ---------------------------------------------------------
+    def submit(self, req: Request):
+        if req.n > self.pool.capacity:
+            raise ValueError(f"request {req.rid}: {req.n} boxes exceed the pool of {self.pool.capacity}")
+        if req.cell_size <= 0:
+            req.cell_size = float(autotune.cell_sizes([0, req.n], [req.L], self.table)[0])
+        self.queue.append(req)
+
+    def close(self):
+        self.torch.cuda.synchronize()
+        for s in self.slots:
+            if s.batch is not None:
+                s.batch.close()
+                s.batch = None
+        self.eng.close()
+
+    # -- server loop -----------------------------------------------------------
+    def _admit(self, now: float) -> bool:
+        torch = self.torch
+        progressed = False
+        while self.queue:
+            r = self.queue[0]
+            off = self.pool.alloc(r.n)
+            if off < 0:
+                break
+            self.queue.pop(0)
+            r._off = off
+            r.t_admit = now
+            if r.n:
+                src = r.host if r.host is not None else torch.from_numpy(np.ascontiguousarray(r.boxes))
+                with torch.cuda.stream(self.copy_stream):
+                    self.d_pool[off:off + r.n].copy_(src, non_blocking=True)
+            self.admitted.append(r)
+            progressed = True
+        if progressed:
+            self.upload_ev = torch.cuda.Event()
+            self.upload_ev.record(self.copy_stream)
+        return progressed
+
+    def _ensure_buffers(self, s: _Slot, k: int, n_boxes: int):
+        torch = self.torch
+        cap = max(4096, 2 * n_boxes)  # two pairs per box before the first regrowth
+        if s.pairs is None or s.pairs.shape[0] < cap:
+            s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
+            s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+        if s.io is None or s.io.shape[0] < k:
+            kk = max(k, 256)
+            s.io = torch.empty((kk, 2), dtype=torch.int64, device=self.dev)
+            s.h_io = torch.empty((kk, 2), dtype=torch.int64, pin_memory=True)
+        if s.res is None:
+            s.res = torch.zeros(2, dtype=torch.int64, device=self.dev)
+            s.h_res = torch.zeros(2, dtype=torch.int64, pin_memory=True)
+
+    def _launch(self, s: _Slot, now: float):
+        torch = self.torch
+        reqs, rest, total = [], [], 0
+        for r in self.admitted:
+            if not reqs or total + r.n <= self.max_batch_boxes:
+                reqs.append(r)
+                total += r.n
+            else:
+                rest.append(r)
+        self.admitted = rest
+        starts = [r._off for r in reqs]
+        counts = [r.n for r in reqs]
+        Ls = [r.L for r in reqs]
+        cells = [r.cell_size for r in reqs]
+        self._ensure_buffers(s, len(reqs), total)
+        if s.batch is None:
+            s.batch = pbig_amd.Batch.from_instances(self.eng, starts, counts, Ls, cells)
+        else:
+            s.batch.update(starts, counts, Ls, cells, stream=s.stream)
+        if self.upload_ev is not None:
+            s.stream.wait_event(self.upload_ev)
+        s.batch.launch(self.d_pool, s.pairs, s.io, s.res, stream=s.stream)
+        k = len(reqs)
+        with torch.cuda.stream(s.stream):
+            s.h_res.copy_(s.res, non_blocking=True)
+            s.h_io[:k].copy_(s.io[:k], non_blocking=True)
+        s.ev = torch.cuda.Event()
+        s.ev.record(s.stream)
+        s.reqs = reqs
+        s.state = "running"
+        s.bid = self.n_batches
+        self.n_batches += 1
+        self.batch_sizes.append(k)
+
+    def _progress(self, s: _Slot, now: float) -> bool:
+        torch = self.torch
+        if s.state == "running" and s.ev.query():
+            total, err = (int(v) for v in s.h_res.tolist())
+            if err != 0 or total > s.pairs.shape[0]:
+                # workspace regrowth, a crowded instance or too many pairs:
+                # the synchronous entry point handles all three
+                total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream)
+                if total > s.pairs.shape[0]:
+                    cap = total + total // 4
+                    s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
+                    s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+                    total, io = s.batch.check(self.d_pool, s.pairs, stream=s.stream)
+                s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
+            s.total = total
+            if total:
+                with torch.cuda.stream(s.stream):
+                    s.h_pairs[:total].copy_(s.pairs[:total], non_blocking=True)
+            s.ev = torch.cuda.Event()
+            s.ev.record(s.stream)
+            s.state = "copying"
+            return True
+        if s.state == "copying" and s.ev.query():
+            io = s.h_io[:len(s.reqs)].numpy()
+            hp = s.h_pairs.numpy().view(np.uint32)
+            for r, (o, c) in zip(s.reqs, io):
+                r.pairs = hp[int(o):int(o) + int(c)].copy()
+                r.t_done = now
+                r.batch_id = s.bid
+                self.pool.release(r._off, r.n)
+                self.done.append(r)
+            s.reqs = []
+            s.state = "free"
+            return True
+        return False
+
+    def poll(self, now: float) -> bool:
+        progressed = self._admit(now)
+        for s in self.slots:
+            progressed |= self._progress(s, now)
+        if self.admitted:
+            for s in self.slots:
+                if s.state == "free" and self.admitted:
+                    self._launch(s, now)
+                    progressed = True
+        if progressed:
+            progressed |= self._admit(now)
+        return progressed
+
+    def busy(self) -> bool:
+        return bool(self.queue or self.admitted or any(s.state != "free" for s in self.slots))
+
+    def run(self, requests: list[Request], realtime: bool = True) -> dict:
+        """Serve requests arriving at their ``arrival`` times (seconds from
+        the start); returns throughput and latency figures."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+        k = 0
+        while k < len(order) or self.busy():
+            now = time.perf_counter() - t0
+            while k < len(order) and (order[k].arrival <= now or not realtime):
+                if not realtime:
+                    order[k].arrival = now
+                self.submit(order[k])
+                k += 1
+            if not self.poll(now) and k < len(order) and realtime:
+                wait = order[k].arrival - (time.perf_counter() - t0)
+                if wait > 0 and not self.busy():
+                    time.sleep(min(wait, 0.002))
+        t_end = time.perf_counter() - t0
+        return summarize(order, t_end, self.batch_sizes)
+
+
+def summarize(reqs: list[Request], t_end: float, batch_sizes=None) -> dict:
+    lat = np.array([r.latency for r in reqs], dtype=np.float64) * 1e3
+    boxes = sum(r.n for r in reqs)
+    pairs = sum(int(r.pairs.shape[0]) for r in reqs)
+    span = t_end - min(r.arrival for r in reqs)
+    out = {
+        "requests": len(reqs),
+        "boxes": boxes,
+        "pairs": pairs,
+        "makespan_s": span,
+        "boxes_per_s": boxes / span,
+        "pairs_per_s": pairs / span,
+        "requests_per_s": len(reqs) / span,
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "latency_ms_mean": float(lat.mean()),
+    }
+    if batch_sizes:
+        out["batches"] = len(batch_sizes)
+        out["mean_requests_per_batch"] = float(np.mean(batch_sizes))
+    return out
+
+
+def serve_per_instance(eng, requests: list[Request], realtime: bool = True) -> dict:
+    """The same stream served one request at a time by the plain path
+    (upload, synchronous check, pairs back), the reference's calling pattern."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    order = sorted(requests, key=lambda r: r.arrival)
+    cap = 1 << 20
+    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    t0 = time.perf_counter()
+    for r in order:
+        now = time.perf_counter() - t0
+        if realtime and r.arrival > now:
+            time.sleep(r.arrival - now)
+        elif not realtime:
+            r.arrival = now
+        src = r.host if r.host is not None else torch.from_numpy(np.ascontiguousarray(r.boxes))
+        d = src.to(dev, non_blocking=True) if r.n else torch.empty((0, 4), dtype=torch.float32, device=dev)
+        cell = r.cell_size
+        stream = torch.cuda.current_stream()  # the upload's stream
+        n = eng.check_device(d, r.L, cell, d_pairs, stream=stream, allow_overflow=True) if r.n else 0
+        if n > cap:
+            cap = n
+            d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+            n = eng.check_device(d, r.L, cell, d_pairs, stream=stream)
+        r.pairs = d_pairs[:n].cpu().numpy().view(np.uint32)
+        r.t_done = time.perf_counter() - t0
+    t_end = time.perf_counter() - t0
+    return summarize(order, t_end)

@@ -1,0 +1,96 @@
+"""GPU tests of the serving loop: every request's sorted pair set equals the
+oracle's and the plain path's on the request alone, while requests share
+batches, wait for pool space and overlap on the device."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def make_requests(count, seed, nmax, rate):
+    from pbig_amd import workloads
+    from pbig_amd.serve import Request
+
+    reqs = []
+    for q in workloads.serving_requests(count, seed, rate, nmin=1000, nmax=nmax):
+        reqs.append(Request(rid=q["id"], boxes=workloads.request_boxes(q), L=q["L"], arrival=q["t"]))
+    return reqs
+
+
+def check_requests(torch, reqs):
+    import oracle
+    import pbig_amd
+
+    eng = pbig_amd.Engine(0)
+    try:
+        for r in reqs:
+            want = oracle.sort_pairs(oracle.sweep_pairs(r.boxes))
+            got = oracle.sort_pairs(r.pairs)
+            assert np.array_equal(got, want), f"request {r.rid}: {got.shape[0]} vs {want.shape[0]} pairs"
+            d = torch.from_numpy(r.boxes).to("cuda:0")
+            out = torch.empty((max(4 * r.n, 1024), 2), dtype=torch.int32, device="cuda:0")
+            n = eng.check_device(d, r.L, r.cell_size, out)
+            alone = oracle.sort_pairs(out[:n].cpu().numpy().view(np.uint32))
+            assert np.array_equal(alone, got), f"request {r.rid}: alone differs from in the mix"
+    finally:
+        eng.close()
+
+
+def test_serve_all_at_once_small_pool(torch_cuda):
+    """A pool of 150,000 boxes for 24 requests of up to 60,000 boxes: requests
+    wait for space, batches hold several requests, results match alone."""
+    from pbig_amd.serve import Server
+
+    reqs = make_requests(24, 71, 60000, 1000.0)
+    srv = Server(hbm_budget=2 * 150_000 * 16, slots=2, pool_frac=0.5)
+    try:
+        stats = srv.run(reqs, realtime=False)
+    finally:
+        srv.close()
+    assert stats["requests"] == 24
+    assert max(srv.batch_sizes) > 1
+    assert stats["batches"] > 2  # the pool forced several rounds
+    check_requests(torch_cuda, reqs)
+
+
+def test_serve_poisson_realtime(torch_cuda):
+    from pbig_amd.serve import Server
+
+    reqs = make_requests(40, 73, 40000, 2000.0)
+    srv = Server(hbm_budget=1 << 30, slots=3)
+    try:
+        stats = srv.run(reqs, realtime=True)
+    finally:
+        srv.close()
+    assert stats["requests"] == 40
+    assert all(r.t_done >= r.arrival for r in reqs)
+    assert stats["latency_ms_p99"] >= stats["latency_ms_p50"] > 0
+    check_requests(torch_cuda, reqs)
+
+
+def test_serve_per_instance_baseline_matches(torch_cuda):
+    import oracle
+    import pbig_amd
+    from pbig_amd import autotune
+    from pbig_amd.serve import serve_per_instance
+
+    reqs = make_requests(8, 79, 30000, 1000.0)
+    for r in reqs:
+        r.cell_size = float(autotune.cell_sizes([0, r.n], [r.L])[0])
+    eng = pbig_amd.Engine(0)
+    try:
+        stats = serve_per_instance(eng, reqs, realtime=False)
+    finally:
+        eng.close()
+    assert stats["requests"] == 8
+    for r in reqs:
+        assert np.array_equal(oracle.sort_pairs(r.pairs), oracle.sort_pairs(oracle.sweep_pairs(r.boxes)))
