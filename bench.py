@@ -200,68 +200,94 @@ def batched_leg(torch, steps, warmup):
 
 def serving_leg(torch, count, rate, budget_gb):
     """Poisson stream of requests (n log-uniform in [1,000, 1,000,000], 50/50
-    uniform and clustered, seed 5): the continuous-batching server against the
-    same stream served one request at a time by the plain path.  Both upload
-    each request's boxes from pinned host memory and bring its pairs back.
-    Throughput is measured with every request arriving at once (capacity);
-    latency at the stated arrival rate."""
+    uniform and clustered, seed 5): the continuous-batching server against
+    the same stream served one request at a time by the plain path.
+
+    Two settings: "resident" (each request hands over a device box array and
+    its pairs stay in HBM for a consumer on the GPU) and "host" (boxes
+    uploaded from pinned host memory, pairs copied back to pinned host
+    memory).  In both, the consumer reads the pairs in place and keeps their
+    count.  Capacity: every request arrives at once.  Latency: arrivals at
+    the stated rate."""
     import numpy as np
 
     import pbig_amd
     from pbig_amd import autotune, workloads
-    from pbig_amd.serve import Request, Server, serve_per_instance
+    from pbig_amd.serve import Request, Server, count_only, keep_copy, serve_per_instance
 
     table = autotune.load_table()
     spec = workloads.serving_requests(count, workloads.SEEDS["c5"], rate)
-
-    def fresh():
-        out = []
-        for q in spec:
-            b = workloads.request_boxes(q)
-            r = Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
-                        cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table)[0]))
-            r.host = torch.from_numpy(b).pin_memory()
-            out.append(r)
-        return out
-
-    reqs = fresh()
+    dev = torch.device("cuda:0")
+    reqs = []
+    for q in spec:
+        b = workloads.request_boxes(q)
+        r = Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
+                    cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table)[0]))
+        r.host = torch.from_numpy(b).pin_memory()
+        r.device = r.host.to(dev)
+        reqs.append(r)
+    torch.cuda.synchronize()
     total_boxes = sum(r.n for r in reqs)
     res = {"workload": f"{count} requests, n log-uniform [1000, 1000000], 50/50 uniform (L = 65536 sqrt(n/4194304), "
-                       f"sides U[1,32]) and clustered (L = 65536 sqrt(n/2097152)), seed 5; boxes uploaded from "
-                       f"pinned host memory, pairs copied back",
-           "boxes": total_boxes, "hbm_budget_gb": budget_gb, "arrival_rate_per_s": rate}
+                       f"sides U[1,32]) and clustered (L = 65536 sqrt(n/2097152)), seed 5",
+           "boxes": total_boxes, "hbm_budget_gb": budget_gb, "arrival_rate_per_s": rate, "slots": 3}
+    eng = pbig_amd.Engine(0)
+    eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
 
-    def run_server(realtime):
-        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, table=table)
+    def reset():
+        for r, q in zip(reqs, spec):
+            r.arrival = q["t"]
+            r.pairs = None
+
+    def server(mode, realtime, consumer=count_only):
+        reset()
+        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, table=table, results=mode,
+                     consumer=consumer)
         try:
             return srv.run(reqs, realtime=realtime)
         finally:
             srv.close()
 
-    eng = pbig_amd.Engine(0)
-    eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
-    # warm-up of both (plans, workspaces, pinned buffers), then the measured runs
-    run_server(False)
-    serve_per_instance(eng, reqs[:32], realtime=False)
-    cap_srv = run_server(False)
-    pairs_srv = [r.pairs for r in reqs]
-    cap_one = serve_per_instance(eng, reqs, realtime=False)
-    same = all(np.array_equal(np.sort(np.ascontiguousarray(a).view(np.uint64).ravel()),
-                              np.sort(np.ascontiguousarray(r.pairs).view(np.uint64).ravel()))
-               for a, r in zip(pairs_srv, reqs))
-    for r, q in zip(reqs, spec):
-        r.arrival = q["t"]
-    lat_srv = run_server(True)
-    for r, q in zip(reqs, spec):
-        r.arrival = q["t"]
-    lat_one = serve_per_instance(eng, reqs, realtime=True)
+    def one(mode, realtime, consumer=count_only):
+        reset()
+        return serve_per_instance(eng, reqs, realtime=realtime, results=mode, consumer=consumer)
+
+    for mode in ("device", "host"):
+        key = "resident" if mode == "device" else "host"
+        if mode == "host":
+            for r in reqs:
+                r.device = None
+        server(mode, False)  # warm-up: plans, workspaces, pinned buffers
+        one(mode, False)
+        cap_s, cap_o = server(mode, False), one(mode, False)
+        lat_s, lat_o = server(mode, True), one(mode, True)
+        res[key] = {"server_capacity": cap_s, "per_instance_capacity": cap_o,
+                    "speedup_capacity": cap_s["boxes_per_s"] / cap_o["boxes_per_s"],
+                    "server_at_rate": lat_s, "per_instance_at_rate": lat_o}
+        if mode == "device":
+            # every request's pairs alone (plain path) and in the mix (server),
+            # compared by count and order-independent sums over the pairs
+            server(mode, False, consumer=keep_copy)
+            mix = {r.rid: r.pairs for r in reqs}
+            one(mode, False, consumer=keep_copy)
+            ok = True
+            for r in reqs:
+                a, b = mix[r.rid], r.pairs
+                if a.shape[0] != b.shape[0]:
+                    ok = False
+                    break
+                if a.shape[0]:
+                    ka = (a[:, 1].to(torch.int64) << 32) | (a[:, 0].to(torch.int64) & 0xFFFFFFFF)
+                    kb = (b[:, 1].to(torch.int64) << 32) | (b[:, 0].to(torch.int64) & 0xFFFFFFFF)
+                    if not torch.equal(torch.sort(ka).values, torch.sort(kb).values):
+                        ok = False
+                        break
+            res["results_equal_alone_and_in_mix"] = ok
+            for r in reqs:
+                r.pairs = None
+            mix = None
+            torch.cuda.empty_cache()
     eng.close()
-    res.update({
-        "server_capacity": cap_srv, "per_instance_capacity": cap_one,
-        "speedup_capacity": cap_srv["boxes_per_s"] / cap_one["boxes_per_s"],
-        "server_at_rate": lat_srv, "per_instance_at_rate": lat_one,
-        "results_equal_alone_and_in_mix": bool(same),
-    })
     return res
 
 
@@ -289,7 +315,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-instance batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -174,8 +174,9 @@ int pbig_hip_batch_destroy(pbig_hip_batch* batch);
 
 /* Re-plan a batch for a new set of instances, keeping (and only growing) its
  * device workspace: the serving loop's way to reuse a batch slot.  The
- * tables are uploaded on `stream`, which this call synchronises; the batch's
- * earlier work must be complete.  Drops a captured graph. */
+ * tables are uploaded asynchronously on `stream` (launches on any stream are
+ * ordered after the upload); the batch's earlier work must be complete.
+ * Drops a captured graph. */
 int pbig_hip_batch_update(pbig_hip_batch* batch, const pbig_hip_instance* inst, uint32_t n_inst, void* stream);
 
 /* Synchronous batch check.  d_boxes holds n_boxes_total boxes (every

@@ -33,6 +33,11 @@ struct BinnedState {
   std::vector<uint32_t> chunk_inst, bin_inst;
   BatchPlan bp;
   uint64_t generation = 0;  // bumped whenever a device buffer moves
+  // Pinned staging of the tables: uploads are asynchronous; stage_ev marks
+  // the last one (launches on other streams wait for it).
+  unsigned char* h_stage = nullptr;
+  size_t cap_stage = 0;
+  hipEvent_t stage_ev = nullptr;
   // plain-path plan key; capacities of chunk_inst/bin_inst known to be zero
   uint64_t zero_chunks = 0, zero_bins = 0;
   bool key_valid = false;
@@ -180,6 +185,7 @@ int prepare(pbig_hip_ctx* c, uint32_t n, uint32_t n_cells) {
 
 hipEvent_t* next_events(pbig_hip_ctx* c, int path);
 int upload_tables(pbig_hip_ctx* c, BinnedState& st, hipStream_t s);
+int after_upload(pbig_hip_ctx* c, BinnedState& st, hipStream_t s);
 
 int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, uint32_t n_cells,
              uint2* d_out, uint64_t cap, hipStream_t s, uint64_t* n_pairs) {
@@ -374,6 +380,12 @@ int size_state(pbig_hip_ctx* c, BinnedState& st) {
 }
 
 void free_state(BinnedState& st) {
+  if (st.stage_ev) (void)hipEventSynchronize(st.stage_ev);
+  if (st.h_stage) (void)hipHostFree(st.h_stage);
+  if (st.stage_ev) (void)hipEventDestroy(st.stage_ev);
+  st.h_stage = nullptr;
+  st.stage_ev = nullptr;
+  st.cap_stage = 0;
   const BinnedWorkspace& b = st.ws;
   for (void* p : {(void*)b.inst, (void*)b.chunk_inst, (void*)b.bin_inst, (void*)b.hist, (void*)b.off,
                   (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bin_seg, (void*)b.bin_out,
@@ -384,15 +396,33 @@ void free_state(BinnedState& st) {
 
 int upload_tables(pbig_hip_ctx* c, BinnedState& st, hipStream_t s) {
   const BatchPlan& bp = st.bp;
-  if (bp.n_inst)
-    HIP_TRY(c, hipMemcpyAsync(st.ws.inst, st.inst.data(), bp.n_inst * sizeof(InstDesc), hipMemcpyHostToDevice, s));
-  if (bp.n_chunks)
-    HIP_TRY(c, hipMemcpyAsync(st.ws.chunk_inst, st.chunk_inst.data(), bp.n_chunks * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
-  if (bp.n_bins)
-    HIP_TRY(c, hipMemcpyAsync(st.ws.bin_inst, st.bin_inst.data(), bp.n_bins * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
+  const size_t b_inst = bp.n_inst * sizeof(InstDesc), b_chunks = bp.n_chunks * sizeof(uint32_t),
+               b_bins = bp.n_bins * sizeof(uint32_t);
+  const size_t bytes = b_inst + b_chunks + b_bins;
+  if (!st.stage_ev) HIP_TRY(c, hipEventCreateWithFlags(&st.stage_ev, hipEventDisableTiming));
+  else HIP_TRY(c, hipEventSynchronize(st.stage_ev));  // the staging buffer is free again
+  if (st.cap_stage < bytes) {
+    if (st.h_stage) HIP_TRY(c, hipHostFree(st.h_stage));
+    st.h_stage = nullptr;
+    st.cap_stage = 0;
+    HIP_TRY(c, hipHostMalloc((void**)&st.h_stage, bytes + bytes / 2, hipHostMallocDefault));
+    st.cap_stage = bytes + bytes / 2;
+  }
+  unsigned char* p = st.h_stage;
+  if (b_inst) memcpy(p, st.inst.data(), b_inst);
+  if (b_chunks) memcpy(p + b_inst, st.chunk_inst.data(), b_chunks);
+  if (b_bins) memcpy(p + b_inst + b_chunks, st.bin_inst.data(), b_bins);
+  if (b_inst) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, p, b_inst, hipMemcpyHostToDevice, s));
+  if (b_chunks) HIP_TRY(c, hipMemcpyAsync(st.ws.chunk_inst, p + b_inst, b_chunks, hipMemcpyHostToDevice, s));
+  if (b_bins) HIP_TRY(c, hipMemcpyAsync(st.ws.bin_inst, p + b_inst + b_chunks, b_bins, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipEventRecord(st.stage_ev, s));
+  return PBIG_OK;
+}
+
+// Order stream s after the last table upload (which may have gone on another
+// stream).
+int after_upload(pbig_hip_ctx* c, BinnedState& st, hipStream_t s) {
+  if (st.stage_ev) HIP_TRY(c, hipStreamWaitEvent(s, st.stage_ev, 0));
   return PBIG_OK;
 }
 
@@ -452,6 +482,7 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
 int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
                    uint64_t* d_inst_out, hipStream_t s, Meta* m) {
   int rc;
+  if ((rc = after_upload(c, st, s))) return rc;
   for (int attempt = 0; attempt < 3; ++attempt) {
     HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, st.bp, d_out, cap, d_inst_out, s,
                                    next_events(c, PBIG_PATH_BINNED)));
@@ -982,6 +1013,7 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
     if (g) return fail(c, PBIG_E_ARG, "the batch has instances only the synchronous entry point can run");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = pick_stream(c, stream);
+  if ((rc = after_upload(c, b->st, s))) return rc;
   HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
   if (d_result)
@@ -999,6 +1031,7 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
     if (g) return fail(c, PBIG_E_ARG, "the batch has instances only the synchronous entry point can run");
   HIP_TRY(c, hipSetDevice(c->device));
   destroy_graph(b);
+  if (b->st.stage_ev) HIP_TRY(c, hipEventSynchronize(b->st.stage_ev));  // tables in place before capture
   hipStream_t s = b->capture_stream;
   HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap,

@@ -7,23 +7,28 @@ server keeps request box sets resident in an HBM pool and runs many requests
 per launch:
 
 * Admission.  A request is admitted when its boxes fit in the pool (first-fit
-  over a free list) and in the HBM budget; its boxes are then copied into the
-  pool on a copy stream.  Requests are admitted in arrival order.
+  over a free list, the pool being the box share of the HBM budget); its
+  boxes are then copied into the pool on a copy stream, from host memory or
+  from the device array the client handed over.  Requests are admitted in
+  arrival order.
 * Continuous batching.  Each of ``slots`` batch slots owns a pbig_hip_batch
-  (workspace reused across batches), a stream and pinned result buffers.
-  Whenever a slot is free, every admitted request whose upload has been
-  issued forms the slot's next batch: the slot's stream waits for the
-  uploads, the batch is re-planned and launched, and the result words, the
-  per-request (offset, count) table and then the pairs come back by
-  asynchronous copies.  Requests that arrive while batches run are admitted
-  as pool space frees up and go into the next free slot, so uploads, kernels
-  of several batches and result copies overlap on the device.
-* Completion.  A request's pairs are delivered when its batch's copies are
-  done; its pool space is freed at once.  Latency = delivery - arrival.
+  (workspace reused across batches and bounded by the slot's share of the
+  budget), a stream and result buffers.  Whenever a slot is free, the
+  admitted requests form its next batch: the slot's stream waits for their
+  uploads, the batch is re-planned and launched, and the result words and
+  the per-request (offset, count) table come back by asynchronous copies.
+  Requests that arrive while batches run are admitted as pool space frees up
+  and go into the next free slot, so uploads, the kernels of several batches
+  and result copies overlap on the device.
+* Completion.  With ``results="device"`` a request's pairs stay in HBM (a
+  view of the slot's pair buffer, valid while the consumer runs); with
+  ``results="host"`` they are copied to pinned host memory first.  The
+  consumer callback receives them; its default keeps a copy in the request.
+  The request's pool space is freed at once.  Latency = delivery - arrival.
 
 Each request's pair set does not depend on what else is in its batch: the
 instance is planned and tested alone inside the batch (its own grid, bins and
-output run), which the tests check against the request run alone.
+output run); the tests check this against the request run alone.
 """
 from __future__ import annotations
 
@@ -39,27 +44,50 @@ from pbig_amd import autotune
 @dataclass
 class Request:
     """One instance: boxes (n, 4) float32 {xmin,xmax,ymin,ymax}, domain side L.
-    ``host`` may be a pinned torch tensor of the boxes (faster uploads)."""
+    The boxes come from ``device`` (a device tensor), else ``host`` (a pinned
+    torch tensor), else ``boxes`` (numpy)."""
     rid: int
-    boxes: np.ndarray
+    boxes: np.ndarray | None
     L: float
     arrival: float = 0.0
     cell_size: float = 0.0           # 0 = the autotuner's table
     host: object = None
+    device: object = None
     # filled by the server
-    pairs: np.ndarray | None = None
+    pairs: object = None             # numpy (host results) or device tensor
+    n_pairs: int = -1
     t_admit: float = 0.0
+    t_launch: float = 0.0
+    t_ready: float = 0.0             # kernels done, result words on the host
     t_done: float = 0.0
     batch_id: int = -1
     _off: int = -1
 
     @property
     def n(self) -> int:
-        return int(self.boxes.shape[0])
+        src = self.device if self.device is not None else (self.host if self.host is not None else self.boxes)
+        return int(src.shape[0])
 
     @property
     def latency(self) -> float:
         return self.t_done - self.arrival
+
+
+def keep_copy(req: Request, pairs, stream):
+    """Default consumer: keep the pairs in the request (host: numpy copy;
+    device: a device copy ordered on the slot's stream)."""
+    if isinstance(pairs, np.ndarray):
+        req.pairs = pairs.copy()
+    else:
+        import torch
+
+        with torch.cuda.stream(stream):
+            req.pairs = pairs.clone()
+
+
+def count_only(req: Request, pairs, stream):
+    """Consumer that keeps only the number of pairs (already in n_pairs)."""
+    req.pairs = None
 
 
 class Pool:
@@ -105,7 +133,7 @@ class _Slot:
     res: object = None               # device (2,) int64
     h_res: object = None             # pinned
     h_io: object = None              # pinned
-    h_pairs: object = None           # pinned
+    h_pairs: object = None           # pinned (host results)
     ev: object = None
     state: str = "free"              # free | running | copying
     reqs: list = field(default_factory=list)
@@ -117,29 +145,35 @@ class Server:
     """Continuous-batching server on one GPU (see the module docstring).
 
     hbm_budget: bytes of device memory the server may hold: the box pool
-    (pool_frac of the budget) plus every slot's workspace and pair buffers.
+    (pool_frac of the budget) plus the slots' workspaces and pair buffers
+    (each slot's batches are capped at slot budget / bytes_per_box boxes; a
+    single larger request still runs, alone).
     """
 
     def __init__(self, device: int = 0, hbm_budget: int = 8 << 30, slots: int = 2, pool_frac: float = 0.5,
-                 table: dict | None = None, bytes_per_box: int = 96):
+                 table: dict | None = None, bytes_per_box: int = 96, results: str = "host", consumer=None,
+                 max_batch_boxes: int | None = None):
         import torch
 
+        if results not in ("host", "device"):
+            raise ValueError("results must be 'host' or 'device'")
         self.torch = torch
         self.dev = torch.device(f"cuda:{device}")
-        torch.zeros(1, device=self.dev)
+        torch.zeros(1, device=self.dev)  # torch's HIP runtime before the engine's context
         self.table = table or autotune.load_table()
+        self.results = results
+        self.consumer = consumer or keep_copy
         self.eng = pbig_amd.Engine(device)
         self.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
         pool_boxes = int(hbm_budget * pool_frac) // 16
         self.pool = Pool(pool_boxes)
         self.d_pool = torch.empty((pool_boxes, 4), dtype=torch.float32, device=self.dev)
-        # per-slot budget -> largest batch (boxes): bucket, regions, pairs
         self.slot_budget = int(hbm_budget * (1 - pool_frac)) // max(slots, 1)
-        self.max_batch_boxes = max(self.slot_budget // bytes_per_box, 1)
+        self.max_batch_boxes = max_batch_boxes or max(self.slot_budget // bytes_per_box, 1)
         self.copy_stream = torch.cuda.Stream(device=self.dev)
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(slots)]
         self.queue: list[Request] = []       # arrived, not admitted
-        self.admitted: list[Request] = []    # uploaded (or upload issued), not yet batched
+        self.admitted: list[Request] = []    # upload issued, not yet batched
         self.upload_ev = None
         self.done: list[Request] = []
         self.n_batches = 0
@@ -174,7 +208,12 @@ class Server:
             r._off = off
             r.t_admit = now
             if r.n:
-                src = r.host if r.host is not None else torch.from_numpy(np.ascontiguousarray(r.boxes))
+                if r.device is not None:
+                    src = r.device
+                elif r.host is not None:
+                    src = r.host
+                else:
+                    src = torch.from_numpy(np.ascontiguousarray(r.boxes))
                 with torch.cuda.stream(self.copy_stream):
                     self.d_pool[off:off + r.n].copy_(src, non_blocking=True)
             self.admitted.append(r)
@@ -189,7 +228,8 @@ class Server:
         cap = max(4096, 2 * n_boxes)  # two pairs per box before the first regrowth
         if s.pairs is None or s.pairs.shape[0] < cap:
             s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
-            s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+            if self.results == "host":
+                s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
         if s.io is None or s.io.shape[0] < k:
             kk = max(k, 256)
             s.io = torch.empty((kk, 2), dtype=torch.int64, device=self.dev)
@@ -226,16 +266,32 @@ class Server:
             s.h_io[:k].copy_(s.io[:k], non_blocking=True)
         s.ev = torch.cuda.Event()
         s.ev.record(s.stream)
+        for r in reqs:
+            r.t_launch = now
         s.reqs = reqs
         s.state = "running"
         s.bid = self.n_batches
         self.n_batches += 1
         self.batch_sizes.append(k)
 
+    def _deliver(self, s: _Slot, pairs, now: float):
+        io = s.h_io[:len(s.reqs)].numpy()
+        for r, (o, c) in zip(s.reqs, io):
+            r.n_pairs = int(c)
+            self.consumer(r, pairs[int(o):int(o) + int(c)], s.stream)
+            r.t_done = now
+            r.batch_id = s.bid
+            self.pool.release(r._off, r.n)
+            self.done.append(r)
+        s.reqs = []
+        s.state = "free"
+
     def _progress(self, s: _Slot, now: float) -> bool:
         torch = self.torch
         if s.state == "running" and s.ev.query():
             total, err = (int(v) for v in s.h_res.tolist())
+            for r in s.reqs:
+                r.t_ready = now
             if err != 0 or total > s.pairs.shape[0]:
                 # workspace regrowth, a crowded instance or too many pairs:
                 # the synchronous entry point handles all three
@@ -243,10 +299,14 @@ class Server:
                 if total > s.pairs.shape[0]:
                     cap = total + total // 4
                     s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
-                    s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+                    if self.results == "host":
+                        s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
                     total, io = s.batch.check(self.d_pool, s.pairs, stream=s.stream)
                 s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
             s.total = total
+            if self.results == "device":
+                self._deliver(s, s.pairs, now)
+                return True
             if total:
                 with torch.cuda.stream(s.stream):
                     s.h_pairs[:total].copy_(s.pairs[:total], non_blocking=True)
@@ -255,16 +315,7 @@ class Server:
             s.state = "copying"
             return True
         if s.state == "copying" and s.ev.query():
-            io = s.h_io[:len(s.reqs)].numpy()
-            hp = s.h_pairs.numpy().view(np.uint32)
-            for r, (o, c) in zip(s.reqs, io):
-                r.pairs = hp[int(o):int(o) + int(c)].copy()
-                r.t_done = now
-                r.batch_id = s.bid
-                self.pool.release(r._off, r.n)
-                self.done.append(r)
-            s.reqs = []
-            s.state = "free"
+            self._deliver(s, s.h_pairs.numpy().view(np.uint32), now)
             return True
         return False
 
@@ -286,7 +337,8 @@ class Server:
 
     def run(self, requests: list[Request], realtime: bool = True) -> dict:
         """Serve requests arriving at their ``arrival`` times (seconds from
-        the start); returns throughput and latency figures."""
+        the start; with realtime=False all arrive at once); returns
+        throughput and latency figures."""
         order = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         k = 0
@@ -297,10 +349,10 @@ class Server:
                     order[k].arrival = now
                 self.submit(order[k])
                 k += 1
-            if not self.poll(now) and k < len(order) and realtime:
+            if not self.poll(time.perf_counter() - t0) and k < len(order) and realtime and not self.busy():
                 wait = order[k].arrival - (time.perf_counter() - t0)
-                if wait > 0 and not self.busy():
-                    time.sleep(min(wait, 0.002))
+                if wait > 2e-4:
+                    time.sleep(min(wait - 1e-4, 0.002))
         t_end = time.perf_counter() - t0
         return summarize(order, t_end, self.batch_sizes)
 
@@ -308,7 +360,7 @@ class Server:
 def summarize(reqs: list[Request], t_end: float, batch_sizes=None) -> dict:
     lat = np.array([r.latency for r in reqs], dtype=np.float64) * 1e3
     boxes = sum(r.n for r in reqs)
-    pairs = sum(int(r.pairs.shape[0]) for r in reqs)
+    pairs = sum(r.n_pairs for r in reqs)
     span = t_end - min(r.arrival for r in reqs)
     out = {
         "requests": len(reqs),
@@ -328,32 +380,49 @@ def summarize(reqs: list[Request], t_end: float, batch_sizes=None) -> dict:
     return out
 
 
-def serve_per_instance(eng, requests: list[Request], realtime: bool = True) -> dict:
-    """The same stream served one request at a time by the plain path
-    (upload, synchronous check, pairs back), the reference's calling pattern."""
+def serve_per_instance(eng, requests: list[Request], realtime: bool = True, results: str = "host",
+                       consumer=None) -> dict:
+    """The same stream served one request at a time by the plain path, the
+    reference's calling pattern: (upload,) synchronous check, pairs back
+    (to pinned host memory, or left on the device)."""
     import torch
 
+    consumer = consumer or keep_copy
     dev = torch.device("cuda:0")
     order = sorted(requests, key=lambda r: r.arrival)
     cap = 1 << 20
     d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
+    h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True) if results == "host" else None
+    stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
     for r in order:
         now = time.perf_counter() - t0
         if realtime and r.arrival > now:
-            time.sleep(r.arrival - now)
+            while time.perf_counter() - t0 < r.arrival:
+                pass
         elif not realtime:
             r.arrival = now
-        src = r.host if r.host is not None else torch.from_numpy(np.ascontiguousarray(r.boxes))
-        d = src.to(dev, non_blocking=True) if r.n else torch.empty((0, 4), dtype=torch.float32, device=dev)
-        cell = r.cell_size
-        stream = torch.cuda.current_stream()  # the upload's stream
-        n = eng.check_device(d, r.L, cell, d_pairs, stream=stream, allow_overflow=True) if r.n else 0
+        if r.device is not None:
+            d = r.device
+        else:
+            src = r.host if r.host is not None else torch.from_numpy(np.ascontiguousarray(r.boxes))
+            d = src.to(dev, non_blocking=True)
+        n = eng.check_device(d, r.L, r.cell_size, d_pairs, stream=stream, allow_overflow=True) if r.n else 0
         if n > cap:
-            cap = n
+            cap = n + n // 4
             d_pairs = torch.empty((cap, 2), dtype=torch.int32, device=dev)
-            n = eng.check_device(d, r.L, cell, d_pairs, stream=stream)
-        r.pairs = d_pairs[:n].cpu().numpy().view(np.uint32)
+            if results == "host":
+                h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+            n = eng.check_device(d, r.L, r.cell_size, d_pairs, stream=stream)
+        r.n_pairs = n
+        if results == "host":
+            if n:
+                h_pairs[:n].copy_(d_pairs[:n], non_blocking=True)
+                stream.synchronize()
+            consumer(r, h_pairs.numpy().view(np.uint32)[:n], stream)
+        else:
+            consumer(r, d_pairs[:n], stream)
         r.t_done = time.perf_counter() - t0
+    torch.cuda.synchronize()
     t_end = time.perf_counter() - t0
     return summarize(order, t_end)

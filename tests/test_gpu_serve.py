@@ -34,7 +34,8 @@ def check_requests(torch, reqs):
     try:
         for r in reqs:
             want = oracle.sort_pairs(oracle.sweep_pairs(r.boxes))
-            got = oracle.sort_pairs(r.pairs)
+            p = r.pairs.cpu().numpy().view(np.uint32) if hasattr(r.pairs, "cpu") else r.pairs
+            got = oracle.sort_pairs(p)
             assert np.array_equal(got, want), f"request {r.rid}: {got.shape[0]} vs {want.shape[0]} pairs"
             d = torch.from_numpy(r.boxes).to("cuda:0")
             out = torch.empty((max(4 * r.n, 1024), 2), dtype=torch.int32, device="cuda:0")
@@ -94,3 +95,20 @@ def test_serve_per_instance_baseline_matches(torch_cuda):
     assert stats["requests"] == 8
     for r in reqs:
         assert np.array_equal(oracle.sort_pairs(r.pairs), oracle.sort_pairs(oracle.sweep_pairs(r.boxes)))
+
+
+def test_serve_device_resident(torch_cuda):
+    """Requests hand over device box arrays and get device pair lists."""
+    from pbig_amd.serve import Server
+
+    reqs = make_requests(30, 83, 50000, 5000.0)
+    for r in reqs:
+        r.device = torch_cuda.from_numpy(r.boxes).to("cuda:0")
+    srv = Server(hbm_budget=2 * 200_000 * 16, slots=3, results="device")
+    try:
+        stats = srv.run(reqs, realtime=True)
+    finally:
+        srv.close()
+    assert stats["requests"] == 30
+    assert all(r.pairs.shape[0] == r.n_pairs for r in reqs)
+    check_requests(torch_cuda, reqs)
