@@ -19,8 +19,17 @@ constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent te
 constexpr uint32_t kBatch = 16;              // boxes per thread loaded before use (hist)
 constexpr uint32_t kScatterChunk = 8192;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
+// Overfull list entries: the bin, with this bit when a single fine cell
+// exceeded kMaxFine (re-binning cannot help; the instance needs the general
+// path) rather than the bin exceeding kMaxCoarse memberships.
+constexpr uint32_t kOverfullCell = 0x80000000u;
 
 // Grid and work layout of one instance (device table, one entry per instance).
+// An instance either owns its boxes (box ids = index - box0) or is a heavy
+// coarse bin of another launch re-binned on its own ("virtual" instance: its
+// boxes are that launch's bin-ordered copies, ids come from the ids array,
+// and its fine cells are the parent's cells [fx0, fx0 + span_x) x [fy0, fy0 +
+// span_y), in bins of k < parent k cells).
 struct InstDesc {
   uint64_t box0;     // first box of the instance in the box array
   uint64_t hist0;    // first entry of its histogram rows (nchunks x (nb + 1))
@@ -36,11 +45,15 @@ struct InstDesc {
   uint32_t nb;       // coarse bins = nbx * nbx
   uint32_t nbx;      // coarse bins per row
   uint32_t k;        // fine cells per coarse bin side (1..kMaxK)
-  uint32_t nf;       // fine cells per row = nbx * k
+  uint32_t nf;       // fine cells per row of the whole grid
   float inv_f;       // 1 / fine cell side (rounded once on the host)
+  uint32_t fx0, fy0; // first fine column / row of the instance's region
+  uint32_t span_x;   // fine columns of the region (nf for an owned instance)
+  uint32_t span_y;   // fine rows of the region
+  uint32_t parent;   // owning instance one level up (virtual instances)
   uint32_t pad;
 };
-static_assert(sizeof(InstDesc) == 96, "InstDesc layout");
+static_assert(sizeof(InstDesc) == 112, "InstDesc layout");
 
 // Totals of a batch (host side).
 struct BatchPlan {
@@ -80,8 +93,16 @@ hipError_t binned_init();
 // tables (uploaded beforehand).  Pairs land in `out` instance by instance;
 // inst_out (optional, device) receives (offset, count) per instance.
 // events (optional): kBinnedKernels + 1.
-hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, uint2* out,
-                         uint64_t cap, uint64_t* inst_out, hipStream_t s, hipEvent_t* events = nullptr);
+// ids_in: null for instances that own their boxes; the ids of the boxes
+// array (a parent launch's bin-ordered ids) for virtual instances.
+hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
+                         const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
+                         hipEvent_t* events = nullptr);
+
+// Only the last two kernels (pair scan and output), after the instance table
+// changed its pairs0 / pairs_extra fields: the test's regions are reused.
+hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
+                                uint64_t* inst_out, hipStream_t s);
 
 }  // namespace pbig
 

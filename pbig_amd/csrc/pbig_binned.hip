@@ -61,12 +61,20 @@ __device__ __forceinline__ uint32_t div_k(uint32_t fine, uint64_t magic) {
   return (uint32_t)(((uint64_t)fine * magic) >> 32);
 }
 
+// Fine index relative to the instance's region, clamped into it (a no-op
+// for an owned instance: fx0 = 0, span = nf).  Bins wholly outside a
+// non-square region receive no memberships.
+__device__ __forceinline__ uint32_t region_of(uint32_t f, uint32_t f0, uint32_t span) {
+  const uint32_t hi = f0 + span - 1u;
+  return (f < f0 ? f0 : (f > hi ? hi : f)) - f0;
+}
+
 __device__ __forceinline__ BinRange bin_range(const float4& b, const InstDesc& p) {
   BinRange r;
-  r.x0 = div_k(fine_of(b.x, p.inv_f, p.nf), p.k_magic);
-  r.x1 = div_k(fine_of(b.y, p.inv_f, p.nf), p.k_magic);
-  r.y0 = div_k(fine_of(b.z, p.inv_f, p.nf), p.k_magic);
-  r.y1 = div_k(fine_of(b.w, p.inv_f, p.nf), p.k_magic);
+  r.x0 = div_k(region_of(fine_of(b.x, p.inv_f, p.nf), p.fx0, p.span_x), p.k_magic);
+  r.x1 = div_k(region_of(fine_of(b.y, p.inv_f, p.nf), p.fx0, p.span_x), p.k_magic);
+  r.y0 = div_k(region_of(fine_of(b.z, p.inv_f, p.nf), p.fy0, p.span_y), p.k_magic);
+  r.y1 = div_k(region_of(fine_of(b.w, p.inv_f, p.nf), p.fy0, p.span_y), p.k_magic);
   return r;
 }
 
@@ -246,7 +254,11 @@ __device__ __forceinline__ uint32_t pack_range(const BinRange& r) {
   return r.x0 | (r.x1 << 8) | (r.y0 << 16) | (r.y1 << 24);
 }
 
+// kIds: the boxes' ids come from ids_in (virtual instances), else they are
+// the index inside the instance.
+template <bool kIds>
 __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* __restrict__ boxes,
+                                                                 const uint32_t* __restrict__ ids_in,
                                                                  const InstDesc* __restrict__ inst,
                                                                  const uint32_t* __restrict__ chunk_inst,
                                                                  uint32_t n_chunks,
@@ -361,7 +373,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       if (pr == 0xFFFFFFFFu) continue;
       const uint32_t x0 = pr & 0xFF, x1 = (pr >> 8) & 0xFF, y0 = (pr >> 16) & 0xFF, y1 = pr >> 24;
       if (y1 * I.nbx + x1 < b_lo || y0 * I.nbx + x0 >= b_hi) continue;
-      const uint32_t i = (uint32_t)(start - I.box0) + tid + k * kScatterThreads;  // id inside the instance
+      uint32_t i = (uint32_t)(start - I.box0) + tid + k * kScatterThreads;  // id inside the instance
+      if (kIds) i = ids_in[start + tid + k * kScatterThreads];
       for (uint32_t cy = y0; cy <= y1; ++cy)
         for (uint32_t cx = x0; cx <= x1; ++cx) {
           const uint32_t b = cy * I.nbx + cx;
@@ -429,7 +442,7 @@ struct TestLds {
   uint32_t start[kMaxCellsPerBin + 1];  // exclusive scan of cnt (start[kk] = total)
   uint2 stage[kTestWaves][kBinStage];   // per-wave pair staging
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
-  uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, k, nf) of each bin
+  uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, kx | ky << 16, nf) of each bin
   float bininv[kMaxBinsPerGroup];  // inv_f of each bin's instance
   uint32_t wsum[kTestWaves];
   uint32_t out_n;                  // pairs appended to this workgroup's region
@@ -447,10 +460,11 @@ __device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn
   sn = 0;
 }
 
-// Fine grid of one bin's instance.
+// Fine grid of one bin's instance and the bin's extent in fine cells (bins
+// of a virtual instance may be cut short at its region's edge).
 struct FineGrid {
   float inv_f;
-  uint32_t nf, k;
+  uint32_t nf, kx, ky;
 };
 
 // Fine-cell range of a box clipped to the bin whose first fine cell is (gx, gy).
@@ -465,8 +479,8 @@ __device__ __forceinline__ FineRange fine_range(const float4& q, const FineGrid&
   const uint32_t fx1 = fine_of(q.y, p.inv_f, p.nf), fy1 = fine_of(q.w, p.inv_f, p.nf);
   r.x0 = r.fx0 > gx ? r.fx0 - gx : 0u;
   r.y0 = r.fy0 > gy ? r.fy0 - gy : 0u;
-  r.x1 = fx1 - gx < p.k ? fx1 - gx : p.k - 1;
-  r.y1 = fy1 - gy < p.k ? fy1 - gy : p.k - 1;
+  r.x1 = fx1 - gx < p.kx ? fx1 - gx : p.kx - 1;
+  r.y1 = fy1 - gy < p.ky ? fy1 - gy : p.ky - 1;
   return r;
 }
 
@@ -476,12 +490,12 @@ __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint3
                                               uint32_t gx, uint32_t gy, uint32_t c_lo, uint32_t c_hi,
                                               uint32_t base) {
   const FineRange r = fine_range(q, p, gx, gy);
-  const uint32_t row_lo = c_lo / p.k, row_hi = (c_hi - 1) / p.k;
+  const uint32_t row_lo = c_lo / p.kx, row_hi = (c_hi - 1) / p.kx;
   const uint32_t y0 = r.y0 > row_lo ? r.y0 : row_lo, y1 = r.y1 < row_hi ? r.y1 : row_hi;
   for (uint32_t cy = y0; cy <= y1; ++cy) {
     const uint32_t fy = (gy + cy == r.fy0) ? kFlagY : 0u;
     for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
-      const uint32_t c = cy * p.k + cx;
+      const uint32_t c = cy * p.kx + cx;
       if (c < c_lo || c >= c_hi) continue;
       const uint32_t pos = atomicAdd(&L.start[c], 1u) - base;
       L.box[pos] = q;
@@ -527,8 +541,11 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
         const uint32_t gb = blockIdx.x + (t + u) * gridDim.x;
         const InstDesc I = inst[bin_inst[gb]];
         const uint32_t lb = gb - I.bin0;
+        const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
+        const uint32_t ex = I.fx0 + I.span_x - gx, ey = I.fy0 + I.span_y - gy;  // cells left in the region
+        const uint32_t kx = I.k < ex ? I.k : ex, ky = I.k < ey ? I.k : ey;
         L.binfo[u] = make_uint2(bin_start[gb], total[gb]);
-        L.bingrid[u] = make_uint4((lb % I.nbx) * I.k, (lb / I.nbx) * I.k, I.k, I.nf);
+        L.bingrid[u] = make_uint4(gx, gy, kx | (ky << 16), I.nf);
         L.bininv[u] = I.inv_f;
       }
       __syncthreads();
@@ -546,8 +563,8 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
     const uint32_t m = L.binfo[slot].y;
     const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
     const uint4 bg = L.bingrid[slot];
-    const uint32_t gx = bg.x, gy = bg.y, K = bg.z, kk = K * K;
-    const FineGrid p{L.bininv[slot], bg.w, K};
+    const uint32_t gx = bg.x, gy = bg.y, Kx = bg.z & 0xFFFFu, Ky = bg.z >> 16, kk = Kx * Ky;
+    const FineGrid p{L.bininv[slot], bg.w, Kx, Ky};
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
@@ -560,7 +577,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
           L.id[j] = ri[k];
           const FineRange r = fine_range(rb[k], p, gx, gy);
           for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-            for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * K + cx], 1u);
+            for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * Kx + cx], 1u);
         }
       }
       __syncthreads();
@@ -609,7 +626,9 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
         const uint32_t c_hi = L.strip_hi;
         if (c_hi == c_lo) {  // one fine cell alone exceeds the LDS buffer
           if (tid == 0) {
-            overfull[atomicAdd(&meta->max_per_cell, 1u)] = b;
+            // drop the bin's pairs so far: the host reruns the whole bin
+            L.out_n = seg0;
+            overfull[atomicAdd(&meta->max_per_cell, 1u)] = b | kOverfullCell;
             atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrOverfull);
           }
           break;
@@ -801,12 +820,25 @@ hipError_t binned_init() {
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(TestLds));
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+  e = hipFuncSetAttribute((const void*)k_bin_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)sizeof(ScatterLds));
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_bin_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)sizeof(ScatterLds));
 }
 
-hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, uint2* out,
-                         uint64_t cap, uint64_t* inst_out, hipStream_t s, hipEvent_t* ev) {
+hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
+                                uint64_t* inst_out, hipStream_t s) {
+  const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;
+  k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
+  k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
+                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
+                         const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
+                         hipEvent_t* ev) {
   hipError_t e;
   // Every kernel tolerates an empty batch (no chunks, no bins): grids of at
   // least one block whose blocks find no work.
@@ -822,9 +854,14 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const B
   PBIG_MARK(2)
   k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, ws.meta);
   PBIG_MARK(3)
-  k_bin_scatter<<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks,
-                                                                ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
-                                                                ws.cap_bucket, ws.meta);
+  if (ids_in)
+    k_bin_scatter<true><<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(
+        boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
+        ws.cap_bucket, ws.meta);
+  else
+    k_bin_scatter<false><<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(
+        boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
+        ws.cap_bucket, ws.meta);
   PBIG_MARK(4)
   k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,

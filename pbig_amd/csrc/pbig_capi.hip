@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,16 @@ struct BinnedState {
   uint32_t key_n = 0, key_target = 0;
   double key_range = 0.0;
   float key_cell = 0.0f;
+  // Next level: the coarse bins that did not fit in LDS, re-binned as
+  // virtual instances; its pairs and (offset, count) table.
+  std::unique_ptr<BinnedState> sub;
+  uint2* sub_out = nullptr;
+  uint64_t cap_sub_out = 0;
+  uint64_t* sub_io = nullptr;
+  uint64_t cap_sub_io = 0;
+  uint64_t* io_buf = nullptr;      // (offset, count) per instance when the caller gives none
+  uint64_t cap_io_buf = 0;
+  bool own_meta = false;
 };
 
 struct pbig_hip_ctx {
@@ -256,6 +267,39 @@ bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, In
   if (nw > 8) nw = (nw + 7) / 8 * 8;
   d->chunk = (uint32_t)std::min<uint64_t>(pbig::kScatterChunk, ((uint64_t)n + nw - 1) / nw);
   d->nchunks = (uint32_t)(((uint64_t)n + d->chunk - 1) / d->chunk);
+  d->fx0 = d->fy0 = 0;
+  d->span_x = d->span_y = d->nf;
+  return true;
+}
+
+// Plan of a virtual instance: coarse bin lb of instance P, holding m
+// memberships at box0 of P's launch's bin-ordered arrays, re-binned into
+// bins of about target memberships.  False when the bin cannot be split
+// (a single fine cell).
+bool plan_virtual(const InstDesc& P, uint32_t parent, uint32_t lb, uint32_t m, uint64_t box0, uint32_t target,
+                  InstDesc* v) {
+  *v = InstDesc{};
+  v->n = m;
+  v->box0 = box0;
+  v->parent = parent;
+  v->inv_f = P.inv_f;
+  v->nf = P.nf;
+  v->fx0 = P.fx0 + (lb % P.nbx) * P.k;
+  v->fy0 = P.fy0 + (lb / P.nbx) * P.k;
+  v->span_x = std::min<uint32_t>(P.k, P.fx0 + P.span_x - v->fx0);
+  v->span_y = std::min<uint32_t>(P.k, P.fy0 + P.span_y - v->fy0);
+  const uint32_t side = std::max(v->span_x, v->span_y);
+  if (side <= 1) return false;
+  double kd = std::round(side * std::sqrt((double)std::max<uint32_t>(target, 16) / std::max<uint32_t>(m, 1)));
+  const uint32_t k = (uint32_t)std::min<double>(std::max<double>(kd, 1.0), side - 1);
+  v->k = k;
+  v->k_magic = ((1ull << 32) + k - 1) / k;
+  v->nbx = (side + k - 1) / k;
+  v->nb = v->nbx * v->nbx;
+  uint64_t nw = ((uint64_t)m + pbig::kScatterChunk - 1) / pbig::kScatterChunk;
+  if (nw > 8) nw = (nw + 7) / 8 * 8;
+  v->chunk = (uint32_t)std::min<uint64_t>(pbig::kScatterChunk, ((uint64_t)m + nw - 1) / nw);
+  v->nchunks = m ? (uint32_t)(((uint64_t)m + v->chunk - 1) / v->chunk) : 0u;
   return true;
 }
 
@@ -386,11 +430,22 @@ void free_state(BinnedState& st) {
   st.h_stage = nullptr;
   st.stage_ev = nullptr;
   st.cap_stage = 0;
+  if (st.sub) {
+    free_state(*st.sub);
+    st.sub.reset();
+  }
+  for (void* p : {(void*)st.sub_out, (void*)st.sub_io, (void*)st.io_buf})
+    if (p) (void)hipFree(p);
+  st.sub_out = nullptr;
+  st.sub_io = nullptr;
+  st.io_buf = nullptr;
+  st.cap_sub_out = st.cap_sub_io = st.cap_io_buf = 0;
   const BinnedWorkspace& b = st.ws;
   for (void* p : {(void*)b.inst, (void*)b.chunk_inst, (void*)b.bin_inst, (void*)b.hist, (void*)b.off,
                   (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bin_seg, (void*)b.bin_out,
                   (void*)b.scratch, (void*)b.bbox, (void*)b.bid})
     if (p) (void)hipFree(p);
+  if (st.own_meta && b.meta) (void)hipFree(b.meta);
   st.ws = BinnedWorkspace{};
 }
 
@@ -479,12 +534,12 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
 // Run a binned state synchronously, regrowing the workspace when the bucket
 // or a pair region overflowed.  On success *m holds the result words; the
 // caller handles kErrOverfull.
-int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
-                   uint64_t* d_inst_out, hipStream_t s, Meta* m) {
+int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, const uint32_t* ids, uint2* d_out,
+                   uint64_t cap, uint64_t* d_inst_out, hipStream_t s, Meta* m) {
   int rc;
   if ((rc = after_upload(c, st, s))) return rc;
   for (int attempt = 0; attempt < 3; ++attempt) {
-    HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, st.bp, d_out, cap, d_inst_out, s,
+    HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, ids, st.bp, d_out, cap, d_inst_out, s,
                                    next_events(c, PBIG_PATH_BINNED)));
     HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
@@ -504,15 +559,171 @@ int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint
   return fail(c, PBIG_E_HIP, "unreachable");
 }
 
-// Binned plain path, synchronous.  Returns PBIG_OK, an error, or 1 when a
-// coarse bin did not fit in LDS (the caller then runs the general path).
+// Pairs of a launch's instances found outside it (by re-binning their
+// heavy bins one level down, or by the general path): per instance the
+// extra count and the device runs holding those pairs, in order; and the
+// instances whose own pairs from the launch are to be discarded.
+struct Splice {
+  std::vector<uint64_t> extra;
+  std::vector<std::vector<std::pair<const uint2*, uint64_t>>> src;
+  std::vector<uint8_t> drop;
+  explicit Splice(size_t n = 0) : extra(n, 0), src(n), drop(n, 0) {}
+  bool empty() const {
+    for (size_t i = 0; i < extra.size(); ++i)
+      if (extra[i] || drop[i]) return false;
+    return true;
+  }
+};
+
+constexpr int kMaxLevels = 4;  // re-binning depth before an instance goes to the general path
+
+// Give every instance its extra pairs (pairs0 / pairs_extra), discard the
+// dropped instances' bins, rerun the pair scan and output kernels and copy
+// the extra pairs after each instance's own run.  *m receives the new result
+// words.  The device table is restored to the plan afterwards.
+int apply_splice(pbig_hip_ctx* c, BinnedState& st, const Splice& sp, uint2* d_out, uint64_t cap, uint64_t* d_io,
+                 hipStream_t s, Meta* m) {
+  const size_t n = st.inst.size();
+  int rc;
+  if (!d_io) {
+    if ((rc = grow(c, &st.io_buf, &st.cap_io_buf, 2 * std::max<uint64_t>(n, 1)))) return rc;
+    d_io = st.io_buf;
+  }
+  std::vector<InstDesc> tab = st.inst;
+  uint64_t acc = 0;
+  for (size_t i = 0; i < n; ++i) {
+    tab[i].pairs0 = acc;
+    tab[i].pairs_extra = sp.extra[i];
+    acc += sp.extra[i];
+  }
+  BatchPlan bp = st.bp;
+  bp.extra_total = acc;
+  if (n) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, tab.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, s));
+  for (size_t i = 0; i < n; ++i)
+    if (sp.drop[i] && tab[i].nb && tab[i].n)
+      HIP_TRY(c, hipMemsetAsync(st.ws.bin_seg + tab[i].bin0, 0, tab[i].nb * sizeof(uint2), s));
+  HIP_TRY(c, pbig::launch_binned_output(st.ws, bp, d_out, cap, d_io, s));
+  std::vector<uint64_t> io(2 * std::max<size_t>(n, 1));
+  HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+  if (n) HIP_TRY(c, hipMemcpyAsync(io.data(), d_io, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  *m = *c->h_meta;
+  for (size_t i = 0; i < n; ++i) {
+    if (sp.src[i].empty()) continue;
+    uint64_t dst = io[2 * i] + io[2 * i + 1] - sp.extra[i];  // after the instance's own pairs
+    for (const auto& run : sp.src[i]) {
+      if (dst < cap && run.second)
+        HIP_TRY(c, hipMemcpyAsync(d_out + dst, run.first, std::min<uint64_t>(run.second, cap - dst) * sizeof(uint2),
+                                  hipMemcpyDeviceToDevice, s));
+      dst += run.second;
+    }
+  }
+  if (n) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, st.inst.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return PBIG_OK;
+}
+
+// Run a state synchronously and resolve its coarse bins that did not fit in
+// LDS: each becomes a virtual instance of the next level (the same kernels
+// over this launch's bin-ordered boxes, with smaller bins), recursively; the
+// next level's pairs come back as *sp for the caller to apply_splice.
+// Instances that cannot be resolved so (a single fine cell over the LDS
+// buffer, or kMaxLevels reached) are marked in *failed.
+int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_t* ids, uint2* d_out,
+            uint64_t cap, uint64_t* d_io, hipStream_t s, int level, Meta* m, Splice* sp,
+            std::vector<uint8_t>* failed) {
+  const size_t n_inst = st.inst.size();
+  failed->assign(n_inst, 0);
+  *sp = Splice(n_inst);
+  int rc;
+  if ((rc = run_state_sync(c, st, boxes, ids, d_out, cap, d_io, s, m))) return rc;
+  if (!(m->error & pbig::kErrOverfull)) return PBIG_OK;
+  const uint32_t n_bins = st.bp.n_bins;
+  const uint32_t n_over = (uint32_t)std::min<uint64_t>(m->max_per_cell, n_bins);
+  std::vector<uint32_t> over(std::max<uint32_t>(n_over, 1)), total(std::max<uint32_t>(n_bins, 1)),
+      start(std::max<uint32_t>(n_bins, 1));
+  if (n_over) HIP_TRY(c, hipMemcpyAsync(over.data(), st.ws.overfull, n_over * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(total.data(), st.ws.total, n_bins * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(start.data(), st.ws.bin_start, n_bins * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  std::vector<uint32_t> heavy;
+  for (uint32_t k = 0; k < n_over; ++k) {
+    const uint32_t b = over[k] & ~pbig::kOverfullCell;
+    if (b >= n_bins) continue;
+    const uint32_t i = st.bin_inst[b];
+    if ((over[k] & pbig::kOverfullCell) || level + 1 >= kMaxLevels) (*failed)[i] = 1;
+    else heavy.push_back(b);
+  }
+  std::sort(heavy.begin(), heavy.end());
+  if (!st.sub) {
+    st.sub = std::make_unique<BinnedState>();
+    HIP_TRY(c, hipMalloc((void**)&st.sub->ws.meta, sizeof(Meta)));
+    st.sub->own_meta = true;
+  }
+  BinnedState& V = *st.sub;
+  V.inst.clear();
+  const uint32_t target = std::max<uint32_t>(c->cfg.bin_target / 2, 64);
+  for (uint32_t b : heavy) {
+    const uint32_t i = st.bin_inst[b];
+    const InstDesc& P = st.inst[i];
+    InstDesc v;
+    if (!plan_virtual(P, i, b - P.bin0, total[b], start[b], target, &v)) (*failed)[i] = 1;
+    else V.inst.push_back(v);
+  }
+  {
+    std::vector<InstDesc> keep;
+    for (const InstDesc& v : V.inst)
+      if (!(*failed)[v.parent]) keep.push_back(v);
+    V.inst.swap(keep);
+  }
+  sp->drop = *failed;
+  if (V.inst.empty()) return PBIG_OK;
+  if (!layout_tables(V, (uint32_t)c->n_cu, nullptr)) return fail(c, PBIG_E_ARG, "re-binning layout failed");
+  if ((rc = size_state(c, V))) return rc;
+  if ((rc = upload_tables(c, V, s))) return rc;
+  const size_t nv = V.inst.size();
+  if ((rc = grow(c, &st.sub_io, &st.cap_sub_io, 2 * nv))) return rc;
+  if ((rc = grow(c, &st.sub_out, &st.cap_sub_out, std::max<uint64_t>(4096, 2 * V.bp.n_boxes)))) return rc;
+  Meta m2;
+  Splice sp2;
+  std::vector<uint8_t> f2;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if ((rc = after_upload(c, V, s))) return rc;
+    if ((rc = resolve(c, V, st.ws.bbox, st.ws.bid, st.sub_out, st.cap_sub_out, st.sub_io, s, level + 1, &m2, &sp2,
+                      &f2)))
+      return rc;
+    if (!sp2.empty() && (rc = apply_splice(c, V, sp2, st.sub_out, st.cap_sub_out, st.sub_io, s, &m2))) return rc;
+    if (m2.n_pairs <= st.cap_sub_out) break;
+    if (attempt == 1) return fail(c, PBIG_E_NOMEM, "re-binned pairs did not fit after regrowth");
+    if ((rc = grow(c, &st.sub_out, &st.cap_sub_out, m2.n_pairs + m2.n_pairs / 8 + 1024))) return rc;
+  }
+  std::vector<uint64_t> io(2 * nv);
+  HIP_TRY(c, hipMemcpy(io.data(), st.sub_io, 2 * nv * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  for (size_t v = 0; v < nv; ++v)
+    if (f2[v]) (*failed)[V.inst[v].parent] = 1;
+  for (size_t v = 0; v < nv; ++v) {
+    const uint32_t i = V.inst[v].parent;
+    if ((*failed)[i]) continue;
+    sp->extra[i] += io[2 * v + 1];
+    sp->src[i].emplace_back(st.sub_out + io[2 * v], io[2 * v + 1]);
+  }
+  sp->drop = *failed;
+  return PBIG_OK;
+}
+
+// Binned plain path, synchronous.  Returns PBIG_OK, an error, or 1 when the
+// instance cannot be resolved on the binned path (the caller then runs the
+// general path).
 int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d_out, uint64_t cap,
                     hipStream_t s, uint64_t* n_pairs) {
   BinnedState& st = c->plain;
   Meta m;
+  Splice sp;
+  std::vector<uint8_t> failed;
   int rc;
-  if ((rc = run_state_sync(c, st, d_boxes, d_out, cap, nullptr, s, &m))) return rc;
-  if (m.error & pbig::kErrOverfull) return 1;
+  if ((rc = resolve(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, 0, &m, &sp, &failed))) return rc;
+  if (failed[0]) return 1;
+  if (!sp.empty() && (rc = apply_splice(c, st, sp, d_out, cap, nullptr, s, &m))) return rc;
   const InstDesc& d = st.inst[0];
   c->last_path = PBIG_PATH_BINNED;
   c->stats.n_boxes = n;
@@ -723,7 +934,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
     binned = ok;
   }
   if (binned) {
-    HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, c->plain.bp, (uint2*)d_pairs, cap,
+    HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, nullptr, c->plain.bp, (uint2*)d_pairs, cap,
                                    nullptr, s, next_events(c, PBIG_PATH_BINNED)));
   } else {
     if ((rc = prepare(c, n, n_cells))) return rc;
@@ -1014,7 +1225,7 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = pick_stream(c, stream);
   if ((rc = after_upload(c, b->st, s))) return rc;
-  HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
+  HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
   if (d_result)
     HIP_TRY(c, hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
@@ -1034,7 +1245,7 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
   if (b->st.stage_ev) HIP_TRY(c, hipEventSynchronize(b->st.stage_ev));  // tables in place before capture
   hipStream_t s = b->capture_stream;
   HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, b->st.bp, (uint2*)d_pairs, cap,
+  hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap,
                                      d_inst_pairs, s, nullptr);
   if (e == hipSuccess && d_result)
     e = hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
@@ -1074,71 +1285,37 @@ int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_box
   const float4* boxes = (const float4*)d_boxes;
   const uint32_t n_inst = (uint32_t)b->desc.size();
   BinnedState& st = b->st;
-  // Instances for the general path: those the binned path cannot plan, and
-  // those with a coarse bin or fine cell beyond LDS (found by a first run).
-  std::vector<uint8_t> gen = b->general;
-  bool any_gen = false;
-  for (uint8_t g : gen) any_gen = any_gen || g;
+  // The launch, with heavy coarse bins re-binned one level down; instances
+  // that stay unresolved (or that the binned path cannot plan) go to the
+  // general path, and every instance's pairs are then placed in its run.
   Meta m{};
-  if (!any_gen) {
-    if ((rc = run_state_sync(c, st, boxes, (uint2*)d_pairs, cap, b->d_inst_out, s, &m))) return rc;
-    if (m.error & pbig::kErrOverfull) {
-      const uint32_t n_over = (uint32_t)std::min<uint64_t>(m.max_per_cell, st.bp.n_bins);
-      std::vector<uint32_t> over(n_over);
-      if (n_over)
-        HIP_TRY(c, hipMemcpy(over.data(), st.ws.overfull, n_over * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      for (uint32_t gb : over)
-        if (gb < st.bin_inst.size()) gen[st.bin_inst[gb]] = 1;
-      any_gen = true;
-    }
+  Splice sp;
+  std::vector<uint8_t> failed;
+  if ((rc = resolve(c, st, boxes, nullptr, (uint2*)d_pairs, cap, b->d_inst_out, s, 0, &m, &sp, &failed))) return rc;
+  std::vector<uint64_t> cnt(n_inst, 0), at(n_inst, 0);
+  uint64_t tmp_total = 0;
+  bool any_gen = false;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    if (!(b->general[i] || failed[i])) continue;
+    any_gen = true;
+    uint64_t k = 0;
+    if (b->desc[i].n_boxes && (rc = run_general_instance(b, boxes, i, nullptr, 0, s, &k))) return rc;
+    cnt[i] = k;
+    at[i] = tmp_total;
+    tmp_total += k;
   }
   if (any_gen) {
-    // General path per flagged instance: counts first, then the pairs into
-    // one temporary buffer, instance after instance.
-    std::vector<uint64_t> cnt(n_inst, ~0ull), at(n_inst, 0);
-    uint64_t tmp_total = 0;
-    for (uint32_t i = 0; i < n_inst; ++i) {
-      if (!gen[i]) continue;
-      uint64_t k = 0;
-      if (b->desc[i].n_boxes && (rc = run_general_instance(b, boxes, i, nullptr, 0, s, &k))) return rc;
-      cnt[i] = k;
-      at[i] = tmp_total;
-      tmp_total += k;
-    }
     if ((rc = grow(c, &b->d_tmp, &b->cap_tmp, std::max<uint64_t>(tmp_total, 1)))) return rc;
     for (uint32_t i = 0; i < n_inst; ++i) {
-      if (!gen[i] || cnt[i] == 0) continue;
+      if (!(b->general[i] || failed[i])) continue;
       uint64_t k = 0;
-      if ((rc = run_general_instance(b, boxes, i, b->d_tmp + at[i], cnt[i], s, &k))) return rc;
+      if (cnt[i] && (rc = run_general_instance(b, boxes, i, b->d_tmp + at[i], cnt[i], s, &k))) return rc;
+      sp.drop[i] = 1;
+      sp.extra[i] = cnt[i];
+      sp.src[i].assign(1, std::make_pair((const uint2*)(b->d_tmp + at[i]), cnt[i]));
     }
-    // The binned launch without them, their slots reserved.
-    const std::vector<InstDesc> saved = st.inst;
-    if (!layout_tables(st, (uint32_t)c->n_cu, &cnt)) return fail(c, PBIG_E_ARG, "batch layout failed");
-    if ((rc = upload_tables(c, st, s))) return rc;
-    rc = run_state_sync(c, st, boxes, (uint2*)d_pairs, cap, b->d_inst_out, s, &m);
-    // restore the full plan for the asynchronous entry points
-    st.inst = saved;
-    std::vector<uint64_t> keep(n_inst, ~0ull);
-    for (uint32_t i = 0; i < n_inst; ++i)
-      if (b->general[i]) keep[i] = 0;
-    layout_tables(st, (uint32_t)c->n_cu, &keep);
-    int rc2 = upload_tables(c, st, s);
-    if (rc) return rc;
-    if (rc2) return rc2;
-    if (m.error & pbig::kErrOverfull) return fail(c, PBIG_E_HIP, "a coarse bin overflowed twice");
-    // Place the general path's pairs in their slots.
-    std::vector<uint64_t> io(2 * (size_t)n_inst);
-    if (n_inst)
-      HIP_TRY(c, hipMemcpy(io.data(), b->d_inst_out, io.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n_inst; ++i) {
-      if (!gen[i] || cnt[i] == 0) continue;
-      const uint64_t off = io[2 * (size_t)i];
-      if (off >= cap) continue;
-      const uint64_t k = std::min<uint64_t>(cnt[i], cap - off);
-      HIP_TRY(c, hipMemcpyAsync(d_pairs + 2 * off, b->d_tmp + at[i], k * sizeof(uint2), hipMemcpyDeviceToDevice, s));
-    }
-    HIP_TRY(c, hipStreamSynchronize(s));
   }
+  if (!sp.empty() && (rc = apply_splice(c, st, sp, (uint2*)d_pairs, cap, b->d_inst_out, s, &m))) return rc;
   if (h_inst_pairs && n_inst)
     HIP_TRY(c, hipMemcpy(h_inst_pairs, b->d_inst_out, 2 * (size_t)n_inst * sizeof(uint64_t), hipMemcpyDeviceToHost));
   c->stats = pbig_hip_stats{};

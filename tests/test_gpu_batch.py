@@ -251,3 +251,34 @@ def test_batch_c4_full(torch_cuda, eng):
         w = oracle.sort_pairs(oracle.grid_pairs(sub, float(ranges[i]), 32, threads=16))
         g = oracle.sort_pairs(got[int(io[i, 0]):int(io[i, 0] + io[i, 1])])
         assert np.array_equal(g, w), i
+
+
+def test_batch_clustered_instances_rebinned(torch_cuda, eng):
+    """Clustered instances whose coarse bins exceed LDS, mixed with uniform
+    ones: heavy bins are re-binned and spliced into their instance's run."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    inst = []
+    for t in range(10):
+        n = [3000, 40000, 120000, 7000, 250000][t % 5]
+        L = 65536.0 * float(np.sqrt(n / 2097152.0))
+        if t % 2:
+            inst.append((workloads.clustered(n, L, 100 + t), L))
+        else:
+            inst.append((workloads.uniform(n, L, 1.0, 32.0, 100 + t), L))
+    boxes, offsets, ranges = concat(inst)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    cells = [float(np.float32(L / np.sqrt(offsets[i + 1] - offsets[i]))) for i, (_, L) in enumerate(inst)]
+    with pbig_amd.Batch(eng, offsets, ranges, cells) as b:
+        n, io = b.check(d, pairs)
+        assert n == total
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+        # and again: the plan is restored after the splice
+        pairs.fill_(-1)
+        n2, io2 = b.check(d, pairs)
+        assert n2 == total and np.array_equal(io2, io)
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io2), want)

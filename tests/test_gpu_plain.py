@@ -310,3 +310,32 @@ def test_auto_falls_back_on_dense_bins(torch_cuda, orc):
         got = run_device(torch_cuda, e, boxes, 64.0, 16)
         assert e.stats()["path"] == PATHS["general"]
     assert np.array_equal(got, orc.sort_pairs(orc.brute_pairs(boxes)))
+
+
+@pytest.mark.parametrize("cs", [16, 45, 64, 128])
+def test_binned_c3_rebins_heavy_bins(torch_cuda, binned, orc, manifest, cs):
+    """The clustered instance: coarse bins in the clusters exceed LDS and are
+    re-binned one level down (recursively); the binned path must still give
+    the exact set, without the general path."""
+    from pbig_amd import workloads
+
+    binned.configure(PATHS["binned"], 2000)
+    boxes, L = workloads.c3_clustered()
+    m = manifest["c3_clustered_2m"]
+    got = run_device(torch_cuda, binned, boxes, L, cs, cap=4 * m["pairs"])
+    assert binned.stats()["path"] == PATHS["binned"]
+    assert len(got) == m["pairs"]
+    assert orc.pair_hash(got) == m["sha256"]
+
+
+def test_binned_deep_rebinning(torch_cuda, binned, orc):
+    """A dense core inside a sparse domain needs several re-binning levels."""
+    from pbig_amd import workloads
+
+    binned.configure(PATHS["binned"], 2000)
+    core = workloads.uniform(60000, 400.0, 0.5, 2.0, 21) + np.float32(3000.0)
+    rest = workloads.uniform(20000, 8192.0, 1.0, 8.0, 22)
+    boxes = np.concatenate([core, rest]).astype(np.float32)
+    got = run_device(torch_cuda, binned, boxes, 8192.0, 4.0)
+    assert binned.stats()["path"] == PATHS["binned"]
+    assert np.array_equal(got, orc.sort_pairs(orc.sweep_pairs(boxes)))
