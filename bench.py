@@ -95,6 +95,42 @@ def read_traffic(path_name, cs, bin_target):
     return out, os.path.relpath(fn, ROOT)
 
 
+def clustered_leg(torch, eng_cls, reps=5):
+    """The clustered 2,097,152-box instance (seed 3), boxes resident: one
+    synchronous call per step (its heavy coarse bins are re-binned, which
+    needs the host between levels), binned path vs general path."""
+    import pbig_amd
+    from pbig_amd import autotune, workloads
+
+    boxes, L = workloads.c3_clustered()
+    n = boxes.shape[0]
+    dev = torch.device("cuda:0")
+    d = torch.from_numpy(boxes).to(dev)
+    pairs = torch.empty((40 * n, 2), dtype=torch.int32, device=dev)
+    cell = float(autotune.cell_sizes([0, n], [L])[0])
+    out = {"workload": "2,097,152 float32 rects, 90% in 64 Gaussian clusters (sigma 327.68) + 10% uniform "
+                       "in [0,65536)^2, log-normal sides, seed 3, boxes resident in HBM",
+           "cell_size": cell}
+    for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL)):
+        e = eng_cls(0)
+        e.configure(path)
+        k = e.check_device(d, L, cell, pairs)
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            k2 = e.check_device(d, L, cell, pairs)
+            t.append(time.perf_counter() - t0)
+            assert k2 == k
+        st = e.stats()
+        e.close()
+        ms = statistics.median(t) * 1e3
+        out[name] = {"ms_per_call": ms, "pairs": k, "overlaps_per_s": k / (ms / 1e3), "path": st["path"]}
+    del d, pairs
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_leg(torch, steps, warmup):
     """The 4,096-instance workload (n log-uniform in [64, 262144], L = 16
     sqrt(n), sides U[1,8], seed 4): one HIP-graph replay of the batch per
@@ -314,6 +350,7 @@ def main():
     ap.add_argument("--bin-target", type=int, default=2000)
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-instance batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-clustered", action="store_true", help="skip the clustered 2M-box leg")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
@@ -448,6 +485,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = batched_leg(torch, args.batched_steps, 2)
+    if not args.no_clustered:
+        out["clustered"] = clustered_leg(torch, pbig_amd.Engine)
     if not args.no_serving:
         out["serving"] = serving_leg(torch, args.serve_requests, args.serve_rate, args.serve_budget_gb)
     if not args.no_cpu_baseline:

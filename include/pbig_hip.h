@@ -181,13 +181,23 @@ int pbig_hip_batch_update(pbig_hip_batch* batch, const pbig_hip_instance* inst, 
 
 /* Synchronous batch check.  d_boxes holds n_boxes_total boxes (every
  * instance's range must lie inside).  h_inst_pairs (optional, host) receives
- * n_inst x {offset, count}.  Regrows the workspace when needed; an instance
- * whose boxes crowd a coarse bin beyond LDS runs on the general path and its
- * pairs are placed in its slot.  *n_pairs = total; PBIG_E_OVERFLOW when it
- * exceeds pair_capacity (offsets and counts are still returned). */
+ * n_inst x {offset, count}.  Regrows the workspace when needed.  A coarse
+ * bin whose boxes exceed LDS is re-binned into smaller bins (recursively);
+ * an instance with a single fine cell beyond LDS runs on the general path.
+ * Either way its pairs are placed in its run.  *n_pairs = total;
+ * PBIG_E_OVERFLOW when it exceeds pair_capacity (offsets and counts are
+ * still returned). */
 int pbig_hip_batch_check(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
                          uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
                          uint64_t* n_pairs, void* stream);
+
+/* Complete a pbig_hip_batch_launch (same arguments, same stream) whose error
+ * word was nonzero: when only coarse bins beyond LDS were reported, they are
+ * re-binned and spliced without running the launch again; otherwise this is
+ * pbig_hip_batch_check. */
+int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
+                          uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
+                          uint64_t* n_pairs, void* stream);
 
 /* Asynchronous batch launch: enqueues the seven kernels and returns; no host
  * synchronisation, no allocation, no memset, so it may be captured in a HIP

@@ -45,6 +45,7 @@ EXPORTS = (
     "pbig_hip_batch_destroy",
     "pbig_hip_batch_update",
     "pbig_hip_batch_check",
+    "pbig_hip_batch_finish",
     "pbig_hip_batch_launch",
     "pbig_hip_batch_capture",
     "pbig_hip_batch_replay",
@@ -142,6 +143,7 @@ def load_library(path: str | None = None):
         "pbig_hip_batch_destroy": (i32, [P]),
         "pbig_hip_batch_update": (i32, [P, ctypes.POINTER(Instance), u32, P]),
         "pbig_hip_batch_check": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
+        "pbig_hip_batch_finish": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
         "pbig_hip_batch_launch": (i32, [P, P, u64, P, u64, P, P, P]),
         "pbig_hip_batch_capture": (i32, [P, P, u64, P, u64, P, P]),
         "pbig_hip_batch_replay": (i32, [P, P]),
@@ -383,15 +385,16 @@ class Batch:
         self._lib.pbig_hip_batch_info(self._h, ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(nbin))
         return {"n_boxes": int(nb.value), "n_chunks": int(nc.value), "n_bins": int(nbin.value)}
 
-    def check(self, boxes, pairs, allow_overflow: bool = False, stream=None):
+    def check(self, boxes, pairs, allow_overflow: bool = False, stream=None, finish: bool = False):
         """Synchronous.  Returns (total pairs, (k, 2) uint64 numpy array of
-        per-instance (offset, count))."""
+        per-instance (offset, count)).  finish=True completes the last
+        ``launch`` (same arguments and stream) instead of starting over."""
         cap = int(pairs.shape[0]) if pairs is not None else 0
         io = np.zeros((max(self.n_inst, 1), 2), dtype=np.uint64)
         out = ctypes.c_uint64(0)
-        rc = self._lib.pbig_hip_batch_check(self._h, _ptr(boxes) if self.n_boxes_total else None,
-                                            int(boxes.shape[0]), _ptr(pairs) if cap else None, cap,
-                                            io.ctypes.data, ctypes.byref(out), _stream_ptr(stream))
+        fn = self._lib.pbig_hip_batch_finish if finish else self._lib.pbig_hip_batch_check
+        rc = fn(self._h, _ptr(boxes) if self.n_boxes_total else None, int(boxes.shape[0]),
+                _ptr(pairs) if cap else None, cap, io.ctypes.data, ctypes.byref(out), _stream_ptr(stream))
         if rc == PBIG_E_OVERFLOW and allow_overflow:
             return int(out.value), io[: self.n_inst]
         if rc != PBIG_OK:

@@ -99,6 +99,21 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
                          const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
                          hipEvent_t* events = nullptr);
 
+// One run of pairs found outside a launch, to be placed after instance
+// inst's own pairs at position `within` of its extra area.
+struct SpliceRun {
+  const uint2* src;
+  uint64_t len;
+  uint64_t within;
+  uint32_t inst;
+  uint32_t pad;
+};
+
+// Copy every run to out[off_i + count_i - extra_i + within] (off_i, count_i
+// from inst_out, extra_i = the instance's pairs_extra), clipped to cap.
+hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint64_t* inst_out,
+                              const InstDesc* inst, uint2* out, uint64_t cap, hipStream_t s);
+
 // Only the last two kernels (pair scan and output), after the instance table
 // changed its pairs0 / pairs_extra fields: the test's regions are reused.
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,

@@ -813,6 +813,27 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
   }
 }
 
+// Place runs of pairs found outside the launch (block r copies run r).
+__global__ __launch_bounds__(256) void k_splice_copy(const SpliceRun* __restrict__ runs,
+                                                      const uint64_t* __restrict__ inst_out,
+                                                      const InstDesc* __restrict__ inst, uint2* __restrict__ out,
+                                                      uint64_t cap) {
+  const SpliceRun r = runs[blockIdx.x];
+  const uint64_t dst0 = inst_out[2 * (uint64_t)r.inst] + inst_out[2 * (uint64_t)r.inst + 1] -
+                        inst[r.inst].pairs_extra + r.within;
+  for (uint64_t k = threadIdx.x; k < r.len; k += 256) {
+    const uint64_t d = dst0 + k;
+    if (d < cap) out[d] = r.src[k];
+  }
+}
+
+hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint64_t* inst_out,
+                              const InstDesc* inst, uint2* out, uint64_t cap, hipStream_t s) {
+  if (!n_runs) return hipSuccess;
+  k_splice_copy<<<n_runs, 256, 0, s>>>(runs, inst_out, inst, out, cap);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 size_t binned_test_lds_bytes() { return sizeof(TestLds); }
 

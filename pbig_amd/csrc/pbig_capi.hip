@@ -54,6 +54,8 @@ struct BinnedState {
   uint64_t cap_sub_io = 0;
   uint64_t* io_buf = nullptr;      // (offset, count) per instance when the caller gives none
   uint64_t cap_io_buf = 0;
+  pbig::SpliceRun* runs = nullptr; // device table of the runs to place
+  uint64_t cap_runs = 0;
   bool own_meta = false;
 };
 
@@ -273,25 +275,44 @@ bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, In
 }
 
 // Plan of a virtual instance: coarse bin lb of instance P, holding m
-// memberships at box0 of P's launch's bin-ordered arrays, re-binned into
-// bins of about target memberships.  False when the bin cannot be split
-// (a single fine cell).
+// memberships at box0 of P's launch's bin-ordered arrays.  Its fine cells are
+// P's cells split 2^j ways per side, j chosen so that a cell holds about two
+// memberships at the bin's density (dense cells cost quadratically): the
+// cell side is P's divided by 2^j, and inv_f * 2^j is exact, so a point's
+// cell at this level lies inside its cell at P's level and the
+// reference-cell rule claims every pair in the same region as at P's level.
+// Bins of about `target` memberships, at most kMaxK cells per side.  False
+// when the bin cannot be split.
 bool plan_virtual(const InstDesc& P, uint32_t parent, uint32_t lb, uint32_t m, uint64_t box0, uint32_t target,
                   InstDesc* v) {
   *v = InstDesc{};
   v->n = m;
   v->box0 = box0;
   v->parent = parent;
-  v->inv_f = P.inv_f;
-  v->nf = P.nf;
-  v->fx0 = P.fx0 + (lb % P.nbx) * P.k;
-  v->fy0 = P.fy0 + (lb / P.nbx) * P.k;
-  v->span_x = std::min<uint32_t>(P.k, P.fx0 + P.span_x - v->fx0);
-  v->span_y = std::min<uint32_t>(P.k, P.fy0 + P.span_y - v->fy0);
+  const uint32_t fx0 = P.fx0 + (lb % P.nbx) * P.k, fy0 = P.fy0 + (lb / P.nbx) * P.k;
+  const uint32_t sx = std::min<uint32_t>(P.k, P.fx0 + P.span_x - fx0);
+  const uint32_t sy = std::min<uint32_t>(P.k, P.fy0 + P.span_y - fy0);
+  const double per_cell = (double)m / ((double)sx * sy);
+  uint32_t j = 0;
+  while (j < 6 && per_cell / (double)(1u << (2 * j)) > 2.0 && ((uint64_t)P.nf << (j + 1)) <= (1u << 21)) ++j;
+  const uint32_t f = 1u << j;
+  v->inv_f = P.inv_f * (float)f;
+  v->nf = P.nf * f;
+  v->fx0 = fx0 * f;
+  v->fy0 = fy0 * f;
+  v->span_x = sx * f;
+  v->span_y = sy * f;
   const uint32_t side = std::max(v->span_x, v->span_y);
   if (side <= 1) return false;
-  double kd = std::round(side * std::sqrt((double)std::max<uint32_t>(target, 16) / std::max<uint32_t>(m, 1)));
-  const uint32_t k = (uint32_t)std::min<double>(std::max<double>(kd, 1.0), side - 1);
+  // bins: about m / target of them, square, k <= kMaxK, at most kMaxBins
+  const double nbx_want = std::max(2.0, std::ceil(std::sqrt((double)m / std::max<uint32_t>(target, 16))));
+  uint32_t k = (uint32_t)std::ceil(side / nbx_want);
+  k = std::min<uint32_t>(std::max<uint32_t>(k, 1), pbig::kMaxK);
+  if (j == 0) k = std::min<uint32_t>(k, side - 1);  // same cells: the bins must shrink
+  while (((side + k - 1) / k) * ((side + k - 1) / k) > pbig::kMaxBins) {
+    if (k >= pbig::kMaxK) return false;
+    ++k;
+  }
   v->k = k;
   v->k_magic = ((1ull << 32) + k - 1) / k;
   v->nbx = (side + k - 1) / k;
@@ -434,12 +455,13 @@ void free_state(BinnedState& st) {
     free_state(*st.sub);
     st.sub.reset();
   }
-  for (void* p : {(void*)st.sub_out, (void*)st.sub_io, (void*)st.io_buf})
+  for (void* p : {(void*)st.sub_out, (void*)st.sub_io, (void*)st.io_buf, (void*)st.runs})
     if (p) (void)hipFree(p);
   st.sub_out = nullptr;
   st.sub_io = nullptr;
   st.io_buf = nullptr;
-  st.cap_sub_out = st.cap_sub_io = st.cap_io_buf = 0;
+  st.runs = nullptr;
+  st.cap_sub_out = st.cap_sub_io = st.cap_io_buf = st.cap_runs = 0;
   const BinnedWorkspace& b = st.ws;
   for (void* p : {(void*)b.inst, (void*)b.chunk_inst, (void*)b.bin_inst, (void*)b.hist, (void*)b.off,
                   (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bin_seg, (void*)b.bin_out,
@@ -591,35 +613,37 @@ int apply_splice(pbig_hip_ctx* c, BinnedState& st, const Splice& sp, uint2* d_ou
   }
   std::vector<InstDesc> tab = st.inst;
   uint64_t acc = 0;
+  std::vector<pbig::SpliceRun> runs;
   for (size_t i = 0; i < n; ++i) {
     tab[i].pairs0 = acc;
     tab[i].pairs_extra = sp.extra[i];
     acc += sp.extra[i];
+    uint64_t within = 0;
+    for (const auto& r : sp.src[i]) {
+      // pieces of at most kPiece pairs, one copy block each
+      constexpr uint64_t kPiece = 16384;
+      for (uint64_t o = 0; o < r.second; o += kPiece)
+        runs.push_back(pbig::SpliceRun{r.first + o, std::min<uint64_t>(kPiece, r.second - o), within + o,
+                                       (uint32_t)i, 0u});
+      within += r.second;
+    }
   }
   BatchPlan bp = st.bp;
   bp.extra_total = acc;
+  if ((rc = grow(c, &st.runs, &st.cap_runs, std::max<uint64_t>(runs.size(), 1)))) return rc;
   if (n) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, tab.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, s));
+  if (!runs.empty())
+    HIP_TRY(c, hipMemcpyAsync(st.runs, runs.data(), runs.size() * sizeof(pbig::SpliceRun), hipMemcpyHostToDevice,
+                              s));
   for (size_t i = 0; i < n; ++i)
     if (sp.drop[i] && tab[i].nb && tab[i].n)
       HIP_TRY(c, hipMemsetAsync(st.ws.bin_seg + tab[i].bin0, 0, tab[i].nb * sizeof(uint2), s));
   HIP_TRY(c, pbig::launch_binned_output(st.ws, bp, d_out, cap, d_io, s));
-  std::vector<uint64_t> io(2 * std::max<size_t>(n, 1));
+  HIP_TRY(c, pbig::launch_splice_copy(st.runs, (uint32_t)runs.size(), d_io, st.ws.inst, d_out, cap, s));
   HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
-  if (n) HIP_TRY(c, hipMemcpyAsync(io.data(), d_io, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
-  *m = *c->h_meta;
-  for (size_t i = 0; i < n; ++i) {
-    if (sp.src[i].empty()) continue;
-    uint64_t dst = io[2 * i] + io[2 * i + 1] - sp.extra[i];  // after the instance's own pairs
-    for (const auto& run : sp.src[i]) {
-      if (dst < cap && run.second)
-        HIP_TRY(c, hipMemcpyAsync(d_out + dst, run.first, std::min<uint64_t>(run.second, cap - dst) * sizeof(uint2),
-                                  hipMemcpyDeviceToDevice, s));
-      dst += run.second;
-    }
-  }
   if (n) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, st.inst.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, s));
   HIP_TRY(c, hipStreamSynchronize(s));
+  *m = *c->h_meta;
   return PBIG_OK;
 }
 
@@ -629,14 +653,16 @@ int apply_splice(pbig_hip_ctx* c, BinnedState& st, const Splice& sp, uint2* d_ou
 // next level's pairs come back as *sp for the caller to apply_splice.
 // Instances that cannot be resolved so (a single fine cell over the LDS
 // buffer, or kMaxLevels reached) are marked in *failed.
+// pass_done: the state's launch already ran with these results in *m (an
+// asynchronous launch whose only error is kErrOverfull).
 int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_t* ids, uint2* d_out,
             uint64_t cap, uint64_t* d_io, hipStream_t s, int level, Meta* m, Splice* sp,
-            std::vector<uint8_t>* failed) {
+            std::vector<uint8_t>* failed, bool pass_done = false) {
   const size_t n_inst = st.inst.size();
   failed->assign(n_inst, 0);
   *sp = Splice(n_inst);
   int rc;
-  if ((rc = run_state_sync(c, st, boxes, ids, d_out, cap, d_io, s, m))) return rc;
+  if (!pass_done && (rc = run_state_sync(c, st, boxes, ids, d_out, cap, d_io, s, m))) return rc;
   if (!(m->error & pbig::kErrOverfull)) return PBIG_OK;
   const uint32_t n_bins = st.bp.n_bins;
   const uint32_t n_over = (uint32_t)std::min<uint64_t>(m->max_per_cell, n_bins);
@@ -698,7 +724,8 @@ int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_
     if ((rc = grow(c, &st.sub_out, &st.cap_sub_out, m2.n_pairs + m2.n_pairs / 8 + 1024))) return rc;
   }
   std::vector<uint64_t> io(2 * nv);
-  HIP_TRY(c, hipMemcpy(io.data(), st.sub_io, 2 * nv * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpyAsync(io.data(), st.sub_io, 2 * nv * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
   for (size_t v = 0; v < nv; ++v)
     if (f2[v]) (*failed)[V.inst[v].parent] = 1;
   for (size_t v = 0; v < nv; ++v) {
@@ -1272,8 +1299,21 @@ int pbig_hip_batch_replay(pbig_hip_batch* b, void* stream) {
   return PBIG_OK;
 }
 
+int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                     uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream, bool after_launch);
+
 int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
                          uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream) {
+  return batch_check_impl(b, d_boxes, n_boxes_total, d_pairs, cap, h_inst_pairs, n_pairs, stream, false);
+}
+
+int pbig_hip_batch_finish(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                          uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream) {
+  return batch_check_impl(b, d_boxes, n_boxes_total, d_pairs, cap, h_inst_pairs, n_pairs, stream, true);
+}
+
+int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
+                     uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream, bool after_launch) {
   if (!b) return PBIG_E_ARG;
   pbig_hip_ctx* c = b->ctx;
   if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
@@ -1291,7 +1331,18 @@ int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_box
   Meta m{};
   Splice sp;
   std::vector<uint8_t> failed;
-  if ((rc = resolve(c, st, boxes, nullptr, (uint2*)d_pairs, cap, b->d_inst_out, s, 0, &m, &sp, &failed))) return rc;
+  bool pass_done = false;
+  if (after_launch) {
+    // Results of the last pbig_hip_batch_launch on this stream: continue
+    // from them when the only trouble was bins beyond LDS.
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    m = *c->h_meta;
+    pass_done = (m.error & ~(unsigned long long)pbig::kErrOverfull) == 0 && m.n_pairs <= cap;
+  }
+  if ((rc = resolve(c, st, boxes, nullptr, (uint2*)d_pairs, cap, b->d_inst_out, s, 0, &m, &sp, &failed,
+                    pass_done)))
+    return rc;
   std::vector<uint64_t> cnt(n_inst, 0), at(n_inst, 0);
   uint64_t tmp_total = 0;
   bool any_gen = false;

@@ -293,9 +293,10 @@ class Server:
             for r in s.reqs:
                 r.t_ready = now
             if err != 0 or total > s.pairs.shape[0]:
-                # workspace regrowth, a crowded instance or too many pairs:
-                # the synchronous entry point handles all three
-                total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream)
+                # bins beyond LDS (finished from this launch's results),
+                # workspace regrowth or too many pairs (run again)
+                total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream,
+                                          finish=True)
                 if total > s.pairs.shape[0]:
                     cap = total + total // 4
                     s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)

@@ -282,3 +282,29 @@ def test_batch_clustered_instances_rebinned(torch_cuda, eng):
         n2, io2 = b.check(d, pairs)
         assert n2 == total and np.array_equal(io2, io)
         check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io2), want)
+
+
+def test_batch_finish_after_launch(torch_cuda, eng):
+    """An asynchronous launch that reports bins beyond LDS is completed by
+    finish() from its own results (no second launch), exactly."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    inst = [(workloads.clustered(150000, 17000.0, 300), 17000.0),
+            (workloads.uniform(50000, 3000.0, 1.0, 8.0, 301), 3000.0)]
+    boxes, offsets, ranges = concat(inst)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    cells = [40.0, 13.0]
+    with pbig_amd.Batch(eng, offsets, ranges, cells) as b:
+        res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+        b.check(d, pairs)  # sizes the workspace
+        pairs.fill_(-1)
+        b.launch(d, pairs, None, res)
+        torch_cuda.cuda.synchronize()
+        assert res[1].item() == 8  # bins beyond LDS, nothing else
+        n, io = b.check(d, pairs, finish=True)
+        assert n == total
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)

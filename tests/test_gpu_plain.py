@@ -298,14 +298,27 @@ def test_binned_edge_cases(torch_cuda, binned, orc, name):
     assert np.array_equal(got, want), (name, len(got), len(want))
 
 
-def test_auto_falls_back_on_dense_bins(torch_cuda, orc):
-    """6000 boxes in one fine cell overflow the binned path's LDS buffer:
-    AUTO must still return the exact set through the general path."""
+def test_dense_spot_resolved_on_binned_path(torch_cuda, orc):
+    """6000 boxes in a 2.5 x 2.5 spot of a 16-unit cell: the coarse bin and
+    the fine cell overflow LDS; re-binning with finer cells resolves them."""
     import pbig_amd
 
     rng = np.random.default_rng(11)
     c = rng.uniform(10, 12, size=(6000, 2)).astype(np.float32)
     boxes = np.stack([c[:, 0], c[:, 0] + 0.5, c[:, 1], c[:, 1] + 0.5], 1).astype(np.float32)
+    with pbig_amd.Engine(0) as e:
+        got = run_device(torch_cuda, e, boxes, 64.0, 16)
+        assert e.stats()["path"] == PATHS["binned"]
+    assert np.array_equal(got, orc.sort_pairs(orc.brute_pairs(boxes)))
+
+
+def test_auto_falls_back_on_unsplittable_stack(torch_cuda, orc):
+    """5000 identical boxes share every cell at any refinement: AUTO must
+    return the exact set through the general path."""
+    import pbig_amd
+
+    boxes = np.tile(np.array([[20.0, 20.5, 30.0, 30.25]], np.float32), (5000, 1))
+    boxes = np.concatenate([boxes, np.array([[1, 2, 1, 2], [20.5, 21, 30, 31]], np.float32)])
     with pbig_amd.Engine(0) as e:
         got = run_device(torch_cuda, e, boxes, 64.0, 16)
         assert e.stats()["path"] == PATHS["general"]
