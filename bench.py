@@ -131,7 +131,7 @@ def clustered_leg(torch, eng_cls, reps=5):
     return out
 
 
-def batched_leg(torch, steps, warmup):
+def batched_leg(torch, steps, warmup, baseline=True):
     """The 4,096-instance workload (n log-uniform in [64, 262144], L = 16
     sqrt(n), sides U[1,8], seed 4): one HIP-graph replay of the batch per
     step, against per-instance calls of the plain path on the same instances
@@ -177,6 +177,22 @@ def batched_leg(torch, steps, warmup):
     assert r == [total, 0], (r, total)
     assert np.array_equal(io_dev.cpu().numpy().astype(np.uint64), io)
 
+    batch_s = ms_batch / 1e3
+    out = {
+        "workload": "4,096 instances, n log-uniform [64, 262144], L = 16 sqrt(n), sides U[1,8], seed 4, "
+                    "boxes resident in HBM",
+        "instances": k, "boxes": N, "pairs": total, "chunks": info["n_chunks"], "bins": info["n_bins"],
+        "cell_sizes": "autotuner table (" + str(table.get("source", "default")) + ")",
+        "bin_target": int(table["bin_target"]),
+        "graph_replay_ms": ms_batch,
+        "graph_replay_host_wall_ms": wall_batch * 1e3,
+        "boxes_per_s": N / batch_s,
+        "pairs_per_s": total / batch_s,
+    }
+    if not baseline:
+        b.close()
+        eng.close()
+        return out
     # Per-instance plain path on the same device array and cell sizes.
     views = [d[int(offsets[i]):int(offsets[i + 1])] for i in range(k)]
     L = [float(x) for x in ranges]
@@ -215,23 +231,14 @@ def batched_leg(torch, steps, warmup):
     eng.close()
     del d, pairs, views
     torch.cuda.empty_cache()
-    batch_s = ms_batch / 1e3
-    return {
-        "workload": "4,096 instances, n log-uniform [64, 262144], L = 16 sqrt(n), sides U[1,8], seed 4, "
-                    "boxes resident in HBM",
-        "instances": k, "boxes": N, "pairs": total, "chunks": info["n_chunks"], "bins": info["n_bins"],
-        "cell_sizes": "autotuner table (" + str(table.get("source", "default")) + ")",
-        "bin_target": int(table["bin_target"]),
-        "graph_replay_ms": ms_batch,
-        "graph_replay_host_wall_ms": wall_batch * 1e3,
-        "boxes_per_s": N / batch_s,
-        "pairs_per_s": total / batch_s,
+    out.update({
         "per_instance_sync_ms": wall_sync * 1e3,
         "per_instance_async_ms": ms_async,
         "per_instance_async_host_wall_ms": wall_async * 1e3,
         "speedup_vs_per_instance_sync": wall_sync / batch_s,
         "speedup_vs_per_instance_async": (max(ms_async, wall_async * 1e3) / 1e3) / batch_s,
-    }
+    })
+    return out
 
 
 def serving_leg(torch, count, rate, budget_gb):
@@ -349,6 +356,8 @@ def main():
     ap.add_argument("--path", choices=("auto", "general", "binned"), default="auto")
     ap.add_argument("--bin-target", type=int, default=2000)
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-instance batch leg")
+    ap.add_argument("--no-batched-baseline", action="store_true",
+                    help="batch leg without the per-instance comparison loops (for profiling)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-clustered", action="store_true", help="skip the clustered 2M-box leg")
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -484,7 +493,7 @@ def main():
     del d_boxes, d_pairs
     torch.cuda.empty_cache()
     if not args.no_batched:
-        out["batched"] = batched_leg(torch, args.batched_steps, 2)
+        out["batched"] = batched_leg(torch, args.batched_steps, 2, baseline=not args.no_batched_baseline)
     if not args.no_clustered:
         out["clustered"] = clustered_leg(torch, pbig_amd.Engine)
     if not args.no_serving:
