@@ -186,38 +186,59 @@ __global__ __launch_bounds__(256) void k_bin_scan(const uint32_t* __restrict__ h
   }
 }
 
-// Exclusive scan of all bins' totals by one 1024-thread workgroup: bucket
-// offset of every bin, the grand total and the workspace check.
+// Exclusive scan of n values by one 1024-thread workgroup, in tiles of 4096:
+// each thread takes 4 consecutive entries (a wave reads one contiguous 1 KB
+// run), wave and block scans, the running total carried between tiles.
+// Returns the grand total (every thread).
+struct ScanLds {
+  uint64_t w[16];
+};
+
+template <typename Get, typename Put>
+__device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get, Put put) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < n; base += 4096) {
+    const uint32_t i0 = base + tid * 4;
+    uint64_t x[4], acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      x[k] = i0 + k < n ? get(i0 + k) : 0ull;
+      acc += x[k];
+    }
+    uint64_t inc = acc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t o = __shfl_up(inc, d, 64);
+      if (lane >= (uint32_t)d) inc += o;
+    }
+    if (lane == 63) L.w[w] = inc;
+    __syncthreads();
+    uint64_t ex = carry + inc - acc, tile = 0;
+    for (uint32_t k = 0; k < 16; ++k) {
+      if (k < w) ex += L.w[k];
+      tile += L.w[k];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      if (i0 + k < n) put(i0 + k, ex);
+      ex += x[k];
+    }
+    carry += tile;
+    __syncthreads();  // L.w is reused by the next tile
+  }
+  return carry;
+}
+
+// Bucket offset of every bin (exclusive scan of all bins' totals), the
+// grand total and the workspace check.
 __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
                                                      uint32_t* __restrict__ bin_start, uint64_t cap,
                                                      Meta* __restrict__ meta) {
-  __shared__ uint64_t wsum[16];
-  const uint32_t per = (n_bins + 1023) / 1024;
-  const uint32_t b0 = threadIdx.x * per;
-  uint64_t acc = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < n_bins) acc += total[b0 + k];
-  uint64_t v = acc;
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint64_t o = __shfl_up(v, d, 64);
-    if (lane >= (uint32_t)d) v += o;
-  }
-  if (lane == 63) wsum[w] = v;
-  __syncthreads();
-  uint64_t ex = v - acc, grand = 0;
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (k < w) ex += wsum[k];
-    grand += wsum[k];
-  }
-  for (uint32_t k = 0; k < per; ++k) {
-    const uint32_t b = b0 + k;
-    if (b < n_bins) {
-      bin_start[b] = (uint32_t)(ex < cap ? ex : cap);
-      ex += total[b];
-    }
-  }
+  __shared__ ScanLds L;
+  const uint64_t grand = block_exscan(
+      n_bins, L, [&](uint32_t i) { return (uint64_t)total[i]; },
+      [&](uint32_t i, uint64_t ex) { bin_start[i] = (uint32_t)(ex < cap ? ex : cap); });
   if (threadIdx.x == 0) {
     meta->memberships = grand;
     if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
@@ -438,7 +459,7 @@ struct TestLds {
   float4 box[kMaxFine];            // strip memberships in fine-cell order
   uint32_t word[kMaxFine];         // position words
   uint32_t id[kMaxCoarse];         // box id of each coarse membership
-  uint32_t cnt[kMaxCellsPerBin];   // fine-cell counts of the bin
+  uint32_t cnt[kMaxCellsPerBin];   // fine-cell counts of the bin, then the scatter cursors
   uint32_t start[kMaxCellsPerBin + 1];  // exclusive scan of cnt (start[kk] = total)
   uint2 stage[kTestWaves][kBinStage];   // per-wave pair staging
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
@@ -485,7 +506,8 @@ __device__ __forceinline__ FineRange fine_range(const float4& q, const FineGrid&
 }
 
 // Scatter the fine memberships of one coarse membership that fall in cells
-// [c_lo, c_hi) to their positions in the strip (start[] is the cursor).
+// [c_lo, c_hi) to their positions in the strip (cnt[] is the cursor, set to
+// start[] by the scan).
 __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint32_t j, const FineGrid& p,
                                               uint32_t gx, uint32_t gy, uint32_t c_lo, uint32_t c_hi,
                                               uint32_t base) {
@@ -497,7 +519,7 @@ __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint3
     for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
       const uint32_t c = cy * p.kx + cx;
       if (c < c_lo || c >= c_hi) continue;
-      const uint32_t pos = atomicAdd(&L.start[c], 1u) - base;
+      const uint32_t pos = atomicAdd(&L.cnt[c], 1u) - base;
       L.box[pos] = q;
       L.word[pos] = j | (c << kWordCellShift) | fy | ((gx + cx == r.fx0) ? kFlagX : 0u);
     }
@@ -525,6 +547,9 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
   uint32_t sn = 0;
   uint32_t seg0 = 0;  // region offset of the current bin's first pair (thread 0)
+  // cnt[] is zero at the start of every bin: cleared here, then after each
+  // bin's last scatter (and on the overflow exit)
+  for (uint32_t c = tid; c < kMaxCellsPerBin; c += kTestThreads) L.cnt[c] = 0u;
   if (tid == 0) L.out_n = 0u;
 
   // Registers: unconditional loads with the index clamped into the bucket.
@@ -551,9 +576,11 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
       __syncthreads();
       if (t == 0) {
         const uint64_t s0 = L.binfo[0].x;
+        const uint32_t j_hi = L.binfo[0].y ? L.binfo[0].y - 1u : 0u;
 #pragma unroll
         for (uint32_t k = 0; k < kRegs; ++k) {
-          const uint64_t at = s0 + tid + k * kTestThreads < last ? s0 + tid + k * kTestThreads : last;
+          const uint32_t j = tid + k * kTestThreads < j_hi ? tid + k * kTestThreads : j_hi;
+          const uint64_t at = s0 + j < last ? s0 + j : last;
           rb[k] = bbox[at];
           ri[k] = bid[at];
         }
@@ -561,15 +588,17 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
     }
     const uint32_t b = blockIdx.x + t * gridDim.x;
     const uint32_t m = L.binfo[slot].y;
+    // Next bin's registers: entries past its end repeat its last one (same
+    // cache line, no extra HBM traffic) so that every load is unconditional.
     const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
+    const uint32_t mN = t + 1 < my_bins ? total[b + gridDim.x] : 0u;
+    const uint32_t jN = mN ? mN - 1u : 0u;
     const uint4 bg = L.bingrid[slot];
     const uint32_t gx = bg.x, gy = bg.y, Kx = bg.z & 0xFFFFu, Ky = bg.z >> 16, kk = Kx * Ky;
     const FineGrid p{L.bininv[slot], bg.w, Kx, Ky};
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
-      for (uint32_t c = tid; c < kk; c += kTestThreads) L.cnt[c] = 0u;
-      __syncthreads();
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
@@ -603,27 +632,34 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t c = tid * 4 + k;
-          if (c < kk) L.start[c] = ex;
+          if (c < kk) {
+            L.start[c] = ex;
+            L.cnt[c] = ex;  // scatter cursor
+          }
           ex += v[k];
         }
         if (tid == kTestThreads - 1) L.start[kk] = ex;
       }
       __syncthreads();
       // C/D per strip
+      // start[] is read-only from here on
       uint32_t c_lo = 0;
       while (c_lo < kk) {
-        if (tid == 0) {
-          // largest c_hi with start[c_hi] - start[c_lo] <= kMaxFine
-          const uint32_t lim = L.start[c_lo] + kMaxFine;
-          uint32_t lo = c_lo, hi = kk;  // start[lo] <= lim; find max index with start <= lim
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (L.start[mid] <= lim) lo = mid; else hi = mid - 1;
+        uint32_t c_hi = kk;  // usually the rest of the bin fits in one strip
+        if (L.start[kk] - L.start[c_lo] > kMaxFine) {
+          if (tid == 0) {
+            // largest c_hi with start[c_hi] - start[c_lo] <= kMaxFine
+            const uint32_t lim = L.start[c_lo] + kMaxFine;
+            uint32_t lo = c_lo, hi = kk;  // start[lo] <= lim; find max index with start <= lim
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1) >> 1;
+              if (L.start[mid] <= lim) lo = mid; else hi = mid - 1;
+            }
+            L.strip_hi = lo;
           }
-          L.strip_hi = lo;
+          __syncthreads();
+          c_hi = L.strip_hi;
         }
-        __syncthreads();
-        const uint32_t c_hi = L.strip_hi;
         if (c_hi == c_lo) {  // one fine cell alone exceeds the LDS buffer
           if (tid == 0) {
             // drop the bin's pairs so far: the host reruns the whole bin
@@ -631,30 +667,31 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
             overfull[atomicAdd(&meta->max_per_cell, 1u)] = b | kOverfullCell;
             atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrOverfull);
           }
+          for (uint32_t c = tid; c < kMaxCellsPerBin; c += kTestThreads) L.cnt[c] = 0u;
+          __syncthreads();
           break;
         }
         const uint32_t base = L.start[c_lo];
         const uint32_t len = L.start[c_hi] - base;
-        __syncthreads();  // every thread has read start[] before it becomes a cursor
 #pragma unroll
         for (uint32_t k = 0; k < kRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads;
           if (j < m) scatter_strip(L, rb[k], j, p, gx, gy, c_lo, c_hi, base);
         }
         __syncthreads();
-        // restore start[] for the strip's cells (the cursors advanced by cnt)
-        for (uint32_t c = c_lo + tid; c < c_hi; c += kTestThreads) L.start[c] -= L.cnt[c];
         const bool last_strip = c_hi == kk;
+        if (last_strip)  // the cursors are dead: zero for the next bin
+          for (uint32_t c = tid; c < kMaxCellsPerBin; c += kTestThreads) L.cnt[c] = 0u;
         if (last_strip && t + 1 < my_bins) {
 #pragma unroll
           for (uint32_t k = 0; k < kRegs; ++k) {
-            const uint64_t at = sN + tid + k * kTestThreads < last ? sN + tid + k * kTestThreads : last;
+            const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
+            const uint64_t at = sN + j < last ? sN + j : last;
             rb[k] = bbox[at];
             ri[k] = bid[at];
           }
           prefetched = true;
         }
-        __syncthreads();
         // D. test every pair of each cell of the strip, one position per lane
         for (uint32_t p0 = w * 64; p0 < len; p0 += kTestThreads) {
           const uint32_t pp = p0 + lane;
@@ -664,7 +701,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
             aw = L.word[pp];
             const uint32_t c = (aw >> kWordCellShift) & kWordCellMask;
             cs = L.start[c] - base;
-            cn = L.cnt[c];
+            cn = L.start[c + 1] - L.start[c];
             const uint32_t r = pp - cs;
             // (n-1)/2 partners, plus one for the first half of an even cell
             h = (cn - 1 + (((cn & 1u) == 0u) & (r < (cn >> 1)))) >> 1;
@@ -731,7 +768,8 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
       __syncthreads();
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
-        const uint64_t at = sN + tid + k * kTestThreads < last ? sN + tid + k * kTestThreads : last;
+        const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
+        const uint64_t at = sN + j < last ? sN + j : last;
         rb[k] = bbox[at];
         ri[k] = bid[at];
       }
@@ -749,33 +787,9 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
 __global__ __launch_bounds__(1024) void k_bin_pair_scan(const uint2* __restrict__ bin_seg, uint32_t n_bins,
                                                         uint64_t* __restrict__ bin_out, uint64_t extra_total,
                                                         Meta* __restrict__ meta) {
-  __shared__ uint64_t wsum[16];
-  const uint32_t per = (n_bins + 1023) / 1024;
-  const uint32_t b0 = threadIdx.x * per;
-  uint64_t acc = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < n_bins) acc += bin_seg[b0 + k].y;
-  uint64_t v = acc;
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint64_t o = __shfl_up(v, d, 64);
-    if (lane >= (uint32_t)d) v += o;
-  }
-  if (lane == 63) wsum[w] = v;
-  __syncthreads();
-  uint64_t ex = v - acc, grand = 0;
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (k < w) ex += wsum[k];
-    grand += wsum[k];
-  }
-  for (uint32_t k = 0; k < per; ++k) {
-    const uint32_t b = b0 + k;
-    if (b < n_bins) {
-      bin_out[b] = ex;
-      ex += bin_seg[b].y;
-    }
-  }
+  __shared__ ScanLds L;
+  const uint64_t grand = block_exscan(
+      n_bins, L, [&](uint32_t i) { return (uint64_t)bin_seg[i].y; }, [&](uint32_t i, uint64_t ex) { bin_out[i] = ex; });
   if (threadIdx.x == 0) {
     bin_out[n_bins] = grand;
     meta->n_pairs = grand + extra_total;  // the true count, also past the capacity
