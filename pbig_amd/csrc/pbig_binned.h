@@ -60,6 +60,7 @@ struct BatchPlan {
   uint32_t n_inst = 0, n_chunks = 0, n_bins = 0, test_blocks = 0;
   uint64_t n_boxes = 0, hist_entries = 0, off_entries = 0;
   uint64_t extra_total = 0;  // pairs found outside the launch (sum of pairs_extra)
+  uint32_t nb_max = 0;       // bins of the largest instance (sizes the scatter's LDS)
 };
 
 struct BinnedWorkspace {

@@ -254,15 +254,19 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
 // same short window and merge in that XCD's L2.
 constexpr uint32_t kScatterThreads = 1024;
 constexpr uint32_t kScatterPer = 8;
-constexpr uint32_t kMaxRounds = 1024;
+constexpr uint32_t kMaxRounds = 512;
 constexpr uint32_t kStageEntries = 2048;
 
+// Fixed part of the scatter's LDS; after it, sized per launch to the
+// batch's largest instance (nb_cap bins), come
+//   uint32_t cur[nb_cap]   next destination of each bin (global index)
+//   int32_t woff[nb_cap]   staging position of a destination = dst + woff - round base
+// so that two 1024-thread workgroups share a CU when instances have up to
+// ~2,900 bins.
 struct ScatterLds {
   float4 box[kStageEntries];
   uint32_t id[kStageEntries];
   uint32_t dst[kStageEntries];
-  uint32_t cur[kMaxBins];          // next destination of each bin (global index)
-  int32_t woff[kMaxBins];          // staging position of a destination = dst + woff - round base
   uint32_t round_hi[kMaxRounds];   // first bin after round r
   uint32_t round_n[kMaxRounds];    // memberships of this chunk in round r
   uint32_t round_base[kMaxRounds]; // memberships of this chunk before round r (bin order)
@@ -288,9 +292,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
                                                                  const uint32_t* __restrict__ bin_start,
                                                                  float4* __restrict__ bbox,
                                                                  uint32_t* __restrict__ bid, uint64_t cap,
-                                                                 Meta* __restrict__ meta) {
+                                                                 uint32_t nb_cap, Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   ScatterLds& L = *reinterpret_cast<ScatterLds*>(lds_raw);
+  uint32_t* const cur = reinterpret_cast<uint32_t*>(lds_raw + sizeof(ScatterLds));
+  int32_t* const woff = reinterpret_cast<int32_t*>(cur + nb_cap);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (blockIdx.x >= n_chunks) return;
   if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
@@ -311,6 +317,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   const uint32_t* myoff = off + I.off0 + lc * nb;
   const uint32_t* myhist = hist + I.hist0 + lc * (nb + 1);
   const uint32_t* mystart = bin_start + I.bin0;
+  uint32_t log_h;  // a bin's round is its chunk-local position >> log_h (uniform)
   {
     const uint32_t per = (nb + kScatterThreads - 1) / kScatterThreads;
     const uint32_t b0 = tid * per;
@@ -331,8 +338,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       const uint32_t b = b0 + k;
       if (b < nb) {
         const uint32_t dst0 = mystart[b] + myoff[b];
-        L.cur[b] = dst0;
-        L.woff[b] = (int32_t)(ex - dst0);
+        cur[b] = dst0;
+        woff[b] = (int32_t)(ex - dst0);
         ex += myhist[b];
       }
     }
@@ -344,17 +351,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     __syncthreads();
     uint32_t chunk_total = 0;
     for (uint32_t k = 0; k < kScatterThreads / 64; ++k) chunk_total += (uint32_t)L.wsum[k];
-    uint32_t log_h = 0;
+    log_h = 0;
     while ((1u << (log_h + 1)) <= kStageEntries / 2) ++log_h;
     while ((chunk_total >> log_h) >= kMaxRounds - 1) ++log_h;
     // rounds 0 .. (wpos of the last bin) >> log_h; each is started by the first
     // bin whose round is at least its index, so every entry below is set
-    const uint32_t wp_last = (uint32_t)((int32_t)L.cur[nb - 1] + L.woff[nb - 1]);
+    const uint32_t wp_last = (uint32_t)((int32_t)cur[nb - 1] + woff[nb - 1]);
     const uint32_t n_rounds = chunk_total ? (wp_last >> log_h) + 1 : 0;
     for (uint32_t b = tid; b < nb; b += kScatterThreads) {
-      const uint32_t wp = (uint32_t)((int32_t)L.cur[b] + L.woff[b]);
+      const uint32_t wp = (uint32_t)((int32_t)cur[b] + woff[b]);
       const uint32_t rb = wp >> log_h;
-      const uint32_t rprev = b ? ((uint32_t)((int32_t)L.cur[b - 1] + L.woff[b - 1]) >> log_h) : 0xFFFFFFFFu;
+      const uint32_t rprev = b ? ((uint32_t)((int32_t)cur[b - 1] + woff[b - 1]) >> log_h) : 0xFFFFFFFFu;
       // bin b starts rounds rprev+1 .. rb (empty rounds share its start)
       for (uint32_t rr = rprev + 1; rr <= rb && rr < n_rounds; ++rr) {
         L.round_hi[rr] = b;  // start of round rr
@@ -380,6 +387,22 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     const bool ok = i < end && (b.x <= b.y) && (b.z <= b.w);
     rng[k] = ok ? pack_range(bin_range(b, I)) : 0xFFFFFFFFu;
   }
+  // 4b. the first and last round of each box: its bins are the bin-order
+  //     range [y0 * nbx + x0, y1 * nbx + x1] and rounds are ranges of bins,
+  //     read before the rounds move the cursors
+  uint32_t rr[kScatterPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kScatterPer; ++k) {
+    const uint32_t pr = rng[k];
+    rr[k] = 0x0000FFFFu;  // no round
+    if (pr != 0xFFFFFFFFu) {
+      const uint32_t b0 = ((pr >> 16) & 0xFF) * I.nbx + (pr & 0xFF), b1 = (pr >> 24) * I.nbx + ((pr >> 8) & 0xFF);
+      const uint32_t r0 = (uint32_t)((int32_t)cur[b0] + woff[b0]) >> log_h;
+      const uint32_t r1 = (uint32_t)((int32_t)cur[b1] + woff[b1]) >> log_h;
+      rr[k] = r0 | (r1 << 16);
+    }
+  }
+  __syncthreads();  // every wave has read the cursors before the rounds advance them
   // 5. rounds
   const uint32_t n_rounds = L.n_rounds;
   for (uint32_t r = 0; r < n_rounds; ++r) {
@@ -390,18 +413,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     const int32_t base = (int32_t)L.round_base[r];
 #pragma unroll
     for (uint32_t k = 0; k < kScatterPer; ++k) {
+      if (r < (rr[k] & 0xFFFFu) || r > (rr[k] >> 16)) continue;
       const uint32_t pr = rng[k];
-      if (pr == 0xFFFFFFFFu) continue;
       const uint32_t x0 = pr & 0xFF, x1 = (pr >> 8) & 0xFF, y0 = (pr >> 16) & 0xFF, y1 = pr >> 24;
-      if (y1 * I.nbx + x1 < b_lo || y0 * I.nbx + x0 >= b_hi) continue;
       uint32_t i = (uint32_t)(start - I.box0) + tid + k * kScatterThreads;  // id inside the instance
       if (kIds) i = ids_in[start + tid + k * kScatterThreads];
       for (uint32_t cy = y0; cy <= y1; ++cy)
         for (uint32_t cx = x0; cx <= x1; ++cx) {
           const uint32_t b = cy * I.nbx + cx;
           if (b < b_lo || b >= b_hi) continue;
-          const uint32_t d = atomicAdd(&L.cur[b], 1u);
-          const uint32_t at = (uint32_t)((int32_t)d + L.woff[b] - base);
+          const uint32_t d = atomicAdd(&cur[b], 1u);
+          const uint32_t at = (uint32_t)((int32_t)d + woff[b] - base);
           if (direct || at >= kStageEntries) {
             if (d < cap) {
               bbox[d] = q[k];
@@ -855,11 +877,12 @@ hipError_t binned_init() {
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(TestLds));
   if (e != hipSuccess) return e;
+  const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
   e = hipFuncSetAttribute((const void*)k_bin_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)sizeof(ScatterLds));
+                          scatter_max);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_bin_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)sizeof(ScatterLds));
+                             scatter_max);
 }
 
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
@@ -889,14 +912,16 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(2)
   k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, ws.meta);
   PBIG_MARK(3)
+  const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
+  const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;
   if (ids_in)
-    k_bin_scatter<true><<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(
+    k_bin_scatter<true><<<nc, kScatterThreads, scatter_lds, s>>>(
         boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
-        ws.cap_bucket, ws.meta);
+        ws.cap_bucket, nb_cap, ws.meta);
   else
-    k_bin_scatter<false><<<nc, kScatterThreads, sizeof(ScatterLds), s>>>(
+    k_bin_scatter<false><<<nc, kScatterThreads, scatter_lds, s>>>(
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
-        ws.cap_bucket, ws.meta);
+        ws.cap_bucket, nb_cap, ws.meta);
   PBIG_MARK(4)
   k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,

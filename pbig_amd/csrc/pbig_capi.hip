@@ -351,6 +351,7 @@ bool layout_tables(BinnedState& st, uint32_t n_cu, const std::vector<uint64_t>* 
     pairs0 += d.pairs_extra;
     chunks += nchunks;
     bins += nb;
+    if (nchunks) bp.nb_max = std::max(bp.nb_max, d.nb);
     hist += (uint64_t)nchunks * (d.nb + 1);
     off += (uint64_t)nchunks * d.nb;
     bp.n_boxes += skipped ? 0u : d.n;
