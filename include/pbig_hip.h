@@ -222,6 +222,25 @@ int pbig_hip_batch_replay(pbig_hip_batch* batch, void* stream);
  * reference). */
 int pbig_hip_batch_info(const pbig_hip_batch* batch, uint64_t* n_boxes, uint32_t* n_chunks, uint32_t* n_bins);
 
+/* ------------------------------------------------------------------------
+ * Reference-compatible unsigned mode: the reference's shipped build
+ * (CType = unsigned int, DIMS = 2 or 3; src/pbig/rect.h:4-12,
+ * config.cmake:8), restated literally: cells v >> log2(cs), 32-bit cell ids
+ * with out-of-grid ids skipped, unsigned <= overlap tests and the
+ * reference-cell rule by cell id (pbig.cu:174-221, 562-603).  Boxes are
+ * n x 2*dims uint32 {xmin,xmax,ymin,ymax[,zmin,zmax]}; cell_size is a power of
+ * two (0 = the reference's rule on the first 1000 boxes).  With
+ * pair_capacity 0 the pairs are only counted.
+ * ------------------------------------------------------------------------ */
+uint32_t pbig_hip_reference_cell_size_u32(const uint32_t* h_boxes, uint32_t n, int dims, uint32_t range);
+int pbig_hip_check_u32(pbig_hip_ctx* ctx, const uint32_t* d_boxes, uint32_t n, int dims, uint32_t range,
+                       uint32_t cell_size, uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* n_pairs,
+                       void* stream);
+/* Host records: n x RectInfo {CoordInfo (2*dims uint32), rid} with rid ==
+ * index, as pbig_check receives them in the reference's build. */
+int pbig_hip_check_host_u32(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                            uint32_t cell_size, uint32_t* h_pairs, uint64_t pair_capacity, uint64_t* n_pairs);
+
 #ifdef __cplusplus
 }
 #endif

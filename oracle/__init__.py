@@ -22,6 +22,8 @@ Pinning (each function below is checked in ``tests/test_oracle.py``):
     answers (benchmarks/test1/filter_file:7 -> 49,514,252 pairs;
     benchmarks/test3/filter_file:17-49 -> cs, cells, memberships, max per
     cell, jobs, compared pairs and 1,121,815,231 pairs).
+  * ``brute_u32`` -- the same predicate by brute force; must equal
+    ``grid_u32``'s count on in-range inputs.
 """
 from __future__ import annotations
 
@@ -82,6 +84,7 @@ def _load():
         "orc_ref_generate": (None, [u32, u32, u32, ctypes.c_char_p, i32, P]),
         "orc_decide_cell_size_u32": (u32, [P, u32, i32, u32]),
         "orc_grid_u32": (u64, [P, u32, i32, u32, u32, i32, P]),
+        "orc_brute_u32": (u64, [P, u32, i32, P, u64, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -173,6 +176,17 @@ def ref_generate(n: int, range_: int, size: int, simcase: str, dims: int = 3) ->
     out = np.empty((n, 2 * dims), dtype=np.uint32)
     _load().orc_ref_generate(n, range_, size, simcase.encode(), dims, out.ctypes.data)
     return out
+
+
+def brute_u32(boxes: np.ndarray, dims: int, threads: int = 0) -> np.ndarray:
+    """Every overlapping pair (lo, hi) of unsigned boxes (n, 2*dims) by brute force."""
+    b = np.ascontiguousarray(boxes, dtype=np.uint32).reshape(-1, 2 * dims)
+    lib = _load()
+    n = b.shape[0]
+    cnt = lib.orc_brute_u32(b.ctypes.data, n, dims, None, 0, threads)
+    out = np.empty((max(int(cnt), 1), 2), dtype=np.uint32)
+    lib.orc_brute_u32(b.ctypes.data, n, dims, out.ctypes.data, int(cnt), threads)
+    return out[: int(cnt)]
 
 
 def decide_cell_size_u32(boxes: np.ndarray, dims: int, range_: int) -> int:

@@ -472,4 +472,28 @@ uint64_t orc_grid_u32(const uint32_t* boxes, uint32_t n, int dims, uint32_t rang
   return total;
 }
 
+// All-pairs brute force for unsigned boxes of `dims` dimensions with the
+// reference's predicate (every dimension's two unsigned <= tests,
+// pbig.cu:562-577); the pair set any correct grid run must report.  O(n^2).
+uint64_t orc_brute_u32(const uint32_t* boxes, uint32_t n, int dims, uint32_t* out, uint64_t cap, int threads) {
+  const int D2 = 2 * dims;
+  int T = nthreads(threads);
+  std::vector<std::vector<uint32_t>> parts(T);
+#pragma omp parallel num_threads(T)
+  {
+    auto& mine = parts[omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+      const uint32_t* a = boxes + (size_t)i * D2;
+      for (uint32_t j = (uint32_t)i + 1; j < n; ++j) {
+        const uint32_t* o = boxes + (size_t)j * D2;
+        bool ov = true;
+        for (int d = 0; d < dims; ++d) ov = ov && (a[2 * d] <= o[2 * d + 1]) && (o[2 * d] <= a[2 * d + 1]);
+        if (ov) { mine.push_back((uint32_t)i); mine.push_back(j); }
+      }
+    }
+  }
+  return gather(parts, out, cap);
+}
+
 }  // extern "C"

@@ -50,6 +50,9 @@ EXPORTS = (
     "pbig_hip_batch_capture",
     "pbig_hip_batch_replay",
     "pbig_hip_batch_info",
+    "pbig_hip_reference_cell_size_u32",
+    "pbig_hip_check_u32",
+    "pbig_hip_check_host_u32",
 )
 
 PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
@@ -148,6 +151,9 @@ def load_library(path: str | None = None):
         "pbig_hip_batch_capture": (i32, [P, P, u64, P, u64, P, P]),
         "pbig_hip_batch_replay": (i32, [P, P]),
         "pbig_hip_batch_info": (i32, [P, ctypes.POINTER(u64), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "pbig_hip_reference_cell_size_u32": (u32, [P, u32, i32, u32]),
+        "pbig_hip_check_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64), P]),
+        "pbig_hip_check_host_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -162,6 +168,12 @@ def reference_cell_size(boxes: np.ndarray, range_: float) -> int:
     """The reference's decideCellSize (src/pbig/pbig.cu:1285-1357), via the C-ABI."""
     b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 4)
     return int(load_library().pbig_hip_reference_cell_size(b.ctypes.data, b.shape[0], float(range_)))
+
+
+def reference_cell_size_u32(boxes: np.ndarray, dims: int, range_: int) -> int:
+    """The reference's decideCellSize for its unsigned build, via the C-ABI."""
+    b = np.ascontiguousarray(boxes, dtype=np.uint32).reshape(-1, 2 * dims)
+    return int(load_library().pbig_hip_reference_cell_size_u32(b.ctypes.data, b.shape[0], int(dims), int(range_)))
 
 
 def _ptr(t) -> int:
@@ -278,6 +290,45 @@ class Engine:
                                                    _stream_ptr(stream))
         if rc != PBIG_OK:
             self._raise(rc)
+
+    # -- the reference's unsigned build (CType = unsigned int, DIMS 2 or 3) ---
+    def check_u32(self, boxes, dims: int, range_: int, cell_size: int = 0, pairs=None, stream=None,
+                  allow_overflow: bool = False) -> int:
+        """boxes: (n, 2*dims) int32/uint32 device tensor {xmin,xmax,ymin,ymax[,zmin,zmax]};
+        pairs: (cap, 2) device tensor, or None to count only.  Returns the pair count."""
+        n = int(boxes.shape[0])
+        cap = int(pairs.shape[0]) if pairs is not None else 0
+        out = ctypes.c_uint64(0)
+        rc = self._lib.pbig_hip_check_u32(self._h, _ptr(boxes) if n else None, n, int(dims), int(range_),
+                                          int(cell_size), _ptr(pairs) if cap else None, cap, ctypes.byref(out),
+                                          _stream_ptr(stream))
+        if rc == PBIG_E_OVERFLOW and allow_overflow:
+            return int(out.value)
+        if rc != PBIG_OK:
+            self._raise(rc)
+        return int(out.value)
+
+    def check_host_u32(self, boxes: np.ndarray, dims: int, range_: int, cell_size: int = 0,
+                       capacity: int | None = None) -> np.ndarray:
+        """boxes: (n, 2*dims) uint32 host array (rid = index).  Returns (k, 2) uint32 pairs."""
+        b = np.ascontiguousarray(boxes, dtype=np.uint32).reshape(-1, 2 * dims)
+        n = b.shape[0]
+        rec = np.empty((n, 2 * dims + 1), dtype=np.uint32)
+        rec[:, :2 * dims] = b
+        rec[:, 2 * dims] = np.arange(n, dtype=np.uint32)
+        cap = int(capacity) if capacity is not None else max(1024, 2 * n)
+        while True:
+            out = np.empty((max(cap, 1), 2), dtype=np.uint32)
+            cnt = ctypes.c_uint64(0)
+            rc = self._lib.pbig_hip_check_host_u32(self._h, rec.ctypes.data if n else None, n, int(dims),
+                                                   int(range_), int(cell_size), out.ctypes.data, cap,
+                                                   ctypes.byref(cnt))
+            if rc == PBIG_E_OVERFLOW and capacity is None:
+                cap = int(cnt.value)
+                continue
+            if rc != PBIG_OK:
+                self._raise(rc)
+            return out[: int(cnt.value)]
 
     def check_host(self, rects, range_: float, cell_size: float = 0.0, capacity: int | None = None):
         """rects: RectInfo records (RECT_DTYPE) or an (n, 4) float32 array

@@ -2,7 +2,9 @@
 
   libpbig_hip.so  HIP kernels for gfx950 + the C-ABI of include/pbig_hip.h
   libpbig.so      the reference-compatible C++ API (include/pbig.h,
-                  report.h, rect.h) on top of the C-ABI
+                  report.h, rect.h) on top of the C-ABI, float 2-D records
+  libpbig_u32.so  the same API for the reference's shipped build: unsigned
+                  3-D records (rect.h with PBIG_COMPAT_U32)
 
 Both are written in-tree next to this file so that they travel with the
 source tree.  hipcc cross-compiles for gfx950 without a GPU.
@@ -33,9 +35,10 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-HIP_SOURCES = ["pbig_kernels.hip", "pbig_binned.hip", "pbig_capi.hip"]
+HIP_SOURCES = ["pbig_kernels.hip", "pbig_binned.hip", "pbig_u32.hip", "pbig_capi.hip"]
 LIB_HIP = os.path.join(HERE, "libpbig_hip.so")
 LIB_API = os.path.join(HERE, "libpbig.so")
+LIB_API_U32 = os.path.join(HERE, "libpbig_u32.so")
 
 
 def _run(cmd):
@@ -57,6 +60,11 @@ def build(verbose: bool = True) -> None:
           os.path.join(CSRC, "pbig_api.cpp"), f"-L{HERE}", "-lpbig_hip",
           "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, LIB_API)
+    tmp = LIB_API_U32 + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-DPBIG_COMPAT_U32", "-DDIMS=3",
+          f"-I{INCLUDE}", os.path.join(CSRC, "pbig_api.cpp"), f"-L{HERE}", "-lpbig_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, LIB_API_U32)
     for o in objs:
         os.remove(o)
 

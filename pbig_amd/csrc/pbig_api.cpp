@@ -63,13 +63,20 @@ void pbig_check(RectInfo* input, report* r) {
     exit(-1);
   }
   unsigned long long cap = g_pairs.size() / 2, np = 0;
-  int rc = pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs,
-                               g_pairs.empty() ? nullptr : g_pairs.data(), cap, (uint64_t*)&np);
+  // One entry point per build of rect.h: the native float 2-D records, or the
+  // reference's unsigned records (DIMS 2 or 3).
+  auto run = [&](unsigned int* out, unsigned long long c) {
+#if defined(PBIG_COMPAT_U32)
+    return pbig_hip_check_host_u32(g_ctx, input, g_n, DIMS, g_range, g_cs, out, c, (uint64_t*)&np);
+#else
+    return pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs, out, c, (uint64_t*)&np);
+#endif
+  };
+  int rc = run(g_pairs.empty() ? nullptr : g_pairs.data(), cap);
   if (rc == PBIG_E_OVERFLOW) {
     g_pairs.resize(2 * (np + np / 8 + 1024));
     cap = g_pairs.size() / 2;
-    rc = pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs, g_pairs.data(), cap,
-                             (uint64_t*)&np);
+    rc = run(g_pairs.data(), cap);
   }
   if (rc) die("pbig_check", rc);
   for (unsigned long long k = 0; k < np; ++k) (*r)(g_pairs[2 * k], g_pairs[2 * k + 1]);

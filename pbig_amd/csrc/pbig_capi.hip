@@ -17,6 +17,7 @@
 #include "pbig_binned.h"
 #include "pbig_device.h"
 #include "pbig_hip.h"
+#include "pbig_u32.h"
 
 using pbig::GridParams;
 using pbig::Meta;
@@ -73,6 +74,10 @@ struct pbig_hip_ctx {
   uint2* d_pairs = nullptr;
   uint64_t cap_pairs = 0;
   BinnedState plain;
+  // Unsigned (reference-compatible) mode: memberships and host staging.
+  pbig::U32Workspace uw;
+  uint32_t* d_in32 = nullptr;
+  uint64_t cap_in32 = 0;
   // Pinned ring for the plain path's instance record: uploads stay ordered
   // on the stream without a host synchronisation.
   InstDesc* h_ring = nullptr;
@@ -856,6 +861,8 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
                   (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs})
     if (p) (void)hipFree(p);
   free_state(c->plain);
+  for (void* p : {(void*)c->uw.mxy, (void*)c->uw.mz, (void*)c->uw.mid, (void*)c->uw.mcell, (void*)c->d_in32})
+    if (p) (void)hipFree(p);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ring_ev)
     if (e) (void)hipEventDestroy(e);
@@ -1383,3 +1390,185 @@ int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference-compatible unsigned mode.
+namespace {
+
+int ensure_u32(pbig_hip_ctx* c, uint64_t memb) {
+  pbig::U32Workspace& u = c->uw;
+  if (memb > 0xFFFFFFFFull) return fail(c, PBIG_E_NOMEM, "more than 2^32-1 cell memberships");
+  if (u.cap_memb >= memb && u.mxy) return PBIG_OK;
+  for (void* p : {(void*)u.mxy, (void*)u.mz, (void*)u.mid, (void*)u.mcell})
+    if (p) HIP_TRY(c, hipFree(p));
+  u = pbig::U32Workspace{};
+  uint64_t d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+  int rc;
+  if ((rc = grow(c, &u.mxy, &d1, memb)) || (rc = grow(c, &u.mz, &d2, memb)) || (rc = grow(c, &u.mid, &d3, memb)) ||
+      (rc = grow(c, &u.mcell, &d4, memb)))
+    return rc;
+  u.cap_memb = memb;
+  return PBIG_OK;
+}
+
+int check_u32(pbig_hip_ctx* c, const uint32_t* d_boxes, uint32_t n, int dims, uint32_t range, uint32_t cs,
+              uint2* d_out, uint64_t cap, uint64_t* n_pairs, hipStream_t s) {
+  if (cs == 0 || (cs & (cs - 1)) != 0) return fail(c, PBIG_E_ARG, "cell size %u is not a power of two", cs);
+  if (range == 0) return fail(c, PBIG_E_ARG, "range must be > 0");
+  pbig::U32Params p;
+  p.dims = (uint32_t)dims;
+  p.shift = 0;
+  while ((1u << p.shift) < cs) ++p.shift;
+  p.nrc = (uint32_t)(((uint64_t)range + cs - 1) / cs);
+  p.nrc2 = p.nrc * p.nrc;
+  const uint64_t cells = dims == 2 ? (uint64_t)p.nrc * p.nrc : (uint64_t)p.nrc * p.nrc * p.nrc;
+  if (cells > 0x7FFFFFFFull)
+    return fail(c, PBIG_E_ARG, "grid of %llu cells is too large (cell size %u, range %u)",
+                (unsigned long long)cells, cs, range);
+  p.n_cells = (uint32_t)cells;
+  int rc;
+  if ((rc = ensure_cells(c, p.n_cells))) return rc;
+  if ((rc = grow(c, &c->ws.items, &c->ws.cap_items, std::max<uint64_t>(c->ws.cap_items, p.n_cells / 2 + 1024))))
+    return rc;
+  if ((rc = ensure_u32(c, std::max<uint64_t>(c->uw.cap_memb, (uint64_t)n * 2 + 1024)))) return rc;
+  const bool count_only = cap == 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    HIP_TRY(c, pbig::launch_u32(c->ws, c->uw, d_boxes, n, p, d_out, cap, count_only, s));
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, c->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const Meta m = *c->h_meta;
+    if (m.error & pbig::kErrTooDense)
+      return fail(c, PBIG_E_TOO_DENSE, "a cell holds %u boxes, above the %u-tile limit", m.max_per_cell,
+                  pbig::kMaxTilesPerCell);
+    if (m.error & pbig::kErrWorkspace) {
+      if (attempt == 2) return fail(c, PBIG_E_NOMEM, "workspace did not fit after regrowth");
+      if ((rc = ensure_u32(c, m.memberships + m.memberships / 16 + 1024))) return rc;
+      if ((rc = grow(c, &c->ws.items, &c->ws.cap_items, m.items + m.items / 16 + 1024))) return rc;
+      continue;
+    }
+    c->stats = pbig_hip_stats{};
+    c->stats.cell_size_num = cs;
+    c->stats.cell_size_den = 1;
+    c->stats.cells_per_row = p.nrc;
+    c->stats.n_boxes = n;
+    c->stats.n_cells = p.n_cells;
+    c->stats.memberships = m.memberships;
+    c->stats.tile_pairs = m.items;
+    c->stats.n_pairs = m.n_pairs;
+    c->stats.max_per_cell = m.max_per_cell;
+    c->stats.path = PBIG_PATH_GENERAL;
+    *n_pairs = m.n_pairs;
+    if (!count_only && m.n_pairs > cap)
+      return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)m.n_pairs,
+                  (unsigned long long)cap);
+    return PBIG_OK;
+  }
+  return fail(c, PBIG_E_HIP, "unreachable");
+}
+
+}  // namespace
+
+extern "C" {
+
+// decideCellSize (pbig.cu:1285-1357) for unsigned boxes of `dims`
+// dimensions: the extent xmax - xmin is taken in unsigned arithmetic and
+// converted to float (avgX, pbig.cu:1255-1260), accumulated with the int
+// init value of the reference's transform_reduce, divided by the sample
+// count in integers; then the loop of the float rule with NMR_BOUND = 4
+// (2-D) or 8 (3-D) and nc = nrc^dims, the third factor multiplying the double
+// nmr (pbig.cu:1280-1325).
+uint32_t pbig_hip_reference_cell_size_u32(const uint32_t* h_boxes, uint32_t n, int dims, uint32_t range) {
+  int64_t cs = 1;
+  while (cs * 2 < (int64_t)range) cs *= 2;
+  const int samples = n < 1000 ? (int)n : 1000;
+  if (!h_boxes || samples == 0 || (dims != 2 && dims != 3)) return (uint32_t)cs;
+  int acc[3] = {0, 0, 0};
+  for (int k = 0; k < samples; ++k) {
+    const uint32_t* b = h_boxes + (size_t)k * 2 * dims;
+    for (int d = 0; d < dims; ++d) acc[d] = (int)((float)acc[d] + (float)(b[2 * d + 1] - b[2 * d]));
+  }
+  const float x = (float)(acc[0] / samples), y = (float)(acc[1] / samples), z = (float)(acc[2] / samples);
+  const double bound = dims == 2 ? 4.0 : 8.0;
+  while (true) {
+    const int64_t nrc = ((int64_t)range + cs - 1) / cs;
+    int64_t nc = nrc * nrc;
+    double nmr = (double)((1 + x / (float)cs) * (1 + y / (float)cs));
+    if (dims == 3) {
+      nc *= nrc;
+      nmr *= (double)(1 + z / (float)cs);
+    }
+    const int per_cell = (int)(nmr * (double)n / (double)nc);
+    if ((float)cs < x) break;
+    if (per_cell < 32 || nmr > bound) break;
+    if (cs == 1) break;
+    cs /= 2;
+  }
+  return (uint32_t)(cs * 2);
+}
+
+int pbig_hip_check_u32(pbig_hip_ctx* c, const uint32_t* d_boxes, uint32_t n, int dims, uint32_t range,
+                       uint32_t cell_size, uint32_t* d_pairs, uint64_t cap, uint64_t* n_pairs, void* stream) {
+  if (!c) return PBIG_E_ARG;
+  if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
+  if (dims != 2 && dims != 3) return fail(c, PBIG_E_ARG, "dims must be 2 or 3");
+  if (n && !d_boxes) return fail(c, PBIG_E_ARG, "d_boxes is null");
+  if (cap && !d_pairs) return fail(c, PBIG_E_ARG, "d_pairs is null with a nonzero capacity");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = pick_stream(c, stream);
+  if (cell_size == 0) {
+    std::vector<uint32_t> head((size_t)std::min<uint32_t>(n, 1000) * 2 * dims);
+    if (!head.empty()) {
+      HIP_TRY(c, hipMemcpyAsync(head.data(), d_boxes, head.size() * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+    }
+    cell_size = pbig_hip_reference_cell_size_u32(head.data(), n, dims, range);
+  }
+  return check_u32(c, d_boxes, n, dims, range, cell_size, (uint2*)d_pairs, cap, n_pairs, s);
+}
+
+int pbig_hip_check_host_u32(pbig_hip_ctx* c, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                            uint32_t cell_size, uint32_t* h_pairs, uint64_t cap, uint64_t* n_pairs) {
+  if (!c) return PBIG_E_ARG;
+  if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
+  if (dims != 2 && dims != 3) return fail(c, PBIG_E_ARG, "dims must be 2 or 3");
+  if (n && !h_rects) return fail(c, PBIG_E_ARG, "h_rects is null");
+  if (cap && !h_pairs) return fail(c, PBIG_E_ARG, "h_pairs is null with a nonzero capacity");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // RectInfo {CoordInfo (2*dims unsigned), rid}: check rid == index, copy the
+  // coordinates out.
+  const size_t w = 2 * (size_t)dims;
+  std::vector<uint32_t> coords((size_t)n * w);
+  const uint32_t* src = (const uint32_t*)h_rects;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t* r = src + (size_t)i * (w + 1);
+    if (r[w] != i) return fail(c, PBIG_E_ARG, "rect %u has rid %u; rid must equal the index", i, r[w]);
+    memcpy(coords.data() + (size_t)i * w, r, w * 4);
+  }
+  int rc;
+  if ((rc = grow(c, &c->d_in32, &c->cap_in32, std::max<uint64_t>((uint64_t)n * w, 1)))) return rc;
+  if (n) HIP_TRY(c, hipMemcpyAsync(c->d_in32, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, s));
+  if (cell_size == 0) cell_size = pbig_hip_reference_cell_size_u32(coords.data(), n, dims, range);
+  uint64_t want = std::max<uint64_t>(c->cap_pairs, std::max<uint64_t>(1024, (uint64_t)n));
+  if ((rc = grow(c, &c->d_pairs, &c->cap_pairs, want))) return rc;
+  uint64_t np = 0;
+  rc = check_u32(c, c->d_in32, n, dims, range, cell_size, c->d_pairs, c->cap_pairs, &np, s);
+  if (rc == PBIG_E_OVERFLOW) {
+    if ((rc = grow(c, &c->d_pairs, &c->cap_pairs, np + np / 8 + 1024))) return rc;
+    rc = check_u32(c, c->d_in32, n, dims, range, cell_size, c->d_pairs, c->cap_pairs, &np, s);
+  }
+  if (rc) return rc;
+  *n_pairs = np;
+  const uint64_t ncopy = std::min<uint64_t>(np, cap);
+  if (ncopy) {
+    HIP_TRY(c, hipMemcpyAsync(h_pairs, c->d_pairs, ncopy * sizeof(uint2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+  }
+  if (np > cap)
+    return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)np,
+                (unsigned long long)cap);
+  return PBIG_OK;
+}
+
+}  // extern "C"
+

@@ -65,6 +65,12 @@ constexpr int kPlainKernels = 6;
 
 // Optional per-kernel timing: when `events` is non-null, events[k] is
 // recorded before kernel k and events[kPlainKernels] after the last one.
+// The general path's first and scan stages, shared with the unsigned mode:
+// clear the cell counters and result words; scan the counts into starts and
+// 64 x 64 tile-pair items (cnt is reset to serve as the scatter cursor).
+hipError_t launch_clear(const Workspace& ws, uint32_t n_cells, hipStream_t s);
+hipError_t launch_scan(const Workspace& ws, uint32_t n_cells, uint64_t memb_cap, hipStream_t s);
+
 hipError_t launch_plain(const Workspace& ws, const float4* boxes, uint32_t n, GridParams gp,
                         uint32_t n_cells, uint2* out, uint64_t cap, hipStream_t s,
                         hipEvent_t* events = nullptr);

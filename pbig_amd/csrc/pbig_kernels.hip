@@ -345,6 +345,20 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
 // Host launcher of the plain path: six kernels on one stream, no host sync.
 namespace pbig {
 
+hipError_t launch_clear(const Workspace& ws, uint32_t n_cells, hipStream_t s) {
+  const uint32_t nb_cells = (n_cells + 255) / 256;
+  k_clear<<<nb_cells ? nb_cells : 1, 256, 0, s>>>(ws.cnt, n_cells, ws.meta);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const Workspace& ws, uint32_t n_cells, uint64_t memb_cap, hipStream_t s) {
+  const uint32_t n_partials = (n_cells + kScanTile - 1) / kScanTile;
+  k_scan_reduce<<<n_partials, 256, 0, s>>>(ws.cnt, n_cells, ws.partials);
+  k_scan_apply<<<n_partials, 256, 0, s>>>(ws.cnt, n_cells, ws.partials, n_partials, ws.start, ws.items,
+                                          ws.cap_items, memb_cap, ws.meta);
+  return hipGetLastError();
+}
+
 hipError_t launch_plain(const Workspace& ws, const float4* boxes, uint32_t n, GridParams gp,
                         uint32_t n_cells, uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev) {
   const uint32_t nb_cells = (n_cells + 255) / 256;

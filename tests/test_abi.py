@@ -36,13 +36,25 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_cpp_api_library_exports_reference_symbols():
-    """libpbig.so exports the reference's C++ API (src/pbig/pbig.h:9-11) with
-    the same mangled names, plus report's out-of-line members."""
-    path = os.path.join(ROOT, "pbig_amd", "libpbig.so")
-    syms = dynsyms(path)
-    for name in ("_Z9pbig_initjjjjj", "_Z10pbig_checkP8RectInfoP6report", "_Z13pbig_finalizev",
-                 "_ZN6reportC1Ev", "_ZN6report5printEv", "_ZN6report5foundEPK8RectInfoS2_"):
-        assert name in syms, name
+    """libpbig.so (float 2-D records) and libpbig_u32.so (the reference's
+    unsigned 3-D records) export the reference's C++ API (src/pbig/pbig.h:9-11)
+    with the same mangled names, plus report's out-of-line members."""
+    for lib in ("libpbig.so", "libpbig_u32.so"):
+        syms = dynsyms(os.path.join(ROOT, "pbig_amd", lib))
+        for name in ("_Z9pbig_initjjjjj", "_Z10pbig_checkP8RectInfoP6report", "_Z13pbig_finalizev",
+                     "_ZN6reportC1Ev", "_ZN6report5printEv", "_ZN6report5foundEPK8RectInfoS2_"):
+            assert name in syms, (lib, name)
+
+
+def test_reference_cell_size_u32_matches_oracle(orc):
+    """The unsigned cell-size rule against the oracle's (pinned by the
+    reference's trace: cs 128 at 10^7 boxes)."""
+    import pbig_amd
+
+    for sim, n, rng_, S, dims in (("UU", 1000, 10000, 100, 3), ("UN", 5000, 10000, 100, 3),
+                                  ("UU", 20000, 4000, 30, 2), ("NU", 300, 100000, 500, 3)):
+        b = orc.ref_generate(n, rng_, S, sim, dims)
+        assert pbig_amd.reference_cell_size_u32(b, dims, rng_) == orc.decide_cell_size_u32(b, dims, rng_)
 
 
 def test_reference_cell_size_matches_oracle(orc):

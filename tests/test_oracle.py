@@ -114,6 +114,18 @@ def test_reference_3d_generator_and_cell_rule_small(orc):
     assert orc.decide_cell_size_u32(big, 3, 10000) == 128
 
 
+def test_brute_u32_equals_grid_u32(orc):
+    """The unsigned brute force reports what the KAT-pinned grid restatement
+    counts, in 3-D and 2-D, at several cell sizes."""
+    for dims in (3, 2):
+        for sim in ("UU", "NU", "UN"):
+            b = orc.ref_generate(3000, 2000, 120, sim, dims)
+            bf = orc.brute_u32(b, dims)
+            assert len(np.unique(bf.view(np.uint64))) == len(bf)
+            for cs in (32, 64, 256, 2048):
+                assert orc.grid_u32(b, dims, 2000, cs)["pairs"] == len(bf), (dims, sim, cs)
+
+
 def test_reference_logged_answer_test1(orc):
     """benchmarks/test1/filter_file:7 -- 10^7 boxes, UU, S=100, range 1e4,
     3-D unsigned, srand(100): 49,514,252 pairs."""
