@@ -131,6 +131,53 @@ def clustered_leg(torch, eng_cls, reps=5):
     return out
 
 
+def compat_leg(torch, reps=3):
+    """The reference's own benchmark inputs (its shipped build: unsigned 3-D
+    boxes, src/pbig/main.cc:204-335 generator, srand(100), 10^7 boxes, range
+    10^4, S = 100): test1 (uniform centres) and test3 (normal centres), with
+    the reference's cell-size rule.  Timed as the reference times pbig_check
+    (copy in, check, pairs handed to the report), and with boxes resident and
+    pairs only counted.  The reference's published times are on a different,
+    much older GPU (context only)."""
+    import numpy as np
+
+    import oracle
+    import pbig_amd
+
+    oracle.build()
+    out = {}
+    published = {"test1_UU": (533211.0, 49514252, "benchmarks/test1/test1.set.csv:8"),
+                 "test3_UN": (1742764.0, 1121815231, "benchmarks/test3/test3.set.csv:8")}
+    eng = pbig_amd.Engine(0)
+    for name, sim in (("test1_UU", "UU"), ("test3_UN", "UN")):
+        b = oracle.ref_generate(10**7, 10000, 100, sim, 3)
+        d = torch.from_numpy(b.view(np.int32)).to("cuda:0")
+        k = eng.check_u32(d, 3, 10000, 0)
+        assert k == published[name][1], (name, k)
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            eng.check_u32(d, 3, 10000, 0)
+            t.append(time.perf_counter() - t0)
+        resident_ms = statistics.median(t) * 1e3
+        # end to end: host records in, every pair back on the host
+        t = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            pairs = eng.check_host_u32(b, 3, 10000, 0, capacity=k)
+            t.append(time.perf_counter() - t0)
+            assert pairs.shape[0] == k
+            del pairs
+        e2e_ms = statistics.median(t) * 1e3
+        out[name] = {"pairs": k, "resident_count_ms": resident_ms, "host_records_to_host_pairs_ms": e2e_ms,
+                     "reference_published_us": published[name][0], "reference_source": published[name][2]}
+        del d
+    eng.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def batched_leg(torch, steps, warmup, baseline=True):
     """The 4,096-instance workload (n log-uniform in [64, 262144], L = 16
     sqrt(n), sides U[1,8], seed 4): one HIP-graph replay of the batch per
@@ -360,6 +407,7 @@ def main():
                     help="batch leg without the per-instance comparison loops (for profiling)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-clustered", action="store_true", help="skip the clustered 2M-box leg")
+    ap.add_argument("--no-compat", action="store_true", help="skip the reference's own 10^7-box unsigned inputs")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
@@ -496,6 +544,8 @@ def main():
         out["batched"] = batched_leg(torch, args.batched_steps, 2, baseline=not args.no_batched_baseline)
     if not args.no_clustered:
         out["clustered"] = clustered_leg(torch, pbig_amd.Engine)
+    if not args.no_compat:
+        out["reference_compat"] = compat_leg(torch)
     if not args.no_serving:
         out["serving"] = serving_leg(torch, args.serve_requests, args.serve_rate, args.serve_budget_gb)
     if not args.no_cpu_baseline:
