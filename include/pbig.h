@@ -31,4 +31,14 @@ void pbig_init(unsigned int, unsigned int, unsigned int, unsigned int, unsigned 
 void pbig_check(struct RectInfo *, struct report *);
 void pbig_finalize();
 
+/* Extensions (no counterpart in the reference's pbig.h): the pairs of a
+ * check as the reference's compressed record stream (64-bit words in the
+ * layout of src/pbig/pbig.cu:849-888) with the cell arrays a reader needs,
+ * and a decoder that calls the report once per pair (decode.cc:26-138, with
+ * the partner offsets read where the encoder writes them). */
+void pbig_check_stream(struct RectInfo *input, std::vector<unsigned long long> *words,
+                       std::vector<unsigned int> *cell_start, std::vector<unsigned int> *cell_recs);
+void pbig_decode_stream(const std::vector<unsigned long long> &words, const std::vector<unsigned int> &cell_start,
+                        const std::vector<unsigned int> &cell_recs, struct report *r);
+
 #endif

@@ -241,6 +241,31 @@ int pbig_hip_check_u32(pbig_hip_ctx* ctx, const uint32_t* d_boxes, uint32_t n, i
 int pbig_hip_check_host_u32(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, int dims, uint32_t range,
                             uint32_t cell_size, uint32_t* h_pairs, uint64_t pair_capacity, uint64_t* n_pairs);
 
+/* ------------------------------------------------------------------------
+ * Compressed pair stream in the reference's record layout (its default
+ * COMPRESS = 2 build: 64-bit words; blocks [0x0fffff11, gid] followed by
+ * records {u8 seg1, u8 seg2, u16 l_rid, u16 nlaps, u8 offset[nlaps]} padded to
+ * whole words; pbig.cu:849-888, 519-527, decode.cc:26-138), with the cell
+ * arrays a reader needs: cell_start (n_cells + 1 u32) and cell_recs (u32 box
+ * ids).  Pair (a, b) = (cell_recs[cell_start[gid] + 256 seg1 + l_rid],
+ * cell_recs[cell_start[gid] + 256 seg2 + offset]).  Host functions, no
+ * context.  The reference's decoder reads the offsets 4 bytes past where its
+ * encoder writes them (decode.cc:80 vs pbig.cu:851); this decoder reads them
+ * where they are written.
+ * ------------------------------------------------------------------------ */
+/* Encode n_pairs pairs (lo, hi) among n_boxes boxes.  Sizes are returned in
+ * *n_words, *n_cells (cell_start holds n_cells + 1 entries) and *n_recs;
+ * PBIG_E_OVERFLOW when a capacity is too small (call with zero capacities to
+ * size the buffers). */
+int pbig_hip_rice_encode(const uint32_t* pairs, uint64_t n_pairs, uint32_t n_boxes, uint64_t* words,
+                         uint64_t word_capacity, uint64_t* n_words, uint32_t* cell_start, uint64_t start_capacity,
+                         uint64_t* n_cells, uint32_t* cell_recs, uint64_t recs_capacity, uint64_t* n_recs);
+/* Decode a stream into (lo, hi) pairs; *n_pairs = the number of pairs
+ * (PBIG_E_OVERFLOW past pair_capacity; PBIG_E_ARG on a malformed stream). */
+int pbig_hip_rice_decode(const uint64_t* words, uint64_t n_words, const uint32_t* cell_start, uint64_t n_cells,
+                         const uint32_t* cell_recs, uint64_t n_recs, uint32_t* pairs, uint64_t pair_capacity,
+                         uint64_t* n_pairs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,8 @@ EXPORTS = (
     "pbig_hip_reference_cell_size_u32",
     "pbig_hip_check_u32",
     "pbig_hip_check_host_u32",
+    "pbig_hip_rice_encode",
+    "pbig_hip_rice_decode",
 )
 
 PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
@@ -154,6 +156,9 @@ def load_library(path: str | None = None):
         "pbig_hip_reference_cell_size_u32": (u32, [P, u32, i32, u32]),
         "pbig_hip_check_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64), P]),
         "pbig_hip_check_host_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64)]),
+        "pbig_hip_rice_encode": (i32, [P, u64, u32, P, u64, ctypes.POINTER(u64), P, u64, ctypes.POINTER(u64), P,
+                                       u64, ctypes.POINTER(u64)]),
+        "pbig_hip_rice_decode": (i32, [P, u64, P, u64, P, u64, P, u64, ctypes.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -174,6 +179,46 @@ def reference_cell_size_u32(boxes: np.ndarray, dims: int, range_: int) -> int:
     """The reference's decideCellSize for its unsigned build, via the C-ABI."""
     b = np.ascontiguousarray(boxes, dtype=np.uint32).reshape(-1, 2 * dims)
     return int(load_library().pbig_hip_reference_cell_size_u32(b.ctypes.data, b.shape[0], int(dims), int(range_)))
+
+
+def rice_encode(pairs: np.ndarray, n_boxes: int):
+    """(k, 2) uint32 pairs -> (words uint64, cell_start uint32, cell_recs
+    uint32): the reference's compressed record stream and its cell arrays."""
+    lib = load_library()
+    p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    nw, nc, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib.pbig_hip_rice_encode(p.ctypes.data if len(p) else None, len(p), int(n_boxes), None, 0, ctypes.byref(nw),
+                                  None, 0, ctypes.byref(nc), None, 0, ctypes.byref(nr))
+    if rc not in (PBIG_OK, PBIG_E_OVERFLOW):
+        raise PbigError(rc, "rice_encode: bad pairs")
+    words = np.zeros(max(nw.value, 1), dtype=np.uint64)
+    start = np.zeros(nc.value + 1, dtype=np.uint32)
+    recs = np.zeros(max(nr.value, 1), dtype=np.uint32)
+    rc = lib.pbig_hip_rice_encode(p.ctypes.data if len(p) else None, len(p), int(n_boxes), words.ctypes.data,
+                                  len(words), ctypes.byref(nw), start.ctypes.data, len(start), ctypes.byref(nc),
+                                  recs.ctypes.data, len(recs), ctypes.byref(nr))
+    if rc != PBIG_OK:
+        raise PbigError(rc, "rice_encode failed")
+    return words[: nw.value], start, recs[: nr.value]
+
+
+def rice_decode(words: np.ndarray, cell_start: np.ndarray, cell_recs: np.ndarray) -> np.ndarray:
+    """The stream and its cell arrays -> (k, 2) uint32 pairs (lo, hi)."""
+    lib = load_library()
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    st = np.ascontiguousarray(cell_start, dtype=np.uint32)
+    rc_ = np.ascontiguousarray(cell_recs, dtype=np.uint32)
+    n = ctypes.c_uint64()
+    rc = lib.pbig_hip_rice_decode(w.ctypes.data if len(w) else None, len(w), st.ctypes.data, max(len(st) - 1, 0),
+                                  rc_.ctypes.data, len(rc_), None, 0, ctypes.byref(n))
+    if rc not in (PBIG_OK, PBIG_E_OVERFLOW):
+        raise PbigError(rc, "rice_decode: malformed stream")
+    out = np.empty((max(n.value, 1), 2), dtype=np.uint32)
+    rc = lib.pbig_hip_rice_decode(w.ctypes.data if len(w) else None, len(w), st.ctypes.data, max(len(st) - 1, 0),
+                                  rc_.ctypes.data, len(rc_), out.ctypes.data, n.value, ctypes.byref(n))
+    if rc != PBIG_OK:
+        raise PbigError(rc, "rice_decode failed")
+    return out[: n.value]
 
 
 def _ptr(t) -> int:

@@ -36,6 +36,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["pbig_kernels.hip", "pbig_binned.hip", "pbig_u32.hip", "pbig_capi.hip"]
+HOST_SOURCES = ["pbig_rice.cpp"]  # host-only parts of libpbig_hip.so
 LIB_HIP = os.path.join(HERE, "libpbig_hip.so")
 LIB_API = os.path.join(HERE, "libpbig.so")
 LIB_API_U32 = os.path.join(HERE, "libpbig_u32.so")
@@ -51,6 +52,10 @@ def build(verbose: bool = True) -> None:
     for src in HIP_SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(CSRC, src.replace(".cpp", ".o"))
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB_HIP + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])

@@ -82,6 +82,42 @@ void pbig_check(RectInfo* input, report* r) {
   for (unsigned long long k = 0; k < np; ++k) (*r)(g_pairs[2 * k], g_pairs[2 * k + 1]);
 }
 
+void pbig_check_stream(RectInfo* input, std::vector<unsigned long long>* words, std::vector<unsigned int>* cell_start,
+                       std::vector<unsigned int>* cell_recs) {
+  // the pairs of the check, then the encoder (sizes first)
+  std::vector<unsigned int> flat;
+  {
+    report r;
+    pbig_check(input, &r);
+    const unsigned long long np = r.count();
+    flat.assign(g_pairs.begin(), g_pairs.begin() + 2 * np);
+  }
+  uint64_t nw = 0, nc = 0, nr = 0;
+  const uint64_t np = flat.size() / 2;
+  int rc = pbig_hip_rice_encode(flat.data(), np, g_n, nullptr, 0, &nw, nullptr, 0, &nc, nullptr, 0, &nr);
+  if (rc != PBIG_OK && rc != PBIG_E_OVERFLOW) die("pbig_check_stream", rc);
+  words->assign(nw, 0ull);
+  cell_start->assign(nc + 1, 0u);
+  cell_recs->assign(nr, 0u);
+  rc = pbig_hip_rice_encode(flat.data(), np, g_n, (uint64_t*)words->data(), nw, &nw, cell_start->data(), nc + 1, &nc,
+                            cell_recs->data(), nr, &nr);
+  if (rc) die("pbig_check_stream", rc);
+}
+
+void pbig_decode_stream(const std::vector<unsigned long long>& words, const std::vector<unsigned int>& cell_start,
+                        const std::vector<unsigned int>& cell_recs, report* r) {
+  uint64_t np = 0;
+  const uint64_t n_cells = cell_start.empty() ? 0 : cell_start.size() - 1;
+  int rc = pbig_hip_rice_decode((const uint64_t*)words.data(), words.size(), cell_start.data(), n_cells,
+                                cell_recs.data(), cell_recs.size(), nullptr, 0, &np);
+  if (rc != PBIG_OK && rc != PBIG_E_OVERFLOW) die("pbig_decode_stream", rc);
+  std::vector<unsigned int> pairs(2 * np + 2);
+  rc = pbig_hip_rice_decode((const uint64_t*)words.data(), words.size(), cell_start.data(), n_cells,
+                            cell_recs.data(), cell_recs.size(), pairs.data(), np, &np);
+  if (rc) die("pbig_decode_stream", rc);
+  for (uint64_t k = 0; k < np; ++k) (*r)(pairs[2 * k], pairs[2 * k + 1]);
+}
+
 void pbig_finalize() {
   if (g_ctx) pbig_hip_destroy(g_ctx);
   g_ctx = nullptr;
