@@ -9,7 +9,7 @@
 
 namespace pbig {
 
-constexpr uint32_t kMaxBins = 8192;          // coarse bins per instance (LDS histogram)
+constexpr uint32_t kMaxBins = 6400;          // coarse bins per instance (LDS cursors of the scatter)
 constexpr uint32_t kMaxFine = 4096;          // fine memberships of one strip held in LDS
 constexpr uint32_t kMaxCoarse = 4096;        // coarse memberships of one bin (registers + LDS ids)
 constexpr uint32_t kMaxCellsPerBin = 4096;   // k*k fine cells per coarse bin (k <= 64)
@@ -17,7 +17,7 @@ constexpr uint32_t kMaxK = 64;
 constexpr uint32_t kBinStage = 128;          // staged pairs per wave
 constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent test workgroup
 constexpr uint32_t kBatch = 16;              // boxes per thread loaded before use (hist)
-constexpr uint32_t kScatterChunk = 8192;     // boxes per histogram/scatter workgroup (at most)
+constexpr uint32_t kScatterChunk = 4096;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
 // Overfull list entries: the bin, with this bit when a single fine cell
 // exceeded kMaxFine (re-binning cannot help; the instance needs the general

@@ -246,38 +246,27 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
 }
 
 // The scatter holds its whole chunk in registers (kScatterPer boxes per
-// thread) and emits the chunk's memberships bin range by bin range.  In each
-// round the memberships of a contiguous range of bins go through an LDS
-// staging buffer in destination order, so a wave's stores cover whole runs;
-// and since every workgroup walks the bins in the same order, the runs that
-// share cache lines (adjacent workgroups on one XCD) are written within the
-// same short window and merge in that XCD's L2.
+// thread) and ranks every membership among the chunk's memberships in bin
+// order with LDS atomics on per-bin cursors; the memberships are staged in
+// LDS at their rank, then written out in rank order, so a wave's stores
+// cover whole runs of each bin.  Every workgroup walks the bins in the same
+// order, and the runs that share cache lines (adjacent chunks on one XCD)
+// are written within the same short window and merge in that XCD's L2.  A
+// chunk with more memberships than the stage writes the rest directly.
 constexpr uint32_t kScatterThreads = 1024;
-constexpr uint32_t kScatterPer = 8;
-constexpr uint32_t kMaxRounds = 512;
-constexpr uint32_t kStageEntries = 2048;
+constexpr uint32_t kScatterPer = kScatterChunk / kScatterThreads;
+constexpr uint32_t kStageEntries = kScatterChunk + kScatterChunk / 8;  // 1.125 memberships per box
 
 // Fixed part of the scatter's LDS; after it, sized per launch to the
 // batch's largest instance (nb_cap bins), come
 //   uint32_t cur[nb_cap]   next destination of each bin (global index)
-//   int32_t woff[nb_cap]   staging position of a destination = dst + woff - round base
-// so that two 1024-thread workgroups share a CU when instances have up to
-// ~2,900 bins.
+//   int32_t woff[nb_cap]   chunk-local rank of a destination = dst + woff
 struct ScatterLds {
   float4 box[kStageEntries];
   uint32_t id[kStageEntries];
   uint32_t dst[kStageEntries];
-  uint32_t round_hi[kMaxRounds];   // first bin after round r
-  uint32_t round_n[kMaxRounds];    // memberships of this chunk in round r
-  uint32_t round_base[kMaxRounds]; // memberships of this chunk before round r (bin order)
-  uint32_t wsum_round_end[kMaxRounds];  // first bin after round r
-  uint64_t wsum[16];
-  uint32_t n_rounds;
+  uint32_t wsum[16];
 };
-
-__device__ __forceinline__ uint32_t pack_range(const BinRange& r) {
-  return r.x0 | (r.x1 << 8) | (r.y0 << 16) | (r.y1 << 24);
-}
 
 // kIds: the boxes' ids come from ids_in (virtual instances), else they are
 // the index inside the instance.
@@ -311,13 +300,13 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     const uint64_t i = start + tid + k * kScatterThreads;
     q[k] = boxes[i < end ? i : end - 1];
   }
-  // 2./3. per bin: destination cursor, and the offset from a destination to
-  //    the membership's position among this chunk's memberships in bin order
+  // 2. per bin: destination cursor, and the offset from a destination to the
+  //    membership's rank among this chunk's memberships in bin order
   const uint64_t lc = blockIdx.x - I.chunk0;
   const uint32_t* myoff = off + I.off0 + lc * nb;
   const uint32_t* myhist = hist + I.hist0 + lc * (nb + 1);
   const uint32_t* mystart = bin_start + I.bin0;
-  uint32_t log_h;  // a bin's round is its chunk-local position >> log_h (uniform)
+  uint32_t chunk_total = 0;
   {
     const uint32_t per = (nb + kScatterThreads - 1) / kScatterThreads;
     const uint32_t b0 = tid * per;
@@ -333,7 +322,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     if (lane == 63) L.wsum[w] = v;
     __syncthreads();
     uint32_t ex = v - acc;
-    for (uint32_t k = 0; k < w; ++k) ex += (uint32_t)L.wsum[k];
+    for (uint32_t k = 0; k < kScatterThreads / 64; ++k) {
+      if (k < w) ex += L.wsum[k];
+      chunk_total += L.wsum[k];
+    }
     for (uint32_t k = 0; k < per; ++k) {
       const uint32_t b = b0 + k;
       if (b < nb) {
@@ -343,109 +335,41 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
         ex += myhist[b];
       }
     }
-    // Rounds: bin b goes to round wpos[b] >> log_h, where wpos is the
-    // chunk's exclusive prefix in bin order and 2^log_h <= kStageEntries / 2,
-    // so a round holds at most 2^log_h plus one bin's memberships (rounds
-    // that still overflow the stage are written directly).
-    if (tid == 0) L.n_rounds = 0;
-    __syncthreads();
-    uint32_t chunk_total = 0;
-    for (uint32_t k = 0; k < kScatterThreads / 64; ++k) chunk_total += (uint32_t)L.wsum[k];
-    log_h = 0;
-    while ((1u << (log_h + 1)) <= kStageEntries / 2) ++log_h;
-    while ((chunk_total >> log_h) >= kMaxRounds - 1) ++log_h;
-    // rounds 0 .. (wpos of the last bin) >> log_h; each is started by the first
-    // bin whose round is at least its index, so every entry below is set
-    const uint32_t wp_last = (uint32_t)((int32_t)cur[nb - 1] + woff[nb - 1]);
-    const uint32_t n_rounds = chunk_total ? (wp_last >> log_h) + 1 : 0;
-    for (uint32_t b = tid; b < nb; b += kScatterThreads) {
-      const uint32_t wp = (uint32_t)((int32_t)cur[b] + woff[b]);
-      const uint32_t rb = wp >> log_h;
-      const uint32_t rprev = b ? ((uint32_t)((int32_t)cur[b - 1] + woff[b - 1]) >> log_h) : 0xFFFFFFFFu;
-      // bin b starts rounds rprev+1 .. rb (empty rounds share its start)
-      for (uint32_t rr = rprev + 1; rr <= rb && rr < n_rounds; ++rr) {
-        L.round_hi[rr] = b;  // start of round rr
-        L.round_base[rr] = wp;
-      }
-    }
-    __syncthreads();
-    for (uint32_t rr = tid; rr < n_rounds; rr += kScatterThreads) {
-      const uint32_t nxt_start = rr + 1 < n_rounds ? L.round_hi[rr + 1] : nb;
-      const uint32_t nxt_base = rr + 1 < n_rounds ? L.round_base[rr + 1] : chunk_total;
-      L.round_n[rr] = nxt_base - L.round_base[rr];
-      L.wsum_round_end[rr] = nxt_start;
-    }
-    if (tid == 0) L.n_rounds = n_rounds;
-    __syncthreads();
   }
-  // 4. bin ranges of this thread's boxes, packed (nbx <= 90 fits 8 bits)
-  uint32_t rng[kScatterPer];
+  __syncthreads();
+  // 3. rank every membership; stage it at its rank (or write it directly
+  //    past the stage)
 #pragma unroll
   for (uint32_t k = 0; k < kScatterPer; ++k) {
-    const uint64_t i = start + tid + k * kScatterThreads;
+    const uint64_t gi = start + tid + k * kScatterThreads;
     const float4 b = q[k];
-    const bool ok = i < end && (b.x <= b.y) && (b.z <= b.w);
-    rng[k] = ok ? pack_range(bin_range(b, I)) : 0xFFFFFFFFu;
-  }
-  // 4b. the first and last round of each box: its bins are the bin-order
-  //     range [y0 * nbx + x0, y1 * nbx + x1] and rounds are ranges of bins,
-  //     read before the rounds move the cursors
-  uint32_t rr[kScatterPer];
-#pragma unroll
-  for (uint32_t k = 0; k < kScatterPer; ++k) {
-    const uint32_t pr = rng[k];
-    rr[k] = 0x0000FFFFu;  // no round
-    if (pr != 0xFFFFFFFFu) {
-      const uint32_t b0 = ((pr >> 16) & 0xFF) * I.nbx + (pr & 0xFF), b1 = (pr >> 24) * I.nbx + ((pr >> 8) & 0xFF);
-      const uint32_t r0 = (uint32_t)((int32_t)cur[b0] + woff[b0]) >> log_h;
-      const uint32_t r1 = (uint32_t)((int32_t)cur[b1] + woff[b1]) >> log_h;
-      rr[k] = r0 | (r1 << 16);
-    }
-  }
-  __syncthreads();  // every wave has read the cursors before the rounds advance them
-  // 5. rounds
-  const uint32_t n_rounds = L.n_rounds;
-  for (uint32_t r = 0; r < n_rounds; ++r) {
-    const uint32_t b_lo = L.round_hi[r], b_hi = L.wsum_round_end[r], rn = L.round_n[r];
-    if (rn == 0) continue;  // uniform: every thread reads the same table
-    const bool direct = rn > kStageEntries;  // one bin alone overflows the stage
-    // Fixed before the round starts: cursors of the round's bins move below.
-    const int32_t base = (int32_t)L.round_base[r];
-#pragma unroll
-    for (uint32_t k = 0; k < kScatterPer; ++k) {
-      if (r < (rr[k] & 0xFFFFu) || r > (rr[k] >> 16)) continue;
-      const uint32_t pr = rng[k];
-      const uint32_t x0 = pr & 0xFF, x1 = (pr >> 8) & 0xFF, y0 = (pr >> 16) & 0xFF, y1 = pr >> 24;
-      uint32_t i = (uint32_t)(start - I.box0) + tid + k * kScatterThreads;  // id inside the instance
-      if (kIds) i = ids_in[start + tid + k * kScatterThreads];
-      for (uint32_t cy = y0; cy <= y1; ++cy)
-        for (uint32_t cx = x0; cx <= x1; ++cx) {
-          const uint32_t b = cy * I.nbx + cx;
-          if (b < b_lo || b >= b_hi) continue;
-          const uint32_t d = atomicAdd(&cur[b], 1u);
-          const uint32_t at = (uint32_t)((int32_t)d + woff[b] - base);
-          if (direct || at >= kStageEntries) {
-            if (d < cap) {
-              bbox[d] = q[k];
-              bid[d] = i;
-            }
-          } else {
-            L.box[at] = q[k];
-            L.id[at] = i;
-            L.dst[at] = d;
-          }
-        }
-    }
-    __syncthreads();
-    if (!direct) {
-      for (uint32_t k = tid; k < rn; k += kScatterThreads) {
-        const uint32_t d = L.dst[k];
-        if (d < cap) {
-          bbox[d] = L.box[k];
-          bid[d] = L.id[k];
+    if (!(gi < end && (b.x <= b.y) && (b.z <= b.w))) continue;
+    const BinRange r = bin_range(b, I);
+    uint32_t i = (uint32_t)(gi - I.box0);  // id inside the instance
+    if (kIds) i = ids_in[gi];
+    for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
+      for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
+        const uint32_t bb = cy * I.nbx + cx;
+        const uint32_t d = atomicAdd(&cur[bb], 1u);
+        const uint32_t at = (uint32_t)((int32_t)d + woff[bb]);
+        if (at < kStageEntries) {
+          L.box[at] = b;
+          L.id[at] = i;
+          L.dst[at] = d;
+        } else if (d < cap) {
+          bbox[d] = b;
+          bid[d] = i;
         }
       }
-      __syncthreads();
+  }
+  __syncthreads();
+  // 4. the staged memberships in rank order
+  const uint32_t staged = chunk_total < kStageEntries ? chunk_total : kStageEntries;
+  for (uint32_t k = tid; k < staged; k += kScatterThreads) {
+    const uint32_t d = L.dst[k];
+    if (d < cap) {
+      bbox[d] = L.box[k];
+      bid[d] = L.id[k];
     }
   }
 }
@@ -527,23 +451,32 @@ __device__ __forceinline__ FineRange fine_range(const float4& q, const FineGrid&
   return r;
 }
 
-// Scatter the fine memberships of one coarse membership that fall in cells
-// [c_lo, c_hi) to their positions in the strip (cnt[] is the cursor, set to
-// start[] by the scan).
-__device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint32_t j, const FineGrid& p,
-                                              uint32_t gx, uint32_t gy, uint32_t c_lo, uint32_t c_hi,
-                                              uint32_t base) {
-  const FineRange r = fine_range(q, p, gx, gy);
-  const uint32_t row_lo = c_lo / p.kx, row_hi = (c_hi - 1) / p.kx;
-  const uint32_t y0 = r.y0 > row_lo ? r.y0 : row_lo, y1 = r.y1 < row_hi ? r.y1 : row_hi;
+// A coarse membership's fine cells inside its bin, packed once per bin:
+// x0 | x1 << 6 | y0 << 12 | y1 << 18 (bin-local, < 64), bit 24 = the box
+// starts in this bin's columns (its first column is x0), bit 25 = it starts
+// in this bin's rows.
+__device__ __forceinline__ uint32_t pack_fine(const FineRange& r, uint32_t gx, uint32_t gy) {
+  return r.x0 | (r.x1 << 6) | (r.y0 << 12) | (r.y1 << 18) | ((r.fx0 >= gx ? 1u : 0u) << 24) |
+         ((r.fy0 >= gy ? 1u : 0u) << 25);
+}
+
+// Scatter the fine memberships of one coarse membership (packed range pr)
+// that fall in cells [c_lo, c_hi) to their positions in the strip (cnt[] is
+// the cursor, set to start[] by the scan).
+__device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint32_t j, uint32_t pr, uint32_t kx,
+                                              uint32_t c_lo, uint32_t c_hi, uint32_t base) {
+  const uint32_t x0 = pr & 63u, x1 = (pr >> 6) & 63u, ry0 = (pr >> 12) & 63u, ry1 = (pr >> 18) & 63u;
+  const uint32_t sx = (pr >> 24) & 1u, sy = (pr >> 25) & 1u;
+  const uint32_t row_lo = c_lo / kx, row_hi = (c_hi - 1) / kx;
+  const uint32_t y0 = ry0 > row_lo ? ry0 : row_lo, y1 = ry1 < row_hi ? ry1 : row_hi;
   for (uint32_t cy = y0; cy <= y1; ++cy) {
-    const uint32_t fy = (gy + cy == r.fy0) ? kFlagY : 0u;
-    for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
-      const uint32_t c = cy * p.kx + cx;
+    const uint32_t fy = (sy && cy == ry0) ? kFlagY : 0u;
+    for (uint32_t cx = x0; cx <= x1; ++cx) {
+      const uint32_t c = cy * kx + cx;
       if (c < c_lo || c >= c_hi) continue;
       const uint32_t pos = atomicAdd(&L.cnt[c], 1u) - base;
       L.box[pos] = q;
-      L.word[pos] = j | (c << kWordCellShift) | fy | ((gx + cx == r.fx0) ? kFlagX : 0u);
+      L.word[pos] = j | (c << kWordCellShift) | fy | ((sx && cx == x0) ? kFlagX : 0u);
     }
   }
 }
@@ -621,12 +554,15 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
+      uint32_t fr[kRegs];  // packed fine ranges, reused by every strip's scatter
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
+        fr[k] = 0u;
         if (j < m) {
           L.id[j] = ri[k];
           const FineRange r = fine_range(rb[k], p, gx, gy);
+          fr[k] = pack_fine(r, gx, gy);
           for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
             for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * Kx + cx], 1u);
         }
@@ -698,7 +634,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
 #pragma unroll
         for (uint32_t k = 0; k < kRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads;
-          if (j < m) scatter_strip(L, rb[k], j, p, gx, gy, c_lo, c_hi, base);
+          if (j < m) scatter_strip(L, rb[k], j, fr[k], Kx, c_lo, c_hi, base);
         }
         __syncthreads();
         const bool last_strip = c_hi == kk;
