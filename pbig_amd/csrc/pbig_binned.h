@@ -16,7 +16,7 @@ constexpr uint32_t kMaxCellsPerBin = 4096;   // k*k fine cells per coarse bin (k
 constexpr uint32_t kMaxK = 64;
 constexpr uint32_t kBinStage = 128;          // staged pairs per wave
 constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent test workgroup
-constexpr uint32_t kBatch = 16;              // boxes per thread loaded before use (hist)
+constexpr uint32_t kBatch = 4;               // boxes per thread loaded before use (hist): a 4,096-box chunk in one pass
 constexpr uint32_t kScatterChunk = 4096;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
 // Overfull list entries: the bin, with this bit when a single fine cell

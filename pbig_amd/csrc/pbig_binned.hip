@@ -287,12 +287,19 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   uint32_t* const cur = reinterpret_cast<uint32_t*>(lds_raw + sizeof(ScatterLds));
   int32_t* const woff = reinterpret_cast<int32_t*>(cur + nb_cap);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (blockIdx.x >= n_chunks) return;
+  // Workgroups go to the 8 XCDs round-robin.  Block i takes chunk
+  // (i mod 8) * per + i / 8, so each XCD walks one contiguous range of
+  // chunks, and the runs that chunks c and c + 1 write side by side in every
+  // bin's bucket come from the same XCD at about the same time (the grid is
+  // rounded up to a multiple of 8; blocks past the last chunk exit).
+  const uint32_t xper = (n_chunks + 7u) / 8u;
+  const uint32_t wc = (blockIdx.x & 7u) * xper + (blockIdx.x >> 3);
+  if (wc >= n_chunks) return;
   if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
-  const InstDesc I = inst[chunk_inst[blockIdx.x]];
+  const InstDesc I = inst[chunk_inst[wc]];
   const uint32_t nb = I.nb;
   uint64_t start, end;
-  chunk_range(I, blockIdx.x, &start, &end);
+  chunk_range(I, wc, &start, &end);
   // 1. this thread's boxes, all loads in flight at once
   float4 q[kScatterPer];
 #pragma unroll
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   }
   // 2. per bin: destination cursor, and the offset from a destination to the
   //    membership's rank among this chunk's memberships in bin order
-  const uint64_t lc = blockIdx.x - I.chunk0;
+  const uint64_t lc = wc - I.chunk0;
   const uint32_t* myoff = off + I.off0 + lc * nb;
   const uint32_t* myhist = hist + I.hist0 + lc * (nb + 1);
   const uint32_t* mystart = bin_start + I.bin0;
@@ -407,7 +414,7 @@ struct TestLds {
   uint32_t id[kMaxCoarse];         // box id of each coarse membership
   uint32_t cnt[kMaxCellsPerBin];   // fine-cell counts of the bin, then the scatter cursors
   uint32_t start[kMaxCellsPerBin + 1];  // exclusive scan of cnt (start[kk] = total)
-  uint2 stage[kTestWaves][kBinStage];   // per-wave pair staging
+  uint2 stage[kTestWaves][kBinStage];   // per-wave staging of hits (coarse membership indices)
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
   uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, kx | ky << 16, nf) of each bin
   float bininv[kMaxBinsPerGroup];  // inv_f of each bin's instance
@@ -416,6 +423,7 @@ struct TestLds {
   uint32_t strip_hi;
 };
 
+// Appends a wave's staged hits to the workgroup's region as (smaller id, larger id).
 __device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn, uint2* __restrict__ region,
                                             uint32_t cap_wg) {
   const uint32_t lane = threadIdx.x & 63;
@@ -423,7 +431,11 @@ __device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn
   if (lane == 0) g = atomicAdd(&L.out_n, sn);
   g = __builtin_amdgcn_readfirstlane(g);
   for (uint32_t k = lane; k < sn; k += 64)
-    if (g + k < cap_wg) region[g + k] = L.stage[w][k];
+    if (g + k < cap_wg) {
+      const uint2 v = L.stage[w][k];
+      const uint32_t ia = L.id[v.x], ib = L.id[v.y];
+      region[g + k] = make_uint2(ia < ib ? ia : ib, ia < ib ? ib : ia);
+    }
   sn = 0;
 }
 
@@ -586,7 +598,12 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
         if (lane == 63) L.wsum[w] = inc;
         __syncthreads();
         uint32_t ex = inc - acc;
-        for (uint32_t k = 0; k < w; ++k) ex += L.wsum[k];
+        {  // sum of the lower waves' totals: one read per lane, then a reduction
+          uint32_t ws = lane < w ? L.wsum[lane] : 0u;
+#pragma unroll
+          for (int d = 1; d < 16; d <<= 1) ws += __shfl_xor(ws, d, 64);
+          ex += __builtin_amdgcn_readfirstlane(ws);
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t c = tid * 4 + k;
@@ -693,10 +710,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
               if (mask) {
                 const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                if (hit) {
-                  const uint32_t ia = L.id[ja], ib = L.id[ow[u] & kWordIdxMask];
-                  L.stage[w][pos] = make_uint2(ia < ib ? ia : ib, ia < ib ? ib : ia);
-                }
+                if (hit) L.stage[w][pos] = make_uint2(ja, ow[u] & kWordIdxMask);
                 sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
                 if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
               }
@@ -851,11 +865,11 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;
   if (ids_in)
-    k_bin_scatter<true><<<nc, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<true><<<(nc + 7u) / 8u * 8u, kScatterThreads, scatter_lds, s>>>(
         boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   else
-    k_bin_scatter<false><<<nc, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<false><<<(nc + 7u) / 8u * 8u, kScatterThreads, scatter_lds, s>>>(
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   PBIG_MARK(4)
