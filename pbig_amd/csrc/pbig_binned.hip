@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
 // last strip's scatter and land while that strip is tested.
 
 constexpr uint32_t kTestThreads = 1024;
-constexpr uint32_t kUnroll = 2;  // partner loads issued together in the test loop
+constexpr uint32_t kUnroll = 1;  // partner loads issued together in the test loop
 constexpr uint32_t kTestWaves = kTestThreads / 64;
 constexpr uint32_t kRegs = kMaxCoarse / kTestThreads;  // coarse memberships per thread in registers
 
@@ -408,12 +408,23 @@ constexpr uint32_t kWordCellMask = (1u << 12) - 1u;
 static_assert(kMaxCoarse <= (1u << kWordCellShift), "membership index field");
 static_assert(kMaxCellsPerBin <= (1u << 12), "cell field");
 
+// Cells of one bin are laid out by size class (min(count, 63), largest
+// first) when the whole bin fits one strip, so that the 64 positions of a
+// wave belong to cells of about the same size and the wave's partner loop
+// (its longest lane) wastes few slots.
+constexpr uint32_t kSizeClasses = 64;
+
 struct TestLds {
   float4 box[kMaxFine];            // strip memberships in fine-cell order
   uint32_t word[kMaxFine];         // position words
   uint32_t id[kMaxCoarse];         // box id of each coarse membership
   uint32_t cnt[kMaxCellsPerBin];   // fine-cell counts of the bin, then the scatter cursors
-  uint32_t start[kMaxCellsPerBin + 1];  // exclusive scan of cnt (start[kk] = total)
+  uint32_t start[kMaxCellsPerBin + 1];  // first position of each cell's memberships
+  uint16_t ccount[kMaxCellsPerBin];     // memberships of each cell
+  uint32_t csum[kSizeClasses];     // memberships of the cells of each size class
+  uint32_t cstart[kSizeClasses];   // first position of each size class, then its cursor
+  uint32_t nfine;                  // fine memberships of the bin
+  uint32_t npair;                  // positions in cells of two or more memberships (sorted layout)
   uint2 stage[kTestWaves][kBinStage];   // per-wave staging of hits (coarse membership indices)
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
   uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, kx | ky << 16, nf) of each bin
@@ -422,6 +433,7 @@ struct TestLds {
   uint32_t out_n;                  // pairs appended to this workgroup's region
   uint32_t strip_hi;
 };
+static_assert(sizeof(TestLds) <= 160u * 1024u, "one test workgroup per CU");
 
 // Appends a wave's staged hits to the workgroup's region as (smaller id, larger id).
 __device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn, uint2* __restrict__ region,
@@ -517,7 +529,11 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   // cnt[] is zero at the start of every bin: cleared here, then after each
   // bin's last scatter (and on the overflow exit)
   for (uint32_t c = tid; c < kMaxCellsPerBin; c += kTestThreads) L.cnt[c] = 0u;
-  if (tid == 0) L.out_n = 0u;
+  if (tid < kSizeClasses) L.csum[tid] = 0u;  // csum and nfine are also cleared after each use
+  if (tid == 0) {
+    L.out_n = 0u;
+    L.nfine = 0u;
+  }
 
   // Registers: unconditional loads with the index clamped into the bucket.
   const uint64_t last = cap_bucket - 1;
@@ -567,6 +583,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
       uint32_t fr[kRegs];  // packed fine ranges, reused by every strip's scatter
+      uint32_t mine = 0;   // this thread's fine memberships
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
@@ -575,18 +592,66 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
           L.id[j] = ri[k];
           const FineRange r = fine_range(rb[k], p, gx, gy);
           fr[k] = pack_fine(r, gx, gy);
+          mine += (r.x1 - r.x0 + 1u) * (r.y1 - r.y0 + 1u);
           for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
             for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * Kx + cx], 1u);
         }
       }
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) mine += __shfl_xor(mine, d, 64);
+      if (lane == 0) atomicAdd(&L.nfine, mine);
       __syncthreads();
-      // B. exclusive scan of the counts (kk <= 4096: 4 per thread)
-      {
+      const uint32_t nfine = L.nfine;
+      const bool sorted = nfine <= kMaxFine;
+      // B. cell positions (kk <= 4096: 4 cells per thread)
+      uint32_t npair = nfine;
+      if (sorted) {
+        // by size class: class totals, their exclusive scan (largest class
+        // first), then each cell takes its run from its class's cursor
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t c = tid * 4 + k;
+          v[k] = c < kk ? L.cnt[c] : 0u;
+          if (c < kk) L.ccount[c] = (uint16_t)v[k];
+          if (v[k]) atomicAdd(&L.csum[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
+        }
+        __syncthreads();
+        if (w == 0) {
+          const uint32_t cls = kSizeClasses - 1u - lane;  // lane 0: the largest class
+          const uint32_t own = L.csum[cls];
+          uint32_t inc = own;
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d) inc += o;
+          }
+          L.cstart[cls] = inc - own;
+          L.csum[cls] = 0u;  // cleared for the next bin
+          if (cls == 1u) L.npair = inc - own;
+          if (lane == 0) L.nfine = 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t c = tid * 4 + k;
+          if (v[k]) {
+            const uint32_t st = atomicAdd(&L.cstart[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
+            L.start[c] = st;
+            L.cnt[c] = st;  // scatter cursor
+          }
+        }
+        if (tid == 0) L.start[kk] = nfine;
+        __syncthreads();
+        npair = L.npair;
+      } else {
+        // row-major: exclusive scan of the counts
         uint32_t v[4], acc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t c = tid * 4 + k;
           v[k] = c < kk ? L.cnt[c] : 0u;
+          if (c < kk) L.ccount[c] = (uint16_t)v[k];
           acc += v[k];
         }
         uint32_t inc = acc;
@@ -614,14 +679,15 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
           ex += v[k];
         }
         if (tid == kTestThreads - 1) L.start[kk] = ex;
+        if (tid == 0) L.nfine = 0u;  // every thread read it before the barrier above
+        __syncthreads();
       }
-      __syncthreads();
       // C/D per strip
       // start[] is read-only from here on
       uint32_t c_lo = 0;
       while (c_lo < kk) {
         uint32_t c_hi = kk;  // usually the rest of the bin fits in one strip
-        if (L.start[kk] - L.start[c_lo] > kMaxFine) {
+        if (!sorted && L.start[kk] - L.start[c_lo] > kMaxFine) {
           if (tid == 0) {
             // largest c_hi with start[c_hi] - start[c_lo] <= kMaxFine
             const uint32_t lim = L.start[c_lo] + kMaxFine;
@@ -646,8 +712,8 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
           __syncthreads();
           break;
         }
-        const uint32_t base = L.start[c_lo];
-        const uint32_t len = L.start[c_hi] - base;
+        const uint32_t base = sorted ? 0u : L.start[c_lo];
+        const uint32_t len = sorted ? npair : L.start[c_hi] - base;
 #pragma unroll
         for (uint32_t k = 0; k < kRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads;
@@ -676,7 +742,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
             aw = L.word[pp];
             const uint32_t c = (aw >> kWordCellShift) & kWordCellMask;
             cs = L.start[c] - base;
-            cn = L.start[c + 1] - L.start[c];
+            cn = L.ccount[c];
             const uint32_t r = pp - cs;
             // (n-1)/2 partners, plus one for the first half of an even cell
             h = (cn - 1 + (((cn & 1u) == 0u) & (r < (cn >> 1)))) >> 1;
