@@ -98,7 +98,9 @@ def read_traffic(path_name, cs, bin_target):
 def clustered_leg(torch, eng_cls, reps=5):
     """The clustered 2,097,152-box instance (seed 3), boxes resident: one
     synchronous call per step (its heavy coarse bins are re-binned, which
-    needs the host between levels), binned path vs general path."""
+    needs the host between levels), binned path vs general path, and the
+    automatic choice (which sends an instance whose heavy bins hold most of
+    its memberships to the general path)."""
     import pbig_amd
     from pbig_amd import autotune, workloads
 
@@ -111,7 +113,8 @@ def clustered_leg(torch, eng_cls, reps=5):
     out = {"workload": "2,097,152 float32 rects, 90% in 64 Gaussian clusters (sigma 327.68) + 10% uniform "
                        "in [0,65536)^2, log-normal sides, seed 3, boxes resident in HBM",
            "cell_size": cell}
-    for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL)):
+    for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL),
+                       ("auto", pbig_amd.PATH_AUTO)):
         e = eng_cls(0)
         e.configure(path)
         k = e.check_device(d, L, cell, pairs)
@@ -467,7 +470,8 @@ def main():
             eng.check_device_async(d_boxes, L, cs, d_pairs, result, stream=stream)
     torch.cuda.synchronize()
 
-    eng.timing_start(args.steps)
+    # Timed region: K steps back to back on one stream, bracketed by two
+    # events and a synchronisation on each side.
     start_ev = torch.cuda.Event(enable_timing=True)
     end_ev = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -480,6 +484,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = start_ev.elapsed_time(end_ev)
+    res = result.cpu().tolist()
+    assert res[1] == 0 and res[0] == n_pairs, res
+    # Kernel durations: the same K steps again with a HIP event recorded
+    # around every kernel on the launch stream (the events add gaps between
+    # kernels, so the step time above is taken without them).
+    eng.timing_start(args.steps)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for _ in range(args.steps):
+            eng.check_device_async(d_boxes, L, cs, d_pairs, result, stream=stream)
+    torch.cuda.synchronize()
     per_kernel, calls, path_name = eng.timing_read()
     res = result.cpu().tolist()
     assert res[1] == 0 and res[0] == n_pairs, res

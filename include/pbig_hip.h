@@ -105,10 +105,12 @@ int pbig_hip_check_device(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, d
  * `stream` and returns without waiting.  cell_size must be > 0.  After the
  * stream reaches this point, d_result[0] holds the pair count and
  * d_result[1] an error word (0 = ok; bit 0 invalid box, bit 1 workspace too
- * small, bit 2 cell too dense, bit 3 a coarse bin did not fit in LDS).
- * d_result is 2 x uint64 in device memory.  The path (and the workspace
- * size) follow the context's last synchronous call; when bit 1 or bit 3 is
- * set, run the synchronous entry point once and call again. */
+ * small, bit 2 cell too dense, bit 3 a coarse bin did not fit in LDS, bit 4
+ * the instance is mostly dense clusters, which the automatic path choice
+ * sends to the general path).  d_result is 2 x uint64 in device memory.  The
+ * path (and the workspace size) follow the context's last synchronous call;
+ * when bit 1, 3 or 4 is set, run the synchronous entry point once and call
+ * again. */
 int pbig_hip_check_device_async(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
                                 float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
                                 uint64_t* d_result, void* stream);

@@ -19,6 +19,8 @@ constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent te
 constexpr uint32_t kBatch = 4;               // boxes per thread loaded before use (hist): a 4,096-box chunk in one pass
 constexpr uint32_t kScatterChunk = 4096;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
+constexpr unsigned long long kErrDense = 16ull;    // heavy bins hold over a quarter of the memberships
+                                                   // (checked only when the plan asks: dense_exit)
 // Overfull list entries: the bin, with this bit when a single fine cell
 // exceeded kMaxFine (re-binning cannot help; the instance needs the general
 // path) rather than the bin exceeding kMaxCoarse memberships.
@@ -61,6 +63,7 @@ struct BatchPlan {
   uint64_t n_boxes = 0, hist_entries = 0, off_entries = 0;
   uint64_t extra_total = 0;  // pairs found outside the launch (sum of pairs_extra)
   uint32_t nb_max = 0;       // bins of the largest instance (sizes the scatter's LDS)
+  uint32_t dense_exit = 0;   // single instance: stop after the bin totals when it is dense (kErrDense)
 };
 
 struct BinnedWorkspace {

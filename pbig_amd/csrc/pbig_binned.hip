@@ -231,17 +231,35 @@ __device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get
 }
 
 // Bucket offset of every bin (exclusive scan of all bins' totals), the
-// grand total and the workspace check.
+// grand total and the workspace check.  With dense_exit (a launch of one
+// instance), an instance whose heavy bins (over kMaxCoarse memberships) hold
+// more than a quarter of its memberships is flagged kErrDense, and the
+// scatter and test kernels then do nothing.
 __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
                                                      uint32_t* __restrict__ bin_start, uint64_t cap,
-                                                     Meta* __restrict__ meta) {
+                                                     uint32_t dense_exit, Meta* __restrict__ meta) {
   __shared__ ScanLds L;
+  __shared__ unsigned long long heavy;
+  if (threadIdx.x == 0) heavy = 0ull;
   const uint64_t grand = block_exscan(
       n_bins, L, [&](uint32_t i) { return (uint64_t)total[i]; },
       [&](uint32_t i, uint64_t ex) { bin_start[i] = (uint32_t)(ex < cap ? ex : cap); });
+  if (dense_exit) {
+    uint64_t h = 0;
+    for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
+      const uint32_t t = total[i];
+      h += t > kMaxCoarse ? t : 0u;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
+    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&heavy, (unsigned long long)h);
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     meta->memberships = grand;
     if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
+    if (dense_exit && 4ull * heavy > grand)
+      atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrDense);
   }
 }
 
@@ -296,6 +314,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   const uint32_t wc = (blockIdx.x & 7u) * xper + (blockIdx.x >> 3);
   if (wc >= n_chunks) return;
   if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
+  if (meta->error & kErrDense) return;  // the host runs the general path
   const InstDesc I = inst[chunk_inst[wc]];
   const uint32_t nb = I.nb;
   uint64_t start, end;
@@ -519,7 +538,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->error & (kErrInvalidBox | kErrWorkspace)) {
+  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
   }
@@ -926,7 +945,7 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   k_bin_scan<<<(nbins + 15) / 16, 256, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
-  k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, ws.meta);
+  k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;

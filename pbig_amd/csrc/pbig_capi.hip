@@ -574,6 +574,7 @@ int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, cons
     *m = *c->h_meta;
     if (m->error & pbig::kErrInvalidBox)
       return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    if (m->error & pbig::kErrDense) return PBIG_OK;  // the caller runs the general path
     if (m->error & pbig::kErrWorkspace) {
       if (attempt == 2) return fail(c, PBIG_E_NOMEM, "bin workspace did not fit after regrowth");
       if (m->items > st.ws.cap_wg)
@@ -669,6 +670,10 @@ int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_
   *sp = Splice(n_inst);
   int rc;
   if (!pass_done && (rc = run_state_sync(c, st, boxes, ids, d_out, cap, d_io, s, m))) return rc;
+  if (m->error & pbig::kErrDense) {  // a dense single instance: the general path
+    failed->assign(n_inst, 1);
+    return PBIG_OK;
+  }
   if (!(m->error & pbig::kErrOverfull)) return PBIG_OK;
   const uint32_t n_bins = st.bp.n_bins;
   const uint32_t n_over = (uint32_t)std::min<uint64_t>(m->max_per_cell, n_bins);
@@ -783,6 +788,7 @@ int run_any_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, double rang
   if (c->cfg.path != PBIG_PATH_GENERAL) {
     bool ok = false;
     if ((rc = plan_plain(c, n, range, cell, s, &ok))) return rc;
+    c->plain.bp.dense_exit = c->cfg.path == PBIG_PATH_AUTO ? 1u : 0u;
     if (ok) {
       rc = run_binned_sync(c, d_boxes, n, d_out, cap, s, n_pairs);
       if (rc != 1) return rc;
@@ -964,6 +970,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
   if (binned) {
     bool ok = false;
     if ((rc = plan_plain(c, n, range, cell_size, s, &ok))) return rc;
+    c->plain.bp.dense_exit = c->cfg.path == PBIG_PATH_AUTO ? 1u : 0u;
     if (!ok && c->cfg.path == PBIG_PATH_BINNED)
       return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
     binned = ok;

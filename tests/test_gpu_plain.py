@@ -307,6 +307,7 @@ def test_dense_spot_resolved_on_binned_path(torch_cuda, orc):
     c = rng.uniform(10, 12, size=(6000, 2)).astype(np.float32)
     boxes = np.stack([c[:, 0], c[:, 0] + 0.5, c[:, 1], c[:, 1] + 0.5], 1).astype(np.float32)
     with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["binned"])  # the automatic choice sends a spot this dense to the general path
         got = run_device(torch_cuda, e, boxes, 64.0, 16)
         assert e.stats()["path"] == PATHS["binned"]
     assert np.array_equal(got, orc.sort_pairs(orc.brute_pairs(boxes)))
@@ -339,6 +340,35 @@ def test_binned_c3_rebins_heavy_bins(torch_cuda, binned, orc, manifest, cs):
     assert binned.stats()["path"] == PATHS["binned"]
     assert len(got) == m["pairs"]
     assert orc.pair_hash(got) == m["sha256"]
+
+
+def test_auto_sends_dense_instance_to_general_path(torch_cuda, orc, manifest):
+    """The clustered instance under the automatic choice: its heavy bins hold
+    most memberships, so the asynchronous binned launch stops after the bin
+    totals with error bit 4 (no pairs), the synchronous call runs the general
+    path, and the next asynchronous call follows it, with the exact set."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c3_clustered()
+    m = manifest["c3_clustered_2m"]
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((m["pairs"] + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["auto"], 2000)
+        e.check_device_async(d, L, 64.0, out, res)
+        torch_cuda.cuda.synchronize()
+        n_async, err = (int(v) for v in res.cpu().tolist())
+        assert err & 16 and n_async == 0, (n_async, err)
+        n = e.check_device(d, L, 64.0, out)
+        assert e.stats()["path"] == PATHS["general"]
+        assert n == m["pairs"] and orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
+        res.zero_()
+        e.check_device_async(d, L, 64.0, out, res)
+        torch_cuda.cuda.synchronize()
+        assert res.cpu().tolist() == [m["pairs"], 0]
+        assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
 
 
 def test_binned_deep_rebinning(torch_cuda, binned, orc):
