@@ -605,6 +605,7 @@ struct Splice {
 };
 
 constexpr int kMaxLevels = 4;  // re-binning depth before an instance goes to the general path
+constexpr size_t kFewDense = 8;  // dense instances of a batch that go to the general path rather than re-binning
 
 // Give every instance its extra pairs (pairs0 / pairs_extra), discard the
 // dropped instances' bins, rerun the pair scan and output kernels and copy
@@ -692,6 +693,22 @@ int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_
     else heavy.push_back(b);
   }
   std::sort(heavy.begin(), heavy.end());
+  if (level == 0 && c->cfg.path == PBIG_PATH_AUTO && !heavy.empty()) {
+    // Instances whose heavy bins hold more than a quarter of their
+    // memberships are mostly dense clusters.  Re-binning them costs a few
+    // host round trips per level (about a millisecond) however many there
+    // are; the general path costs two synchronous runs (about 0.1 ms) per
+    // instance.  A few such instances go to the general path; many are
+    // re-binned together.
+    std::vector<uint64_t> dense(n_inst, 0), all(n_inst, 0);
+    for (uint32_t b : heavy) dense[st.bin_inst[b]] += total[b];
+    for (uint32_t b = 0; b < n_bins; ++b) all[st.bin_inst[b]] += total[b];
+    std::vector<uint32_t> clustered;
+    for (size_t i = 0; i < n_inst; ++i)
+      if (!(*failed)[i] && 4 * dense[i] > all[i]) clustered.push_back((uint32_t)i);
+    if (clustered.size() <= kFewDense)
+      for (uint32_t i : clustered) (*failed)[i] = 1;
+  }
   if (!st.sub) {
     st.sub = std::make_unique<BinnedState>();
     HIP_TRY(c, hipMalloc((void**)&st.sub->ws.meta, sizeof(Meta)));

@@ -19,7 +19,10 @@ per launch:
   the per-request (offset, count) table come back by asynchronous copies.
   Requests that arrive while batches run are admitted as pool space frees up
   and go into the next free slot, so uploads, the kernels of several batches
-  and result copies overlap on the device.
+  and result copies overlap on the device.  A batch of one request (light
+  load) runs on the plain path instead, on the slot's own context: one
+  launch, no batch plan, and a dense request goes to the general path after
+  the bin totals.
 * Completion.  With ``results="device"`` a request's pairs stay in HBM (a
   view of the slot's pair buffer, valid while the consumer runs); with
   ``results="host"`` they are copied to pinned host memory first.  The
@@ -136,6 +139,8 @@ class _Slot:
     h_pairs: object = None           # pinned (host results)
     ev: object = None
     state: str = "free"              # free | running | copying
+    mode: str = "batch"              # batch | plain (a single request on the plain path)
+    eng: object = None               # the slot's own context for plain launches
     reqs: list = field(default_factory=list)
     bid: int = -1
     total: int = 0
@@ -193,6 +198,9 @@ class Server:
             if s.batch is not None:
                 s.batch.close()
                 s.batch = None
+            if s.eng is not None:
+                s.eng.close()
+                s.eng = None
         self.eng.close()
 
     # -- server loop -----------------------------------------------------------
@@ -253,17 +261,29 @@ class Server:
         Ls = [r.L for r in reqs]
         cells = [r.cell_size for r in reqs]
         self._ensure_buffers(s, len(reqs), total)
-        if s.batch is None:
-            s.batch = pbig_amd.Batch.from_instances(self.eng, starts, counts, Ls, cells)
+        s.mode = "plain" if len(reqs) == 1 and reqs[0].n > 0 else "batch"
+        if s.mode == "plain":
+            if s.eng is None:
+                s.eng = pbig_amd.Engine(self.dev.index or 0)
+            s.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))  # no memory of earlier requests
+            if self.upload_ev is not None:
+                s.stream.wait_event(self.upload_ev)
+            r = reqs[0]
+            s.eng.check_device_async(self.d_pool[r._off:r._off + r.n], r.L, r.cell_size, s.pairs, s.res,
+                                     stream=s.stream)
         else:
-            s.batch.update(starts, counts, Ls, cells, stream=s.stream)
-        if self.upload_ev is not None:
-            s.stream.wait_event(self.upload_ev)
-        s.batch.launch(self.d_pool, s.pairs, s.io, s.res, stream=s.stream)
+            if s.batch is None:
+                s.batch = pbig_amd.Batch.from_instances(self.eng, starts, counts, Ls, cells)
+            else:
+                s.batch.update(starts, counts, Ls, cells, stream=s.stream)
+            if self.upload_ev is not None:
+                s.stream.wait_event(self.upload_ev)
+            s.batch.launch(self.d_pool, s.pairs, s.io, s.res, stream=s.stream)
         k = len(reqs)
         with torch.cuda.stream(s.stream):
             s.h_res.copy_(s.res, non_blocking=True)
-            s.h_io[:k].copy_(s.io[:k], non_blocking=True)
+            if s.mode == "batch":
+                s.h_io[:k].copy_(s.io[:k], non_blocking=True)
         s.ev = torch.cuda.Event()
         s.ev.record(s.stream)
         for r in reqs:
@@ -292,7 +312,25 @@ class Server:
             total, err = (int(v) for v in s.h_res.tolist())
             for r in s.reqs:
                 r.t_ready = now
-            if err != 0 or total > s.pairs.shape[0]:
+            if s.mode == "plain":
+                r = s.reqs[0]
+                if err != 0 or total > s.pairs.shape[0]:
+                    # dense (general path), bins beyond LDS, workspace or
+                    # pair capacity: the synchronous call resolves it
+                    view = self.d_pool[r._off:r._off + r.n]
+                    if err & 16:  # dense: straight to the general path
+                        s.eng.configure(pbig_amd.PATH_GENERAL, int(self.table["bin_target"]))
+                    total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream,
+                                               allow_overflow=True)
+                    if total > s.pairs.shape[0]:
+                        cap = total + total // 4
+                        s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
+                        if self.results == "host":
+                            s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+                        total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
+                s.h_io[0, 0] = 0
+                s.h_io[0, 1] = total
+            elif err != 0 or total > s.pairs.shape[0]:
                 # bins beyond LDS (finished from this launch's results),
                 # workspace regrowth or too many pairs (run again)
                 total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream,

@@ -253,11 +253,16 @@ def test_batch_c4_full(torch_cuda, eng):
         assert np.array_equal(g, w), i
 
 
-def test_batch_clustered_instances_rebinned(torch_cuda, eng):
+@pytest.mark.parametrize("path", ["binned", "auto"])
+def test_batch_clustered_instances_rebinned(torch_cuda, eng, path):
     """Clustered instances whose coarse bins exceed LDS, mixed with uniform
-    ones: heavy bins are re-binned and spliced into their instance's run."""
+    ones: on the binned path heavy bins are re-binned and spliced into their
+    instance's run; under the automatic choice these few dense instances go
+    to the general path instead.  Exact either way."""
     import pbig_amd
     from pbig_amd import workloads
+
+    eng.configure(pbig_amd.PATH_BINNED if path == "binned" else pbig_amd.PATH_AUTO)
 
     inst = []
     for t in range(10):
