@@ -34,8 +34,8 @@ constexpr uint32_t kOverfullCell = 0x80000000u;
 // span_y), in bins of k < parent k cells).
 struct InstDesc {
   uint64_t box0;     // first box of the instance in the box array
-  uint64_t hist0;    // first entry of its histogram rows (nchunks x (nb + 1))
-  uint64_t off0;     // first entry of its offset rows (nchunks x nb)
+  uint64_t hist0;    // first entry of its histogram rows (nchunks x hist_pitch(nb))
+  uint64_t off0;     // first entry of its offset rows (nchunks x off_pitch(nb))
   uint64_t k_magic;  // ceil(2^32 / k): fine index -> coarse bin by multiply
   uint64_t pairs0;   // pairs of earlier instances found outside the launch
   uint64_t pairs_extra;  // this instance's pairs found outside the launch
@@ -58,6 +58,12 @@ struct InstDesc {
 static_assert(sizeof(InstDesc) == 112, "InstDesc layout");
 
 // Totals of a batch (host side).
+// Row pitches of the histogram (nb counts + the chunk's invalid-box word)
+// and offset tables, in entries: whole 128-byte lines, so that the scan's
+// 32-bin column reads are whole lines.
+__host__ __device__ inline uint32_t hist_pitch(uint32_t nb) { return (nb + 1u + 31u) & ~31u; }
+__host__ __device__ inline uint32_t off_pitch(uint32_t nb) { return (nb + 31u) & ~31u; }
+
 struct BatchPlan {
   uint32_t n_inst = 0, n_chunks = 0, n_bins = 0, test_blocks = 0;
   uint64_t n_boxes = 0, hist_entries = 0, off_entries = 0;

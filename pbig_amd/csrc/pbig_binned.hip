@@ -122,64 +122,94 @@ __global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ bo
     }
   }
   __syncthreads();
-  uint32_t* row = hist + I.hist0 + (uint64_t)(blockIdx.x - I.chunk0) * (I.nb + 1);
+  uint32_t* row = hist + I.hist0 + (uint64_t)(blockIdx.x - I.chunk0) * hist_pitch(I.nb);
   for (uint32_t b = threadIdx.x; b < I.nb; b += blockDim.x) row[b] = h[b];
   if (threadIdx.x == 0) row[I.nb] = err;
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bin_scan(const uint32_t* __restrict__ hist,
-                                                   const InstDesc* __restrict__ inst,
-                                                   const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
-                                                   const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
-                                                   uint32_t* __restrict__ off, uint32_t* __restrict__ total,
-                                                   Meta* __restrict__ meta) {
-  // Thread t handles global bin blockIdx.x * 16 + (t % 16) over row group t / 16.
-  __shared__ uint32_t part[16][16];
-  const uint32_t bl = threadIdx.x & 15, g = threadIdx.x >> 4;
-  const uint32_t gb = blockIdx.x * 16 + bl;
-  uint32_t s = 0, b = 0, nb = 0, w0 = 0, w1 = 0;
+// Per bin, exclusive scan of its histogram column over the instance's
+// chunks.  A block takes 32 bins (one 128-byte line of a row) and 32 groups
+// of chunks; each thread holds the 32 counts of its chunks in registers, so
+// the table is read once, in whole lines, and the offsets are written from
+// the registers after a prefix over the groups.  Instances of more than
+// 1,024 chunks take several tiles, the running total carried.
+constexpr uint32_t kScanBins = 32, kScanGroups = 32, kScanPer = 32;
+
+__global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ hist,
+                                                    const InstDesc* __restrict__ inst,
+                                                    const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
+                                                    const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
+                                                    uint32_t* __restrict__ off, uint32_t* __restrict__ total,
+                                                    Meta* __restrict__ meta) {
+  __shared__ uint32_t part[kScanGroups][kScanBins + 1];
+  __shared__ uint32_t carry[kScanBins];
+  __shared__ uint32_t tiles;
+  const uint32_t bl = threadIdx.x % kScanBins, g = threadIdx.x / kScanBins;
+  const uint32_t gb = blockIdx.x * kScanBins + bl;
+  uint32_t b = 0, nch = 0, hp = 0, op = 0;
   uint64_t hist0 = 0, off0 = 0;
+  if (threadIdx.x == 0) tiles = 0u;
+  if (g == 0) carry[bl] = 0u;
   if (gb < n_bins) {
     const InstDesc I = inst[bin_inst[gb]];
     b = gb - I.bin0;
-    nb = I.nb;
+    nch = I.nchunks;
+    hp = hist_pitch(I.nb);
+    op = off_pitch(I.nb);
     hist0 = I.hist0;
     off0 = I.off0;
-    const uint32_t per = (I.nchunks + 15) / 16;
-    w0 = g * per;
-    w1 = w0 + per < I.nchunks ? w0 + per : I.nchunks;
-#pragma unroll 8
-    for (uint32_t c = w0; c < w1; ++c) s += hist[hist0 + (uint64_t)c * (nb + 1) + b];
   }
-  part[g][bl] = s;
   __syncthreads();
-  uint32_t base = 0;
-  for (uint32_t k = 0; k < g; ++k) base += part[k][bl];
-  if (gb < n_bins) {
-#pragma unroll 8
-    for (uint32_t c = w0; c < w1; ++c) {
-      const uint32_t v = hist[hist0 + (uint64_t)c * (nb + 1) + b];
-      off[off0 + (uint64_t)c * nb + b] = base;
-      base += v;
+  if (g == 0 && nch) atomicMax(&tiles, (nch + kScanGroups * kScanPer - 1) / (kScanGroups * kScanPer));
+  __syncthreads();
+  const uint32_t nt = tiles;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t c0 = t * kScanGroups * kScanPer + g * kScanPer;
+    uint32_t v[kScanPer], s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+      v[k] = c0 + k < nch ? hist[hist0 + (uint64_t)(c0 + k) * hp + b] : 0u;
+      s += v[k];
     }
-    if (g == 15) total[gb] = base;
+    part[g][bl] = s;
+    __syncthreads();
+    if (g == 0) {  // exclusive prefix over the groups, after the earlier tiles
+      uint32_t acc = carry[bl];
+      for (uint32_t k = 0; k < kScanGroups; ++k) {
+        const uint32_t x = part[k][bl];
+        part[k][bl] = acc;
+        acc += x;
+      }
+      carry[bl] = acc;
+    }
+    __syncthreads();
+    uint32_t base = part[g][bl];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+      if (c0 + k < nch) off[off0 + (uint64_t)(c0 + k) * op + b] = base;
+      base += v[k];
+    }
+    __syncthreads();  // part[] is rewritten by the next tile
   }
+  if (g == 0 && gb < n_bins) total[gb] = carry[bl];
   if (blockIdx.x == 0) {
     // Result words of the call: pair counter and the boxes' validity.
     uint32_t e = 0;
     for (uint32_t w = threadIdx.x; w < n_chunks; w += blockDim.x) {
       const InstDesc I = inst[chunk_inst[w]];
-      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * (I.nb + 1) + I.nb];
+      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * hist_pitch(I.nb) + I.nb];
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
-    __shared__ uint32_t es[4];
+    __shared__ uint32_t es[16];
     if ((threadIdx.x & 63) == 0) es[threadIdx.x >> 6] = e;
     __syncthreads();
     if (threadIdx.x == 0) {
+      uint32_t any = 0;
+      for (int k = 0; k < 16; ++k) any |= es[k];
       meta->n_pairs = 0ull;
-      meta->error = (es[0] | es[1] | es[2] | es[3]) ? kErrInvalidBox : 0ull;
+      meta->error = any ? kErrInvalidBox : 0ull;
       meta->items = 0ull;
       meta->max_per_cell = 0u;
     }
@@ -329,8 +359,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   // 2. per bin: destination cursor, and the offset from a destination to the
   //    membership's rank among this chunk's memberships in bin order
   const uint64_t lc = wc - I.chunk0;
-  const uint32_t* myoff = off + I.off0 + lc * nb;
-  const uint32_t* myhist = hist + I.hist0 + lc * (nb + 1);
+  const uint32_t* myoff = off + I.off0 + lc * off_pitch(nb);
+  const uint32_t* myhist = hist + I.hist0 + lc * hist_pitch(nb);
   const uint32_t* mystart = bin_start + I.bin0;
   uint32_t chunk_total = 0;
   {
@@ -942,7 +972,7 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(0)
   k_bin_hist<<<nc, 1024, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
   PBIG_MARK(1)
-  k_bin_scan<<<(nbins + 15) / 16, 256, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
+  k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
   k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);

@@ -357,8 +357,8 @@ bool layout_tables(BinnedState& st, uint32_t n_cu, const std::vector<uint64_t>* 
     chunks += nchunks;
     bins += nb;
     if (nchunks) bp.nb_max = std::max(bp.nb_max, d.nb);
-    hist += (uint64_t)nchunks * (d.nb + 1);
-    off += (uint64_t)nchunks * d.nb;
+    hist += (uint64_t)nchunks * pbig::hist_pitch(d.nb);
+    off += (uint64_t)nchunks * pbig::off_pitch(d.nb);
     bp.n_boxes += skipped ? 0u : d.n;
   }
   if (chunks > 0x7FFFFFFFull || bins > 0x7FFFFFFFull) return false;
