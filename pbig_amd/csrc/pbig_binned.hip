@@ -481,6 +481,7 @@ struct TestLds {
   uint32_t wsum[kTestWaves];
   uint32_t out_n;                  // pairs appended to this workgroup's region
   uint32_t strip_hi;
+  unsigned long long out_base;     // direct output: this workgroup's first output position
 };
 static_assert(sizeof(TestLds) <= 160u * 1024u, "one test workgroup per CU");
 
@@ -563,6 +564,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                                                            uint64_t cap_bucket, uint2* __restrict__ scratch,
                                                            uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
+                                                           uint2* __restrict__ out, uint64_t cap, uint32_t direct,
                                                            Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
@@ -866,6 +868,18 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
     atomicMax((unsigned long long*)&meta->items, (unsigned long long)seg0);
     atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
   }
+  if (direct) {
+    // A single instance needs no bin order: the region (every bin's pairs,
+    // dropped ones rewound) goes to a range of the output reserved with one
+    // device atomic on the pair counter, which ends as the total.
+    __syncthreads();
+    const uint32_t n = L.out_n < cap_wg ? L.out_n : cap_wg;
+    if (tid == 0) L.out_base = L.out_n ? atomicAdd((unsigned long long*)&meta->n_pairs, (unsigned long long)L.out_n) : 0ull;
+    __syncthreads();
+    const uint64_t base = L.out_base;
+    for (uint32_t k = tid; k < n; k += kTestThreads)
+      if (base + k < cap) out[base + k] = region[k];
+  }
 }
 
 // Output position of every bin's pairs: exclusive scan of the bins' pair
@@ -991,12 +1005,14 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
-                                                                   ws.overfull, ws.meta);
+                                                                   ws.overfull, out, cap, bp.direct, ws.meta);
   PBIG_MARK(5)
-  k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
+  if (!bp.direct)
+    k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
   PBIG_MARK(6)
-  k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
-                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+  if (!bp.direct)
+    k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
+                                        ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
   PBIG_MARK(7)
 #undef PBIG_MARK
   return hipGetLastError();

@@ -643,6 +643,18 @@ int apply_splice(pbig_hip_ctx* c, BinnedState& st, const Splice& sp, uint2* d_ou
   if (!runs.empty())
     HIP_TRY(c, hipMemcpyAsync(st.runs, runs.data(), runs.size() * sizeof(pbig::SpliceRun), hipMemcpyHostToDevice,
                               s));
+  if (bp.direct) {
+    // A single instance whose launch wrote its own pairs to d_out[0, n0):
+    // the extra pairs go right after them (one instance, never dropped).
+    const uint64_t n0 = m->n_pairs;
+    const uint64_t hio[2] = {0, n0 + acc};  // alive until the synchronisation below
+    HIP_TRY(c, hipMemcpyAsync(d_io, hio, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, pbig::launch_splice_copy(st.runs, (uint32_t)runs.size(), d_io, st.ws.inst, d_out, cap, s));
+    if (n) HIP_TRY(c, hipMemcpyAsync(st.ws.inst, st.inst.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    m->n_pairs = n0 + acc;
+    return PBIG_OK;
+  }
   for (size_t i = 0; i < n; ++i)
     if (sp.drop[i] && tab[i].nb && tab[i].n)
       HIP_TRY(c, hipMemsetAsync(st.ws.bin_seg + tab[i].bin0, 0, tab[i].nb * sizeof(uint2), s));
@@ -806,6 +818,7 @@ int run_any_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, double rang
     bool ok = false;
     if ((rc = plan_plain(c, n, range, cell, s, &ok))) return rc;
     c->plain.bp.dense_exit = c->cfg.path == PBIG_PATH_AUTO ? 1u : 0u;
+    c->plain.bp.direct = 1u;
     if (ok) {
       rc = run_binned_sync(c, d_boxes, n, d_out, cap, s, n_pairs);
       if (rc != 1) return rc;
@@ -988,6 +1001,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
     bool ok = false;
     if ((rc = plan_plain(c, n, range, cell_size, s, &ok))) return rc;
     c->plain.bp.dense_exit = c->cfg.path == PBIG_PATH_AUTO ? 1u : 0u;
+    c->plain.bp.direct = 1u;
     if (!ok && c->cfg.path == PBIG_PATH_BINNED)
       return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
     binned = ok;
