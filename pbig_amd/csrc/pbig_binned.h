@@ -69,6 +69,7 @@ struct BatchPlan {
   uint64_t n_boxes = 0, hist_entries = 0, off_entries = 0;
   uint64_t extra_total = 0;  // pairs found outside the launch (sum of pairs_extra)
   uint32_t nb_max = 0;       // bins of the largest instance (sizes the scatter's LDS)
+  uint32_t n_cu = 0;         // compute units of the device (persistent grids)
   uint32_t dense_exit = 0;   // single instance: stop after the bin totals when it is dense (kErrDense)
   uint32_t direct = 0;       // single instance: the test kernel writes the output itself, in no bin order
                              // (no pair scan, no compaction, no per-instance table)

@@ -335,27 +335,37 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   uint32_t* const cur = reinterpret_cast<uint32_t*>(lds_raw + sizeof(ScatterLds));
   int32_t* const woff = reinterpret_cast<int32_t*>(cur + nb_cap);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // Workgroups go to the 8 XCDs round-robin.  Block i takes chunk
-  // (i mod 8) * per + i / 8, so each XCD walks one contiguous range of
-  // chunks, and the runs that chunks c and c + 1 write side by side in every
-  // bin's bucket come from the same XCD at about the same time (the grid is
-  // rounded up to a multiple of 8; blocks past the last chunk exit).
-  const uint32_t xper = (n_chunks + 7u) / 8u;
-  const uint32_t wc = (blockIdx.x & 7u) * xper + (blockIdx.x >> 3);
-  if (wc >= n_chunks) return;
+  // Persistent workgroups, XCD-aware.  Workgroups go to the 8 XCDs
+  // round-robin; the gridDim / 8 workgroups of XCD x walk that XCD's
+  // contiguous range of chunks (x * per .. x * per + per - 1) together, so
+  // the runs that chunks c and c + 1 write side by side in every bin's
+  // bucket come from the same XCD at about the same time and merge in its
+  // L2.  The next chunk's boxes are loaded while the current chunk's staged
+  // memberships are written out.
   if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
   if (meta->error & kErrDense) return;  // the host runs the general path
+  const uint32_t xper = (n_chunks + 7u) / 8u;
+  const uint32_t xnext = gridDim.x / 8u;  // workgroups per XCD (the grid is a multiple of 8)
+  uint32_t r = blockIdx.x >> 3;
+  uint32_t wc = (blockIdx.x & 7u) * xper + r;
+  const uint32_t wend = min((blockIdx.x & 7u) * xper + xper, n_chunks);
+  float4 q[kScatterPer];
+  auto load_chunk = [&](uint32_t c) {
+    const InstDesc J = inst[chunk_inst[c]];
+    uint64_t st, en;
+    chunk_range(J, c, &st, &en);
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterPer; ++k) {
+      const uint64_t i = st + tid + k * kScatterThreads;
+      q[k] = boxes[i < en ? i : en - 1];
+    }
+  };
+  if (wc < wend) load_chunk(wc);
+  for (; wc < wend; wc += xnext) {
   const InstDesc I = inst[chunk_inst[wc]];
   const uint32_t nb = I.nb;
   uint64_t start, end;
   chunk_range(I, wc, &start, &end);
-  // 1. this thread's boxes, all loads in flight at once
-  float4 q[kScatterPer];
-#pragma unroll
-  for (uint32_t k = 0; k < kScatterPer; ++k) {
-    const uint64_t i = start + tid + k * kScatterThreads;
-    q[k] = boxes[i < end ? i : end - 1];
-  }
   // 2. per bin: destination cursor, and the offset from a destination to the
   //    membership's rank among this chunk's memberships in bin order
   const uint64_t lc = wc - I.chunk0;
@@ -363,6 +373,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   const uint32_t* myhist = hist + I.hist0 + lc * hist_pitch(nb);
   const uint32_t* mystart = bin_start + I.bin0;
   uint32_t chunk_total = 0;
+  __syncthreads();  // the previous chunk's stage and cursors are done with
   {
     const uint32_t per = (nb + kScatterThreads - 1) / kScatterThreads;
     const uint32_t b0 = tid * per;
@@ -400,11 +411,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     const uint64_t gi = start + tid + k * kScatterThreads;
     const float4 b = q[k];
     if (!(gi < end && (b.x <= b.y) && (b.z <= b.w))) continue;
-    const BinRange r = bin_range(b, I);
+    const BinRange rg = bin_range(b, I);
     uint32_t i = (uint32_t)(gi - I.box0);  // id inside the instance
     if (kIds) i = ids_in[gi];
-    for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-      for (uint32_t cx = r.x0; cx <= r.x1; ++cx) {
+    for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
+      for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) {
         const uint32_t bb = cy * I.nbx + cx;
         const uint32_t d = atomicAdd(&cur[bb], 1u);
         const uint32_t at = (uint32_t)((int32_t)d + woff[bb]);
@@ -419,6 +430,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       }
   }
   __syncthreads();
+  if (wc + xnext < wend) load_chunk(wc + xnext);  // in flight during the write-out
   // 4. the staged memberships in rank order
   const uint32_t staged = chunk_total < kStageEntries ? chunk_total : kStageEntries;
   for (uint32_t k = tid; k < staged; k += kScatterThreads) {
@@ -427,6 +439,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       bbox[d] = L.box[k];
       bid[d] = L.id[k];
     }
+  }
   }
 }
 
@@ -993,12 +1006,15 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;
+  // persistent: at most one workgroup per CU, a multiple of 8 (one share per XCD)
+  const uint32_t xcd_share = bp.n_cu >= 16u ? bp.n_cu / 8u : 1u;
+  const uint32_t sgrid = (nc + 7u) / 8u < xcd_share ? (nc + 7u) / 8u * 8u : xcd_share * 8u;
   if (ids_in)
-    k_bin_scatter<true><<<(nc + 7u) / 8u * 8u, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<true><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   else
-    k_bin_scatter<false><<<(nc + 7u) / 8u * 8u, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<false><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   PBIG_MARK(4)

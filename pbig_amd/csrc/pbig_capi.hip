@@ -376,6 +376,7 @@ bool layout_tables(BinnedState& st, uint32_t n_cu, const std::vector<uint64_t>* 
   bp.off_entries = off;
   bp.extra_total = pairs0;
   bp.test_blocks = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)n_cu, (uint32_t)std::max<uint64_t>(bins, 1)));
+  bp.n_cu = n_cu;
   return true;
 }
 
