@@ -247,8 +247,10 @@ int run_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, GridParams gp, 
 // Chunks of at most kScatterChunk boxes (the scatter holds its chunk in
 // registers); more than 8 chunks are rounded to a multiple of 8 so that the
 // chunks spread evenly over the 8 XCDs.  False when the grid cannot be
-// planned (the general path then runs).
-bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, InstDesc* d) {
+// planned (the general path then runs).  With n_cu (a launch of this one
+// instance), k may move by one from the target's so that the persistent test
+// workgroups' largest share, ceil(bins / n_cu) bins of k^2 cells, is least.
+bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, InstDesc* d, uint32_t n_cu = 0) {
   *d = InstDesc{};
   d->n = n;
   if (n == 0) return true;  // no chunks, no bins
@@ -257,6 +259,19 @@ bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, In
   double kd = std::sqrt(target / std::max<double>(n, 1)) * nf0;
   kd = std::min<double>(std::max<double>(std::round(kd), 1.0), std::min<double>(pbig::kMaxK, nf0));
   uint32_t k = (uint32_t)kd;
+  if (n_cu) {
+    const uint32_t k0 = k;
+    double best = 0.0;
+    for (uint32_t kc = k0 > 1 ? k0 - 1 : 1; kc <= k0 + 1 && kc <= pbig::kMaxK && kc <= nf0; ++kc) {
+      const double bx = std::ceil(nf0 / kc), nbins = bx * bx;
+      if (nbins > pbig::kMaxBins) continue;
+      const double share = std::ceil(nbins / n_cu) * (double)kc * kc;
+      if (best == 0.0 || share < best) {
+        best = share;
+        k = kc;
+      }
+    }
+  }
   uint32_t nbx = 0;
   while (true) {
     const double nbxd = std::ceil(nf0 / k);
@@ -520,7 +535,7 @@ int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_
     return PBIG_OK;
   st.key_valid = false;
   InstDesc d;
-  if (n == 0 || !plan_instance(n, range, cell, c->cfg.bin_target, &d)) {
+  if (n == 0 || !plan_instance(n, range, cell, c->cfg.bin_target, &d, (uint32_t)c->n_cu)) {
     *ok = false;
     return PBIG_OK;
   }
