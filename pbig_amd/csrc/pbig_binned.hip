@@ -1006,8 +1006,10 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;
-  // persistent: at most one workgroup per CU, a multiple of 8 (one share per XCD)
-  const uint32_t xcd_share = bp.n_cu >= 16u ? bp.n_cu / 8u : 1u;
+  // A single instance's chunks are all alike: persistent workgroups, one per
+  // CU (a multiple of 8, one share per XCD).  A batch's chunks differ in
+  // size: one workgroup per chunk, balanced by the dispatcher.
+  const uint32_t xcd_share = bp.n_inst == 1u && bp.n_cu >= 16u ? bp.n_cu / 8u : (nc + 7u) / 8u;
   const uint32_t sgrid = (nc + 7u) / 8u < xcd_share ? (nc + 7u) / 8u * 8u : xcd_share * 8u;
   if (ids_in)
     k_bin_scatter<true><<<sgrid, kScatterThreads, scatter_lds, s>>>(
