@@ -23,6 +23,11 @@ per launch:
   load) runs on the plain path instead, on the slot's own context: one
   launch, no batch plan, and a dense request goes to the general path after
   the bin totals.
+* Completion work off the loop.  A launch whose result word asks for host
+  work (re-binning dense bins, a dense plain request, regrowth) is finished
+  by a synchronous call in a worker thread; every slot has its own context,
+  so the loop keeps admitting, launching and delivering for the other slots
+  meanwhile.
 * Completion.  With ``results="device"`` a request's pairs stay in HBM (a
   view of the slot's pair buffer, valid while the consumer runs); with
   ``results="host"`` they are copied to pinned host memory first.  The
@@ -36,6 +41,7 @@ output run); the tests check this against the request run alone.
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -138,9 +144,10 @@ class _Slot:
     h_io: object = None              # pinned
     h_pairs: object = None           # pinned (host results)
     ev: object = None
-    state: str = "free"              # free | running | copying
+    state: str = "free"              # free | running | finishing | copying
     mode: str = "batch"              # batch | plain (a single request on the plain path)
-    eng: object = None               # the slot's own context for plain launches
+    eng: object = None               # the slot's own context (its batch and plain launches)
+    future: object = None            # completion work running in a worker thread
     reqs: list = field(default_factory=list)
     bid: int = -1
     total: int = 0
@@ -177,6 +184,10 @@ class Server:
         self.max_batch_boxes = max_batch_boxes or max(self.slot_budget // bytes_per_box, 1)
         self.copy_stream = torch.cuda.Stream(device=self.dev)
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(slots)]
+        for sl in self.slots:
+            sl.eng = pbig_amd.Engine(device)
+            sl.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
+        self.pool_exec = ThreadPoolExecutor(max_workers=max(slots, 1))
         self.queue: list[Request] = []       # arrived, not admitted
         self.admitted: list[Request] = []    # upload issued, not yet batched
         self.upload_ev = None
@@ -193,6 +204,7 @@ class Server:
         self.queue.append(req)
 
     def close(self):
+        self.pool_exec.shutdown(wait=True)
         self.torch.cuda.synchronize()
         for s in self.slots:
             if s.batch is not None:
@@ -263,8 +275,6 @@ class Server:
         self._ensure_buffers(s, len(reqs), total)
         s.mode = "plain" if len(reqs) == 1 and reqs[0].n > 0 else "batch"
         if s.mode == "plain":
-            if s.eng is None:
-                s.eng = pbig_amd.Engine(self.dev.index or 0)
             s.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))  # no memory of earlier requests
             if self.upload_ev is not None:
                 s.stream.wait_event(self.upload_ev)
@@ -273,7 +283,7 @@ class Server:
                                      stream=s.stream)
         else:
             if s.batch is None:
-                s.batch = pbig_amd.Batch.from_instances(self.eng, starts, counts, Ls, cells)
+                s.batch = pbig_amd.Batch.from_instances(s.eng, starts, counts, Ls, cells)
             else:
                 s.batch.update(starts, counts, Ls, cells, stream=s.stream)
             if self.upload_ev is not None:
@@ -306,53 +316,72 @@ class Server:
         s.reqs = []
         s.state = "free"
 
-    def _progress(self, s: _Slot, now: float) -> bool:
+    def _grow_pairs(self, s: _Slot, total: int):
         torch = self.torch
+        cap = total + total // 4
+        s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
+        if self.results == "host":
+            s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
+
+    def _finish(self, s: _Slot, err: int) -> int:
+        """Completion of a launch whose result word asks for host work, run in
+        a worker thread on the slot's own context and stream: the plain path's
+        synchronous call (a dense request straight on the general path), or
+        the batch's finish from its launch (re-binning of dense bins,
+        workspace regrowth), then a rerun if the pairs did not fit."""
+        torch = self.torch
+        if s.mode == "plain":
+            r = s.reqs[0]
+            view = self.d_pool[r._off:r._off + r.n]
+            if err & 16:
+                s.eng.configure(pbig_amd.PATH_GENERAL, int(self.table["bin_target"]))
+            total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream, allow_overflow=True)
+            if total > s.pairs.shape[0]:
+                self._grow_pairs(s, total)
+                total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
+            s.h_io[0, 0] = 0
+            s.h_io[0, 1] = total
+            return total
+        total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream, finish=True)
+        if total > s.pairs.shape[0]:
+            self._grow_pairs(s, total)
+            total, io = s.batch.check(self.d_pool, s.pairs, stream=s.stream)
+        s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
+        return total
+
+    def _results_ready(self, s: _Slot, total: int, now: float) -> bool:
+        torch = self.torch
+        s.total = total
+        if self.results == "device":
+            self._deliver(s, s.pairs, now)
+            return True
+        if total:
+            with torch.cuda.stream(s.stream):
+                s.h_pairs[:total].copy_(s.pairs[:total], non_blocking=True)
+        s.ev = torch.cuda.Event()
+        s.ev.record(s.stream)
+        s.state = "copying"
+        return True
+
+    def _progress(self, s: _Slot, now: float) -> bool:
         if s.state == "running" and s.ev.query():
             total, err = (int(v) for v in s.h_res.tolist())
             for r in s.reqs:
                 r.t_ready = now
+            if err != 0 or total > s.pairs.shape[0]:
+                s.future = self.pool_exec.submit(self._finish, s, err)
+                s.state = "finishing"
+                return True
             if s.mode == "plain":
-                r = s.reqs[0]
-                if err != 0 or total > s.pairs.shape[0]:
-                    # dense (general path), bins beyond LDS, workspace or
-                    # pair capacity: the synchronous call resolves it
-                    view = self.d_pool[r._off:r._off + r.n]
-                    if err & 16:  # dense: straight to the general path
-                        s.eng.configure(pbig_amd.PATH_GENERAL, int(self.table["bin_target"]))
-                    total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream,
-                                               allow_overflow=True)
-                    if total > s.pairs.shape[0]:
-                        cap = total + total // 4
-                        s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
-                        if self.results == "host":
-                            s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
-                        total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
                 s.h_io[0, 0] = 0
                 s.h_io[0, 1] = total
-            elif err != 0 or total > s.pairs.shape[0]:
-                # bins beyond LDS (finished from this launch's results),
-                # workspace regrowth or too many pairs (run again)
-                total, io = s.batch.check(self.d_pool, s.pairs, allow_overflow=True, stream=s.stream,
-                                          finish=True)
-                if total > s.pairs.shape[0]:
-                    cap = total + total // 4
-                    s.pairs = torch.empty((cap, 2), dtype=torch.int32, device=self.dev)
-                    if self.results == "host":
-                        s.h_pairs = torch.empty((cap, 2), dtype=torch.int32, pin_memory=True)
-                    total, io = s.batch.check(self.d_pool, s.pairs, stream=s.stream)
-                s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
-            s.total = total
-            if self.results == "device":
-                self._deliver(s, s.pairs, now)
-                return True
-            if total:
-                with torch.cuda.stream(s.stream):
-                    s.h_pairs[:total].copy_(s.pairs[:total], non_blocking=True)
-            s.ev = torch.cuda.Event()
-            s.ev.record(s.stream)
-            s.state = "copying"
-            return True
+            return self._results_ready(s, total, now)
+        if s.state == "finishing" and s.future.done():
+            total = s.future.result()  # re-raises the worker's error
+            s.future = None
+            for r in s.reqs:
+                r.t_ready = now
+            return self._results_ready(s, total, now)
         if s.state == "copying" and s.ev.query():
             self._deliver(s, s.h_pairs.numpy().view(np.uint32), now)
             return True
@@ -388,10 +417,13 @@ class Server:
                     order[k].arrival = now
                 self.submit(order[k])
                 k += 1
-            if not self.poll(time.perf_counter() - t0) and k < len(order) and realtime and not self.busy():
+            progressed = self.poll(time.perf_counter() - t0)
+            if not progressed and k < len(order) and realtime and not self.busy():
                 wait = order[k].arrival - (time.perf_counter() - t0)
                 if wait > 2e-4:
                     time.sleep(min(wait - 1e-4, 0.002))
+            elif not progressed and any(s.state == "finishing" for s in self.slots):
+                time.sleep(5e-5)  # let a worker thread take the interpreter lock
         t_end = time.perf_counter() - t0
         return summarize(order, t_end, self.batch_sizes)
 
