@@ -39,41 +39,65 @@ def alg_bytes(N, C, M, P):
 
 
 def cpu_baseline(boxes, L, cs, n_pairs):
-    """The oracle (CPU restatement of the reference's grid algorithm) timed on
-    the host cores over the full instance, median of 3 after one warm-up; and
-    the reference checker's brute-force predicate on the first 65,536 boxes."""
+    """CPU baselines on the host cores (about 20 s in all):
+    * value: the oracle (CPU restatement of the reference's grid algorithm)
+      over the full 4M instance on all host threads, median of 5 after one
+      discarded warm-up; also once on one thread, and on the clustered 2M
+      instance (median of 3);
+    * the reference checker's brute-force predicate (src/pbig/pbig.cu:996-1001)
+      on the 16,384-box instance in full (one thread and all threads, median of
+      5) and on the first 2^18 boxes of the 4M instance (all threads, one run).
+    """
     import oracle
+    import pbig_amd
+    from pbig_amd import workloads
 
     oracle.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = min(threads, 16)
-    oracle.grid_pairs(boxes, L, cs, threads=threads)  # warm-up, discarded
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        got = oracle.grid_pairs(boxes, L, cs, threads=threads)
-        times.append(time.perf_counter() - t0)
-        assert len(got) == n_pairs
-    t = statistics.median(times)
-    sample = 65536
-    t0 = time.perf_counter()
-    bf_pairs = oracle.brute_count(boxes[:sample], threads=threads)
-    t_bf = time.perf_counter() - t0
-    tests = sample * (sample - 1) / 2
+
+    def timed(fn, reps, warm=True):
+        if warm:
+            fn()
+        ts, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts), out
+
+    t, got = timed(lambda: oracle.grid_pairs(boxes, L, cs, threads=threads), 5)
+    assert len(got) == n_pairs
+    t1, _ = timed(lambda: oracle.grid_pairs(boxes, L, cs, threads=1), 1, warm=False)
+    b3, L3 = workloads.c3_clustered()
+    cs3 = pbig_amd.reference_cell_size(b3, L3)
+    t3, got3 = timed(lambda: oracle.grid_pairs(b3, L3, cs3, threads=threads), 3)
+    b1, _ = workloads.c1_small_uniform()
+    tests1 = b1.shape[0] * (b1.shape[0] - 1) / 2
+    tb1, n1 = timed(lambda: oracle.brute_count(b1, threads=1), 5)
+    tbn, _ = timed(lambda: oracle.brute_count(b1, threads=threads), 5)
+    sample = 1 << 18
+    tests2 = sample * (sample - 1) / 2
+    tb2, n2 = timed(lambda: oracle.brute_count(boxes[:sample], threads=threads), 1, warm=False)
     return {
         "value": n_pairs / t,
         "unit": "overlaps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"full 4,194,304-box instance through the oracle's grid restatement at cell size {cs} "
-                  f"(median of 3 after 1 warm-up, {t:.3f} s per run)",
+                  f"(median of 5 after 1 warm-up, {t:.3f} s per run, {threads} threads)",
         "seconds_per_instance": t,
+        "seconds_per_instance_one_thread": t1,
+        "clustered_2m": {"seconds": t3, "pairs": len(got3), "cell_size": cs3, "overlaps_per_s": len(got3) / t3,
+                         "threads": threads},
         "bruteforce_checker": {
-            "sample": f"first {sample} boxes, all {int(tests)} pairs, predicate of src/pbig/pbig.cu:996-1001",
-            "seconds": t_bf,
-            "tests_per_s": tests / t_bf,
-            "pairs_in_sample": bf_pairs,
-            "extrapolated_full_instance_seconds": t_bf * (boxes.shape[0] / sample) ** 2,
+            "predicate": "src/pbig/pbig.cu:996-1001, all pairs",
+            "instance_16k": {"pairs": n1, "tests": int(tests1), "seconds_one_thread": tb1,
+                             "tests_per_s_one_thread": tests1 / tb1, "seconds_all_threads": tbn,
+                             "tests_per_s_all_threads": tests1 / tbn},
+            "first_2pow18_of_4m": {"pairs": n2, "tests": int(tests2), "seconds_all_threads": tb2,
+                                   "tests_per_s_all_threads": tests2 / tb2,
+                                   "extrapolated_full_instance_seconds": tb2 * (boxes.shape[0] / sample) ** 2},
         },
         "host_cpu": _cpu_model(),
     }
