@@ -13,18 +13,24 @@
 //   k_bin_hist     each workgroup owns a chunk of one instance's boxes and
 //                  counts its coarse-bin memberships in an LDS histogram,
 //                  written as one row of that instance's hist[chunk][bin]
-//   k_bin_scan     per bin, exclusive scan over the instance's chunks
+//   k_bin_scan     per bin, exclusive scan over the instance's chunks (the
+//                  table read once, in whole lines)
 //   k_bin_starts   exclusive scan of all bins' totals (bucket offsets)
-//   k_bin_scatter  the chunk is held in registers and emitted bin range by
-//                  bin range through an LDS stage, so stores cover whole runs
-//                  and every XCD writes the same bins at the same time
-//   k_bin_test     persistent workgroups over all bins: fine-cell count, scan
-//                  and scatter in LDS (in strips that fit), all-pairs test of
-//                  every fine cell with the reference-cell rule
+//   k_bin_scatter  the chunk is held in registers; every membership is
+//                  ranked among the chunk's memberships in bin order and
+//                  staged in LDS, so stores cover whole runs; each XCD walks
+//                  one contiguous range of chunks, so neighbouring runs merge
+//                  in its L2
+//   k_bin_test     persistent workgroups over all bins: fine-cell count,
+//                  layout and scatter in LDS (in strips that fit), all-pairs
+//                  test of every fine cell with the reference-cell rule
 //                  (pbig.cu:562-603); pairs appended through an LDS counter
-//                  to the workgroup's own region
-//   k_bin_compact  concatenates the regions
-//
+//                  to the workgroup's own region, which a single instance's
+//                  workgroup then copies to its share of the output
+//   k_bin_pair_scan, k_bin_compact   (batches) every bin's output position,
+//                  then the regions' runs placed in bin order, so that each
+//                  instance's pairs are contiguous
+
 // Cell maps.  The fine cell of a coordinate is clamp(floor(v * inv_f), 0,
 // nf-1) with nf = nbx * k fine cells per row and inv_f rounded once on the
 // host; the coarse bin is the fine index / k.  Both maps are monotone, so the
