@@ -75,13 +75,23 @@ __device__ __forceinline__ uint32_t region_of(uint32_t f, uint32_t f0, uint32_t 
   return (f < f0 ? f0 : (f > hi ? hi : f)) - f0;
 }
 
-__device__ __forceinline__ BinRange bin_range(const float4& b, const InstDesc& p) {
+// Coarse bin range from the fine cells of a box's corners (f: x0, x1, y0, y1).
+__device__ __forceinline__ BinRange bin_range_of(const uint4& f, const InstDesc& p) {
   BinRange r;
-  r.x0 = div_k(region_of(fine_of(b.x, p.inv_f, p.nf), p.fx0, p.span_x), p.k_magic);
-  r.x1 = div_k(region_of(fine_of(b.y, p.inv_f, p.nf), p.fx0, p.span_x), p.k_magic);
-  r.y0 = div_k(region_of(fine_of(b.z, p.inv_f, p.nf), p.fy0, p.span_y), p.k_magic);
-  r.y1 = div_k(region_of(fine_of(b.w, p.inv_f, p.nf), p.fy0, p.span_y), p.k_magic);
+  r.x0 = div_k(region_of(f.x, p.fx0, p.span_x), p.k_magic);
+  r.x1 = div_k(region_of(f.y, p.fx0, p.span_x), p.k_magic);
+  r.y0 = div_k(region_of(f.z, p.fy0, p.span_y), p.k_magic);
+  r.y1 = div_k(region_of(f.w, p.fy0, p.span_y), p.k_magic);
   return r;
+}
+
+__device__ __forceinline__ uint4 fine_corners(const float4& b, const InstDesc& p) {
+  return make_uint4(fine_of(b.x, p.inv_f, p.nf), fine_of(b.y, p.inv_f, p.nf), fine_of(b.z, p.inv_f, p.nf),
+                    fine_of(b.w, p.inv_f, p.nf));
+}
+
+__device__ __forceinline__ BinRange bin_range(const float4& b, const InstDesc& p) {
+  return bin_range_of(fine_corners(b, p), p);
 }
 
 // Box range [start, end) of global chunk w.
@@ -119,10 +129,12 @@ __global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ bo
       if (i >= end) continue;
       const float4 b = q[k];
       if (!((b.x <= b.y) && (b.z <= b.w))) {
-        err = 1u;
+        atomicOr(&err, 1u);
         continue;
       }
-      const BinRange r = bin_range(b, I);
+      const uint4 f = fine_corners(b, I);
+      if (f.y - f.x > 1u || f.w - f.z > 1u) atomicOr(&err, 2u);  // spans more than two fine cells
+      const BinRange r = bin_range_of(f, I);
       for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
         for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * I.nbx + cx], 1u);
     }
@@ -215,9 +227,10 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
       uint32_t any = 0;
       for (int k = 0; k < 16; ++k) any |= es[k];
       meta->n_pairs = 0ull;
-      meta->error = any ? kErrInvalidBox : 0ull;
+      meta->error = (any & 1u) ? kErrInvalidBox : 0ull;
       meta->items = 0ull;
       meta->max_per_cell = 0u;
+      meta->wide = (any & 2u) ? 1u : 0u;
     }
   }
 }
@@ -505,7 +518,8 @@ struct TestLds {
 static_assert(sizeof(TestLds) <= 160u * 1024u, "one test workgroup per CU");
 
 // Appends a wave's staged hits to the workgroup's region as (smaller id, larger id).
-__device__ __forceinline__ void stage_flush(TestLds& L, uint32_t w, uint32_t& sn, uint2* __restrict__ region,
+template <typename Lds>
+__device__ __forceinline__ void stage_flush(Lds& L, uint32_t w, uint32_t& sn, uint2* __restrict__ region,
                                             uint32_t cap_wg) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t g = 0;
@@ -589,6 +603,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  if (!meta->wide) return;  // every box spans at most two fine cells per axis: k_bin_test_narrow's launch
   if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
@@ -901,6 +916,275 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   }
 }
 
+// ---------------------------------------------------------------------------
+// Test of a launch whose boxes all span at most two fine cells per axis.
+//
+// Then two overlapping boxes a, b have min-corner cells A, B at most one
+// apart per axis (their cell ranges, of length <= 2, intersect), and the
+// reference cell of the pair, R = (max(A.x, B.x), max(A.y, B.y)), is covered
+// by both boxes; so the bin holding R holds both.  Each coarse membership is
+// keyed once, by its min-corner cell in the bin's grid extended by one halo
+// column and row (x, y in [-1, k)); no fine ranges, no replication.  A box in
+// cell C = (x, y) tests
+//   * if C is inside the bin: the boxes after it in C, and all boxes of W =
+//     (x-1, y), SW = (x-1, y-1), S = (x, y-1)      (R = C);
+//   * if (x+1, y) is inside the bin: the boxes of SE = (x+1, y-1)
+//                                                   (R = (x+1, y)).
+// Case by case (B = A, W, SW, S or SE of A, and the mirror cases), every
+// overlapping pair with R in this bin is tested exactly once, and only
+// there: the overlap test alone decides, with no reference-cell flags.  SW,
+// S and SE are adjacent cells of one row, so a box's partners are three
+// contiguous position ranges.
+constexpr uint32_t kMaxExt = (kMaxK + 1) * (kMaxK + 1);  // extended cells of a bin
+constexpr uint32_t kNarrowCellShift = 12;                 // word: coarse index | extended cell << 12
+static_assert(kMaxCoarse <= (1u << kNarrowCellShift), "coarse index field");
+static_assert(kMaxExt < (1u << (32 - kNarrowCellShift)), "extended cell field");
+
+struct NarrowLds {
+  float4 box[kMaxCoarse];          // the bin's coarse memberships in cell order
+  uint32_t word[kMaxCoarse];       // coarse index | extended cell << kNarrowCellShift
+  uint32_t id[kMaxCoarse];         // box id of each coarse membership
+  uint32_t cnt[kMaxExt];           // counts, then scatter cursors
+  uint32_t start[kMaxExt + 1];     // first position of each extended cell
+  uint2 stage[kTestWaves][kBinStage];
+  uint2 binfo[kMaxBinsPerGroup];
+  uint4 bingrid[kMaxBinsPerGroup];
+  float bininv[kMaxBinsPerGroup];
+  uint32_t wsum[kTestWaves];
+  uint32_t out_n;
+  unsigned long long out_base;
+};
+static_assert(sizeof(NarrowLds) <= 160u * 1024u, "one test workgroup per CU");
+static_assert(kMaxExt <= 5u * kTestThreads, "the cell scan takes five cells per thread");
+
+__global__ __launch_bounds__(kTestThreads) void k_bin_test_narrow(const float4* __restrict__ bbox,
+                                                                  const uint32_t* __restrict__ bid,
+                                                                  const uint32_t* __restrict__ bin_start,
+                                                                  const uint32_t* __restrict__ total,
+                                                                  const InstDesc* __restrict__ inst,
+                                                                  const uint32_t* __restrict__ bin_inst,
+                                                                  uint32_t n_bins, uint64_t cap_bucket,
+                                                                  uint2* __restrict__ scratch, uint32_t cap_wg,
+                                                                  uint2* __restrict__ bin_seg,
+                                                                  uint32_t* __restrict__ overfull,
+                                                                  uint2* __restrict__ out, uint64_t cap,
+                                                                  uint32_t direct, Meta* __restrict__ meta) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  if (meta->wide) return;  // k_bin_test's launch
+  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
+    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
+    return;
+  }
+  uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
+  uint32_t sn = 0;
+  uint32_t seg0 = 0;
+  for (uint32_t c = tid; c < kMaxExt; c += kTestThreads) L.cnt[c] = 0u;  // zero at the start of every bin
+  if (tid == 0) L.out_n = 0u;
+
+  const uint64_t last = cap_bucket - 1;
+  float4 rb[kRegs];
+  uint32_t ri[kRegs];
+  for (uint32_t t = 0; t < my_bins; ++t) {
+    const uint32_t slot = t % kMaxBinsPerGroup;
+    if (slot == 0) {
+      __syncthreads();
+      for (uint32_t u = tid; u < kMaxBinsPerGroup && t + u < my_bins; u += kTestThreads) {
+        const uint32_t gb = blockIdx.x + (t + u) * gridDim.x;
+        const InstDesc I = inst[bin_inst[gb]];
+        const uint32_t lb = gb - I.bin0;
+        const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
+        const uint32_t ex = I.fx0 + I.span_x - gx, ey = I.fy0 + I.span_y - gy;
+        const uint32_t kx = I.k < ex ? I.k : ex, ky = I.k < ey ? I.k : ey;
+        L.binfo[u] = make_uint2(bin_start[gb], total[gb]);
+        L.bingrid[u] = make_uint4(gx, gy, kx | (ky << 16), I.nf);
+        L.bininv[u] = I.inv_f;
+      }
+      __syncthreads();
+      if (t == 0) {
+        const uint64_t s0 = L.binfo[0].x;
+        const uint32_t j_hi = L.binfo[0].y ? L.binfo[0].y - 1u : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kRegs; ++k) {
+          const uint32_t j = tid + k * kTestThreads < j_hi ? tid + k * kTestThreads : j_hi;
+          const uint64_t at = s0 + j < last ? s0 + j : last;
+          rb[k] = bbox[at];
+          ri[k] = bid[at];
+        }
+      }
+    }
+    const uint32_t b = blockIdx.x + t * gridDim.x;
+    const uint32_t m = L.binfo[slot].y;
+    const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
+    const uint32_t mN = t + 1 < my_bins ? total[b + gridDim.x] : 0u;
+    const uint32_t jN = mN ? mN - 1u : 0u;
+    const uint4 bg = L.bingrid[slot];
+    const int gx = (int)bg.x, gy = (int)bg.y;
+    const uint32_t Kx = bg.z & 0xFFFFu, Ky = bg.z >> 16, ex = Kx + 1u, ne = ex * (Ky + 1u);
+    const float inv_f = L.bininv[slot];
+    const uint32_t nf = bg.w;
+    bool prefetched = false;
+    if (m >= 2 && m <= kMaxCoarse) {
+      // A. each membership's extended cell; counts; ids to LDS
+      uint32_t ek[kRegs];
+#pragma unroll
+      for (uint32_t k = 0; k < kRegs; ++k) {
+        const uint32_t j = tid + k * kTestThreads;
+        ek[k] = 0u;
+        if (j < m) {
+          L.id[j] = ri[k];
+          int rx = (int)fine_of(rb[k].x, inv_f, nf) - gx, ry = (int)fine_of(rb[k].z, inv_f, nf) - gy;
+          rx = rx < -1 ? -1 : (rx > (int)Kx - 1 ? (int)Kx - 1 : rx);
+          ry = ry < -1 ? -1 : (ry > (int)Ky - 1 ? (int)Ky - 1 : ry);
+          ek[k] = (uint32_t)(rx + 1) + (uint32_t)(ry + 1) * ex;
+          atomicAdd(&L.cnt[ek[k]], 1u);
+        }
+      }
+      __syncthreads();
+      // B. exclusive scan of the counts (ne <= kMaxExt: 5 per thread)
+      {
+        uint32_t v[5], acc = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const uint32_t c = tid * 5 + k;
+          v[k] = c < ne ? L.cnt[c] : 0u;
+          acc += v[k];
+        }
+        uint32_t inc = acc;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const uint32_t o = __shfl_up(inc, d, 64);
+          if (lane >= (uint32_t)d) inc += o;
+        }
+        if (lane == 63) L.wsum[w] = inc;
+        __syncthreads();
+        uint32_t exs = inc - acc;
+        {
+          uint32_t ws = lane < w ? L.wsum[lane] : 0u;
+#pragma unroll
+          for (int d = 1; d < 16; d <<= 1) ws += __shfl_xor(ws, d, 64);
+          exs += __builtin_amdgcn_readfirstlane(ws);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const uint32_t c = tid * 5 + k;
+          if (c < ne) {
+            L.start[c] = exs;
+            L.cnt[c] = exs;  // scatter cursor
+          }
+          exs += v[k];
+        }
+        if (tid == kTestThreads - 1) L.start[ne] = exs;
+      }
+      __syncthreads();
+      // C. one position per membership, in cell order
+#pragma unroll
+      for (uint32_t k = 0; k < kRegs; ++k) {
+        const uint32_t j = tid + k * kTestThreads;
+        if (j < m) {
+          const uint32_t pos = atomicAdd(&L.cnt[ek[k]], 1u);
+          L.box[pos] = rb[k];
+          L.word[pos] = j | (ek[k] << kNarrowCellShift);
+        }
+      }
+      __syncthreads();
+      for (uint32_t c = tid; c < ne; c += kTestThreads) L.cnt[c] = 0u;  // for the next bin
+      if (t + 1 < my_bins) {
+#pragma unroll
+        for (uint32_t k = 0; k < kRegs; ++k) {
+          const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
+          const uint64_t at = sN + j < last ? sN + j : last;
+          rb[k] = bbox[at];
+          ri[k] = bid[at];
+        }
+        prefetched = true;
+      }
+      // D. one position per lane: partners in W, after me in my cell, and
+      //    SW..S(..SE) of the row below
+      for (uint32_t p0 = w * 64; p0 < m; p0 += kTestThreads) {
+        const uint32_t p = p0 + lane;
+        uint32_t b1 = 0, n1 = 0, b2 = 0, n2 = 0, b3 = 0, n3 = 0, aw = 0;
+        float4 a = make_float4(0.f, -1.f, 0.f, -1.f);
+        if (p < m) {
+          aw = L.word[p];
+          a = L.box[p];
+          const uint32_t e = aw >> kNarrowCellShift;
+          const uint32_t X = e % ex, Y = e / ex;
+          const bool inside = X >= 1u && Y >= 1u;
+          const bool se = Y >= 1u && X + 1u <= Kx;
+          if (inside) {
+            b1 = L.start[e - 1u];                  // W
+            n1 = L.start[e] - b1;
+            b2 = p + 1u;                            // my cell, after me
+            n2 = L.start[e + 1u] - b2;
+            b3 = L.start[e - ex - 1u];              // SW, S (and SE)
+            n3 = L.start[e - ex + (se ? 2u : 1u)] - b3;
+          } else if (se) {
+            b3 = L.start[e - ex + 1u];              // SE only
+            n3 = L.start[e - ex + 2u] - b3;
+          }
+        }
+        const uint32_t ja = aw & ((1u << kNarrowCellShift) - 1u);
+        const uint32_t n12 = n1 + n2, nall = n12 + n3;
+        for (uint32_t k = 0; __ballot(k < nall); ++k) {
+          const bool act = k < nall;
+          uint32_t qq = k < n1 ? b1 + k : (k < n12 ? b2 + (k - n1) : b3 + (k - n12));
+          qq = act ? qq : 0u;
+          const float4 o = L.box[qq];
+          const uint32_t ow = L.word[qq];
+          const uint64_t mask = __ballot(act) & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) &
+                                __ballot(a.z <= o.w) & __ballot(o.z <= a.w);
+          const bool hit = (mask >> lane) & 1ull;
+          if (mask) {
+            const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            if (hit) L.stage[w][pos] = make_uint2(ja, ow & ((1u << kNarrowCellShift) - 1u));
+            sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
+            if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
+          }
+        }
+      }
+      if (sn) stage_flush(L, w, sn, region, cap_wg);
+      __syncthreads();  // the positions are reused by the next bin
+    } else if (m > kMaxCoarse) {
+      if (tid == 0) {
+        overfull[atomicAdd(&meta->max_per_cell, 1u)] = b;
+        atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrOverfull);
+      }
+    }
+    if (tid == 0) {
+      const uint32_t e = L.out_n;
+      bin_seg[b] = make_uint2(seg0, e - seg0);
+      seg0 = e;
+    }
+    if (!prefetched && t + 1 < my_bins) {
+      __syncthreads();
+#pragma unroll
+      for (uint32_t k = 0; k < kRegs; ++k) {
+        const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
+        const uint64_t at = sN + j < last ? sN + j : last;
+        rb[k] = bbox[at];
+        ri[k] = bid[at];
+      }
+    }
+  }
+  if (tid == 0 && seg0 > cap_wg) {
+    atomicMax((unsigned long long*)&meta->items, (unsigned long long)seg0);
+    atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
+  }
+  if (direct) {
+    __syncthreads();
+    const uint32_t n = L.out_n < cap_wg ? L.out_n : cap_wg;
+    if (tid == 0) L.out_base = L.out_n ? atomicAdd((unsigned long long*)&meta->n_pairs, (unsigned long long)L.out_n) : 0ull;
+    __syncthreads();
+    const uint64_t base = L.out_base;
+    for (uint32_t k = tid; k < n; k += kTestThreads)
+      if (base + k < cap) out[base + k] = region[k];
+  }
+}
+
 // Output position of every bin's pairs: exclusive scan of the bins' pair
 // counts by one 1024-thread workgroup (bin_out[n_bins] = the batch's total).
 // Pairs are laid out in bin order, so each instance's pairs are one run.
@@ -975,6 +1259,9 @@ hipError_t binned_init() {
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(TestLds));
   if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_bin_test_narrow, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)sizeof(NarrowLds));
+  if (e != hipSuccess) return e;
   const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
   e = hipFuncSetAttribute((const void*)k_bin_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           scatter_max);
@@ -1026,6 +1313,11 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   PBIG_MARK(4)
+  // Exactly one of the two test kernels works (the histogram's wide flag):
+  // the other returns at once.
+  k_bin_test_narrow<<<bp.test_blocks, kTestThreads, sizeof(NarrowLds), s>>>(
+      ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket, ws.scratch, ws.cap_wg,
+      ws.bin_seg, ws.overfull, out, cap, bp.direct, ws.meta);
   k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,

@@ -33,7 +33,8 @@ struct Meta {
   unsigned long long memberships;
   unsigned long long items;
   unsigned int max_per_cell;
-  unsigned int pad[7];
+  unsigned int wide;  // binned path: some box spans more than two fine cells on an axis
+  unsigned int pad[6];
 };
 
 struct ScanPartial {
