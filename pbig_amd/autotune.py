@@ -26,15 +26,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_PATH = os.path.join(HERE, "tuning.json")
 
-# Used when no tuned table exists.  q = 4 is the cell side that measured best
-# on the 4,194,304-box uniform instance (cell 64 at rho = 2^-10).
+# Used when no tuned table exists.  q = 1 is the cell side that measured best
+# on the 4,194,304-box uniform instance (cell 32 at rho = 2^-10: boxes of side
+# up to 32 span at most two cells per axis, so the narrow-box test kernel runs).
 DEFAULT_TABLE = {
     "version": 1,
-    "bin_target": 2000,
+    "bin_target": 3000,
     "classes": [
-        {"max_n": 4096, "boxes_per_cell": 4.0},
-        {"max_n": 65536, "boxes_per_cell": 4.0},
-        {"max_n": None, "boxes_per_cell": 4.0},
+        {"max_n": 4096, "boxes_per_cell": 1.0},
+        {"max_n": 65536, "boxes_per_cell": 1.0},
+        {"max_n": None, "boxes_per_cell": 1.0},
     ],
     "source": "default",
 }
@@ -118,7 +119,7 @@ def time_batch(engine, d_boxes, starts, counts, ranges, cells, reps: int = 5) ->
         return s.elapsed_time(e) / reps
 
 
-def tune(engine, boxes, offsets, ranges, qs=(1.0, 2.0, 4.0, 8.0, 16.0), bin_targets=(1000, 2000, 3000),
+def tune(engine, boxes, offsets, ranges, qs=(0.5, 1.0, 2.0, 4.0, 8.0), bin_targets=(1000, 2000, 3000, 3500),
          reps: int = 5, base: dict | None = None, log=print) -> dict:
     """Pick boxes-per-cell per size class, then the bin target, by timing
     graph replays of the workload's instances on the GPU."""
