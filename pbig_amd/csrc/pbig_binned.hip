@@ -1128,10 +1128,14 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test_narrow(const float4* 
         }
         const uint32_t ja = aw & ((1u << kNarrowCellShift) - 1u);
         const uint32_t n12 = n1 + n2, nall = n12 + n3;
+        // partner k is position k + off(k), off the offset of its range (selects, no branches)
+        const uint32_t o1 = b1, o2 = b2 - n1, o3 = b3 - n12;
         for (uint32_t k = 0; __ballot(k < nall); ++k) {
           const bool act = k < nall;
-          uint32_t qq = k < n1 ? b1 + k : (k < n12 ? b2 + (k - n1) : b3 + (k - n12));
-          qq = act ? qq : 0u;
+          uint32_t off = o3;
+          off = k < n12 ? o2 : off;
+          off = k < n1 ? o1 : off;
+          const uint32_t qq = act ? k + off : 0u;
           const float4 o = L.box[qq];
           const uint32_t ow = L.word[qq];
           const uint64_t mask = __ballot(act) & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) &
