@@ -382,3 +382,27 @@ def test_binned_deep_rebinning(torch_cuda, binned, orc):
     got = run_device(torch_cuda, binned, boxes, 8192.0, 4.0)
     assert binned.stats()["path"] == PATHS["binned"]
     assert np.array_equal(got, orc.sort_pairs(orc.sweep_pairs(boxes)))
+
+
+@pytest.mark.parametrize("bin_target", [200, 3500])
+def test_narrow_kernel_on_a_touching_lattice(torch_cuda, binned, orc, bin_target):
+    """Boxes exactly one cell wide on a half-cell lattice: every box spans two
+    cells per axis, neighbours share edges and corners (closed intervals), and
+    many pairs have their reference cell on a bin edge or in the halo.  The
+    narrow-box test kernel runs; then one box two cells wide is added, which
+    sends the same launch to the general test kernel.  Both must give the
+    brute-force set."""
+    cs = 8.0
+    g = np.arange(0, 120, dtype=np.float32) * (cs / 2)
+    x, y = np.meshgrid(g, g, indexing="ij")
+    x, y = x.ravel(), y.ravel()
+    boxes = np.stack([x, x + cs, y, y + cs], 1).astype(np.float32)
+    L = float(g[-1] + cs)
+    binned.configure(PATHS["binned"], bin_target)
+    got = run_device(torch_cuda, binned, boxes, L, cs)
+    want = orc.sort_pairs(orc.brute_pairs(boxes))
+    assert np.array_equal(got, want), (len(got), len(want))
+    wide = np.concatenate([boxes, np.array([[100.0, 100.0 + 2 * cs, 100.0, 101.0]], np.float32)])
+    got = run_device(torch_cuda, binned, wide, L, cs)
+    want = orc.sort_pairs(orc.brute_pairs(wide))
+    assert np.array_equal(got, want), (len(got), len(want))
