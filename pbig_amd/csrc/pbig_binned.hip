@@ -50,11 +50,11 @@
 
 namespace pbig {
 
+// Branch-free: fmaxf returns its other operand for a NaN, so a NaN lands in
+// cell 0; the clamped float is an integer in [0, nf-1] (nf < 2^24) and
+// converts exactly.
 __device__ __forceinline__ uint32_t fine_of(float v, float inv, uint32_t nf) {
-  float f = floorf(v * inv);
-  if (!(f >= 0.0f)) return 0u;
-  if (f >= (float)(nf - 1u)) return nf - 1u;
-  return (uint32_t)f;
+  return (uint32_t)fminf(fmaxf(floorf(v * inv), 0.0f), (float)(nf - 1u));
 }
 
 // Coarse bin range of a box: fine index / K (K fine cells per bin side).
@@ -62,17 +62,19 @@ struct BinRange {
   uint32_t x0, x1, y0, y1;
 };
 
-// fine / k as a multiply by ceil(2^32 / k): exact for fine < 2^21, k < 2^11.
+// fine / k as a multiply by magic = ceil(2^32 / k): exact for fine < 2^21,
+// k < 2^11.  The magic fits 32 bits for every k >= 2; k = 1 (magic = 2^32)
+// is the identity.
 __device__ __forceinline__ uint32_t div_k(uint32_t fine, uint64_t magic) {
-  return (uint32_t)(((uint64_t)fine * magic) >> 32);
+  const uint32_t q = __umulhi(fine, (uint32_t)magic);
+  return (magic >> 32) ? fine : q;
 }
 
 // Fine index relative to the instance's region, clamped into it (a no-op
 // for an owned instance: fx0 = 0, span = nf).  Bins wholly outside a
 // non-square region receive no memberships.
 __device__ __forceinline__ uint32_t region_of(uint32_t f, uint32_t f0, uint32_t span) {
-  const uint32_t hi = f0 + span - 1u;
-  return (f < f0 ? f0 : (f > hi ? hi : f)) - f0;
+  return min(max(f, f0), f0 + span - 1u) - f0;
 }
 
 // Coarse bin range from the fine cells of a box's corners (f: x0, x1, y0, y1).
@@ -90,8 +92,21 @@ __device__ __forceinline__ uint4 fine_corners(const float4& b, const InstDesc& p
                     fine_of(b.w, p.inv_f, p.nf));
 }
 
+// The region clamp folded into the float clamp: the region [f0, f0+span-1]
+// lies inside [0, nf-1], so clamping floor(v * inv) straight into it gives
+// region_of(fine_of(v)) for every v (a NaN lands on f0, as fine_of's 0 does).
+__device__ __forceinline__ uint32_t region_bin(float v, float inv, uint32_t f0, uint32_t span, uint64_t magic) {
+  const float c = fminf(fmaxf(floorf(v * inv), (float)f0), (float)(f0 + span - 1u));
+  return div_k((uint32_t)c - f0, magic);
+}
+
 __device__ __forceinline__ BinRange bin_range(const float4& b, const InstDesc& p) {
-  return bin_range_of(fine_corners(b, p), p);
+  BinRange r;
+  r.x0 = region_bin(b.x, p.inv_f, p.fx0, p.span_x, p.k_magic);
+  r.x1 = region_bin(b.y, p.inv_f, p.fx0, p.span_x, p.k_magic);
+  r.y0 = region_bin(b.z, p.inv_f, p.fy0, p.span_y, p.k_magic);
+  r.y1 = region_bin(b.w, p.inv_f, p.fy0, p.span_y, p.k_magic);
+  return r;
 }
 
 // Box range [start, end) of global chunk w.
