@@ -132,6 +132,10 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
                                 uint64_t* inst_out, hipStream_t s);
 
+// The pair count and error word of a call (meta's first two words) copied
+// to the caller's two device words by one small kernel.
+hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s);
+
 }  // namespace pbig
 
 #endif  // PBIG_BINNED_H_

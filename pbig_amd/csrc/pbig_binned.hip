@@ -21,12 +21,16 @@
 //                  staged in LDS, so stores cover whole runs; each XCD walks
 //                  one contiguous range of chunks, so neighbouring runs merge
 //                  in its L2
-//   k_bin_test     persistent workgroups over all bins: fine-cell count,
-//                  layout and scatter in LDS (in strips that fit), all-pairs
-//                  test of every fine cell with the reference-cell rule
-//                  (pbig.cu:562-603); pairs appended through an LDS counter
-//                  to the workgroup's own region, which a single instance's
-//                  workgroup then copies to its share of the output
+//   k_bin_test     persistent workgroups over all bins, one of two
+//                  schedules picked by the histogram's wide flag: when every
+//                  box spans at most two fine cells per axis, a half-stencil
+//                  over neighbouring min-corner cells; otherwise fine-cell
+//                  count, layout and scatter in LDS (in strips that fit) and
+//                  an all-pairs test of every fine cell with the
+//                  reference-cell rule (pbig.cu:562-603).  Pairs are appended
+//                  through an LDS counter to the workgroup's own region,
+//                  which a single instance's workgroup then copies to its
+//                  share of the output
 //   k_bin_pair_scan, k_bin_compact   (batches) every bin's output position,
 //                  then the regions' runs placed in bin order, so that each
 //                  instance's pairs are contiguous
@@ -603,7 +607,7 @@ __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint3
   }
 }
 
-__global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restrict__ bbox,
+__device__ __forceinline__ void test_general(unsigned char* lds_raw, const float4* __restrict__ bbox,
                                                            const uint32_t* __restrict__ bid,
                                                            const uint32_t* __restrict__ bin_start,
                                                            const uint32_t* __restrict__ total,
@@ -614,11 +618,9 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                                                            uint32_t* __restrict__ overfull,
                                                            uint2* __restrict__ out, uint64_t cap, uint32_t direct,
                                                            Meta* __restrict__ meta) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (!meta->wide) return;  // every box spans at most two fine cells per axis: k_bin_test_narrow's launch
   if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
@@ -972,7 +974,7 @@ struct NarrowLds {
 static_assert(sizeof(NarrowLds) <= 160u * 1024u, "one test workgroup per CU");
 static_assert(kMaxExt <= 5u * kTestThreads, "the cell scan takes five cells per thread");
 
-__global__ __launch_bounds__(kTestThreads) void k_bin_test_narrow(const float4* __restrict__ bbox,
+__device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4* __restrict__ bbox,
                                                                   const uint32_t* __restrict__ bid,
                                                                   const uint32_t* __restrict__ bin_start,
                                                                   const uint32_t* __restrict__ total,
@@ -984,11 +986,9 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test_narrow(const float4* 
                                                                   uint32_t* __restrict__ overfull,
                                                                   uint2* __restrict__ out, uint64_t cap,
                                                                   uint32_t direct, Meta* __restrict__ meta) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->wide) return;  // k_bin_test's launch
   if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
@@ -1204,6 +1204,40 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test_narrow(const float4* 
   }
 }
 
+// One launch for both schedules: the histogram's wide flag, uniform over the
+// launch, picks the narrow-box schedule or the general one, so no launch is
+// spent on a schedule that has nothing to do.
+constexpr size_t kTestLdsBytes = sizeof(TestLds) > sizeof(NarrowLds) ? sizeof(TestLds) : sizeof(NarrowLds);
+
+__global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restrict__ bbox,
+                                                           const uint32_t* __restrict__ bid,
+                                                           const uint32_t* __restrict__ bin_start,
+                                                           const uint32_t* __restrict__ total,
+                                                           const InstDesc* __restrict__ inst,
+                                                           const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
+                                                           uint64_t cap_bucket, uint2* __restrict__ scratch,
+                                                           uint32_t cap_wg, uint2* __restrict__ bin_seg,
+                                                           uint32_t* __restrict__ overfull,
+                                                           uint2* __restrict__ out, uint64_t cap, uint32_t direct,
+                                                           Meta* __restrict__ meta) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  if (meta->wide)
+    test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
+                 overfull, out, cap, direct, meta);
+  else
+    test_narrow(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
+                overfull, out, cap, direct, meta);
+}
+
+// The pair count and error word, copied to the caller's two result words
+// after the last kernel of a call.
+__global__ __launch_bounds__(64) void k_bin_result(const Meta* __restrict__ meta, uint64_t* __restrict__ result) {
+  if (threadIdx.x == 0) {
+    result[0] = meta->n_pairs;
+    result[1] = meta->error;
+  }
+}
+
 // Output position of every bin's pairs: exclusive scan of the bins' pair
 // counts by one 1024-thread workgroup (bin_out[n_bins] = the batch's total).
 // Pairs are laid out in bin order, so each instance's pairs are one run.
@@ -1272,14 +1306,11 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 }
 
 // ---------------------------------------------------------------------------
-size_t binned_test_lds_bytes() { return sizeof(TestLds); }
+size_t binned_test_lds_bytes() { return kTestLdsBytes; }
 
 hipError_t binned_init() {
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)sizeof(TestLds));
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_bin_test_narrow, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)sizeof(NarrowLds));
+                                     (int)kTestLdsBytes);
   if (e != hipSuccess) return e;
   const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
   e = hipFuncSetAttribute((const void*)k_bin_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1287,6 +1318,11 @@ hipError_t binned_init() {
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_bin_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              scatter_max);
+}
+
+hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s) {
+  k_bin_result<<<1, 64, 0, s>>>(meta, result);
+  return hipGetLastError();
 }
 
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
@@ -1332,12 +1368,7 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.meta);
   PBIG_MARK(4)
-  // Exactly one of the two test kernels works (the histogram's wide flag):
-  // the other returns at once.
-  k_bin_test_narrow<<<bp.test_blocks, kTestThreads, sizeof(NarrowLds), s>>>(
-      ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket, ws.scratch, ws.cap_wg,
-      ws.bin_seg, ws.overfull, out, cap, bp.direct, ws.meta);
-  k_bin_test<<<bp.test_blocks, kTestThreads, sizeof(TestLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
+  k_bin_test<<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
                                                                    ws.overfull, out, cap, bp.direct, ws.meta);

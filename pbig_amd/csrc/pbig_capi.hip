@@ -1031,7 +1031,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
                                   next_events(c, PBIG_PATH_GENERAL)));
   }
   if (d_result)
-    HIP_TRY(c, hipMemcpyAsync(d_result, c->ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, pbig::launch_result(c->ws.meta, d_result, s));
   return PBIG_OK;
 }
 
@@ -1317,7 +1317,7 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
   if (d_result)
-    HIP_TRY(c, hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, pbig::launch_result(b->st.ws.meta, d_result, s));
   return PBIG_OK;
 }
 
@@ -1337,7 +1337,7 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
   hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap,
                                      d_inst_pairs, s, nullptr);
   if (e == hipSuccess && d_result)
-    e = hipMemcpyAsync(d_result, b->st.ws.meta, 2 * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+    e = pbig::launch_result(b->st.ws.meta, d_result, s);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
