@@ -342,6 +342,7 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
 constexpr uint32_t kScatterThreads = 1024;
 constexpr uint32_t kScatterPer = kScatterChunk / kScatterThreads;
 constexpr uint32_t kStageEntries = kScatterChunk + kScatterChunk / 8;  // 1.125 memberships per box
+constexpr uint32_t kScatterRowsPer = (kMaxBins + kScatterThreads - 1) / kScatterThreads;  // bins per thread, at most
 
 // Fixed part of the scatter's LDS; after it, sized per launch to the
 // batch's largest instance (nb_cap bins), come
@@ -378,8 +379,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   // contiguous range of chunks (x * per .. x * per + per - 1) together, so
   // the runs that chunks c and c + 1 write side by side in every bin's
   // bucket come from the same XCD at about the same time and merge in its
-  // L2.  The next chunk's boxes are loaded while the current chunk's staged
-  // memberships are written out.
+  // L2.  The next chunk's boxes and table rows are loaded while the current
+  // chunk's staged memberships are written out.
   if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
   if (meta->error & kErrDense) return;  // the host runs the general path
   const uint32_t xper = (n_chunks + 7u) / 8u;
@@ -388,6 +389,9 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   uint32_t wc = (blockIdx.x & 7u) * xper + r;
   const uint32_t wend = min((blockIdx.x & 7u) * xper + xper, n_chunks);
   float4 q[kScatterPer];
+  // The chunk's table rows for this thread's bins b0 .. b0 + per - 1 (count,
+  // offset over the earlier chunks, bucket start), loaded with its boxes.
+  uint32_t rh[kScatterRowsPer], ro[kScatterRowsPer], rs[kScatterRowsPer];
   auto load_chunk = [&](uint32_t c) {
     const InstDesc J = inst[chunk_inst[c]];
     uint64_t st, en;
@@ -396,6 +400,18 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     for (uint32_t k = 0; k < kScatterPer; ++k) {
       const uint64_t i = st + tid + k * kScatterThreads;
       q[k] = boxes[i < en ? i : en - 1];
+    }
+    const uint32_t nbj = J.nb, per = (nbj + kScatterThreads - 1) / kScatterThreads, b0 = tid * per;
+    const uint64_t lc = c - J.chunk0;
+    const uint32_t* jh = hist + J.hist0 + lc * hist_pitch(nbj);
+    const uint32_t* jo = off + J.off0 + lc * off_pitch(nbj);
+    const uint32_t* js = bin_start + J.bin0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterRowsPer; ++k) {
+      const bool in = k < per && b0 + k < nbj;
+      rh[k] = in ? jh[b0 + k] : 0u;
+      ro[k] = in ? jo[b0 + k] : 0u;
+      rs[k] = in ? js[b0 + k] : 0u;
     }
   };
   if (wc < wend) load_chunk(wc);
@@ -406,18 +422,14 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   chunk_range(I, wc, &start, &end);
   // 2. per bin: destination cursor, and the offset from a destination to the
   //    membership's rank among this chunk's memberships in bin order
-  const uint64_t lc = wc - I.chunk0;
-  const uint32_t* myoff = off + I.off0 + lc * off_pitch(nb);
-  const uint32_t* myhist = hist + I.hist0 + lc * hist_pitch(nb);
-  const uint32_t* mystart = bin_start + I.bin0;
   uint32_t chunk_total = 0;
   __syncthreads();  // the previous chunk's stage and cursors are done with
   {
     const uint32_t per = (nb + kScatterThreads - 1) / kScatterThreads;
     const uint32_t b0 = tid * per;
     uint32_t acc = 0;
-    for (uint32_t k = 0; k < per; ++k)
-      if (b0 + k < nb) acc += myhist[b0 + k];
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterRowsPer; ++k) acc += rh[k];  // zero past this thread's bins
     uint32_t v = acc;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -431,13 +443,14 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       if (k < w) ex += L.wsum[k];
       chunk_total += L.wsum[k];
     }
-    for (uint32_t k = 0; k < per; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < kScatterRowsPer; ++k) {
       const uint32_t b = b0 + k;
-      if (b < nb) {
-        const uint32_t dst0 = mystart[b] + myoff[b];
+      if (k < per && b < nb) {
+        const uint32_t dst0 = rs[k] + ro[k];
         cur[b] = dst0;
         woff[b] = (int32_t)(ex - dst0);
-        ex += myhist[b];
+        ex += rh[k];
       }
     }
   }
@@ -943,7 +956,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
 // keyed once, by its min-corner cell in the bin's grid extended by one halo
 // column and row (x, y in [-1, k)); no fine ranges, no replication.  A box in
 // cell C = (x, y) tests
-//   * if C is inside the bin: the boxes after it in C, and all boxes of W =
+//   * if C is inside the bin: the boxes before it in C, and all boxes of W =
 //     (x-1, y), SW = (x-1, y-1), S = (x, y-1)      (R = C);
 //   * if (x+1, y) is inside the bin: the boxes of SE = (x+1, y-1)
 //                                                   (R = (x+1, y)).
@@ -951,16 +964,14 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
 // overlapping pair with R in this bin is tested exactly once, and only
 // there: the overlap test alone decides, with no reference-cell flags.  SW,
 // S and SE are adjacent cells of one row, so a box's partners are three
-// contiguous position ranges.
+// contiguous position ranges.  The partner loop reads only the partner's
+// box; a hit stages the two positions, whose ids the flush looks up.
 constexpr uint32_t kMaxExt = (kMaxK + 1) * (kMaxK + 1);  // extended cells of a bin
-constexpr uint32_t kNarrowCellShift = 12;                 // word: coarse index | extended cell << 12
-static_assert(kMaxCoarse <= (1u << kNarrowCellShift), "coarse index field");
-static_assert(kMaxExt < (1u << (32 - kNarrowCellShift)), "extended cell field");
 
 struct NarrowLds {
   float4 box[kMaxCoarse];          // the bin's coarse memberships in cell order
-  uint32_t word[kMaxCoarse];       // coarse index | extended cell << kNarrowCellShift
-  uint32_t id[kMaxCoarse];         // box id of each coarse membership
+  uint32_t cell[kMaxCoarse];       // extended cell of each position
+  uint32_t id[kMaxCoarse];         // box id of each position
   uint32_t cnt[kMaxExt];           // counts, then scatter cursors
   uint32_t start[kMaxExt + 1];     // first position of each extended cell
   uint2 stage[kTestWaves][kBinStage];
@@ -1042,14 +1053,13 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     const uint32_t nf = bg.w;
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
-      // A. each membership's extended cell; counts; ids to LDS
+      // A. each membership's extended cell; counts
       uint32_t ek[kRegs];
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         ek[k] = 0u;
         if (j < m) {
-          L.id[j] = ri[k];
           int rx = (int)fine_of(rb[k].x, inv_f, nf) - gx, ry = (int)fine_of(rb[k].z, inv_f, nf) - gy;
           rx = rx < -1 ? -1 : (rx > (int)Kx - 1 ? (int)Kx - 1 : rx);
           ry = ry < -1 ? -1 : (ry > (int)Ky - 1 ? (int)Ky - 1 : ry);
@@ -1094,14 +1104,15 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         if (tid == kTestThreads - 1) L.start[ne] = exs;
       }
       __syncthreads();
-      // C. one position per membership, in cell order
+      // C. one position per membership, in cell order: box, cell and id
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         if (j < m) {
           const uint32_t pos = atomicAdd(&L.cnt[ek[k]], 1u);
           L.box[pos] = rb[k];
-          L.word[pos] = j | (ek[k] << kNarrowCellShift);
+          L.cell[pos] = ek[k];
+          L.id[pos] = ri[k];
         }
       }
       __syncthreads();
@@ -1116,24 +1127,23 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         }
         prefetched = true;
       }
-      // D. one position per lane: partners in W, after me in my cell, and
-      //    SW..S(..SE) of the row below
+      // D. one position per lane: partners in W and before me in my cell
+      //    (W's positions end where my cell's begin: one range), then
+      //    SW..S(..SE) of the row below.  Each pair inside a cell is tested
+      //    once, by its later member.
       for (uint32_t p0 = w * 64; p0 < m; p0 += kTestThreads) {
         const uint32_t p = p0 + lane;
-        uint32_t b1 = 0, n1 = 0, b2 = 0, n2 = 0, b3 = 0, n3 = 0, aw = 0;
+        uint32_t b1 = 0, n1 = 0, b3 = 0, n3 = 0;
         float4 a = make_float4(0.f, -1.f, 0.f, -1.f);
         if (p < m) {
-          aw = L.word[p];
           a = L.box[p];
-          const uint32_t e = aw >> kNarrowCellShift;
+          const uint32_t e = L.cell[p];
           const uint32_t X = e % ex, Y = e / ex;
           const bool inside = X >= 1u && Y >= 1u;
           const bool se = Y >= 1u && X + 1u <= Kx;
           if (inside) {
-            b1 = L.start[e - 1u];                  // W
-            n1 = L.start[e] - b1;
-            b2 = p + 1u;                            // my cell, after me
-            n2 = L.start[e + 1u] - b2;
+            b1 = L.start[e - 1u];                  // W, then my cell up to me
+            n1 = p - b1;
             b3 = L.start[e - ex - 1u];              // SW, S (and SE)
             n3 = L.start[e - ex + (se ? 2u : 1u)] - b3;
           } else if (se) {
@@ -1141,25 +1151,22 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
             n3 = L.start[e - ex + 2u] - b3;
           }
         }
-        const uint32_t ja = aw & ((1u << kNarrowCellShift) - 1u);
-        const uint32_t n12 = n1 + n2, nall = n12 + n3;
-        // partner k is position k + off(k), off the offset of its range (selects, no branches)
-        const uint32_t o1 = b1, o2 = b2 - n1, o3 = b3 - n12;
-        for (uint32_t k = 0; __ballot(k < nall); ++k) {
-          const bool act = k < nall;
-          uint32_t off = o3;
-          off = k < n12 ? o2 : off;
-          off = k < n1 ? o1 : off;
-          const uint32_t qq = act ? k + off : 0u;
+        const uint32_t nall = n1 + n3;
+        // partner k is position k + o1 (k < n1) or k + o3; a lane past its
+        // partners reads a clamped position and its bit is masked off
+        const uint32_t o1 = b1, o3 = b3 - n1;
+        for (uint32_t k = 0;; ++k) {
+          const uint64_t act = __ballot(k < nall);
+          if (!act) break;
+          const uint32_t qq = min(k + (k < n1 ? o1 : o3), kMaxCoarse - 1u);
           const float4 o = L.box[qq];
-          const uint32_t ow = L.word[qq];
-          const uint64_t mask = __ballot(act) & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) &
-                                __ballot(a.z <= o.w) & __ballot(o.z <= a.w);
+          const uint64_t mask = act & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) & __ballot(a.z <= o.w) &
+                                __ballot(o.z <= a.w);
           const bool hit = (mask >> lane) & 1ull;
           if (mask) {
             const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            if (hit) L.stage[w][pos] = make_uint2(ja, ow & ((1u << kNarrowCellShift) - 1u));
+            if (hit) L.stage[w][pos] = make_uint2(p, qq);  // positions; ids resolved at the flush
             sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
             if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
           }
