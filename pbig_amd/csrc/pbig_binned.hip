@@ -61,6 +61,25 @@ __device__ __forceinline__ uint32_t fine_of(float v, float inv, uint32_t nf) {
   return (uint32_t)fminf(fmaxf(floorf(v * inv), 0.0f), (float)(nf - 1u));
 }
 
+// Inclusive prefix sums across lanes by DPP lane moves (GFX9 family): row
+// shifts by 1, 2, 4 and 8 within each 16-lane row; then, for the whole
+// wave, lane 15 added into rows 1 and 3 and lane 31 into rows 2 and 3.  A
+// lane with no source lane adds the `old` operand, zero.
+__device__ __forceinline__ uint32_t row_prefix_sum(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t v) {
+  v = row_prefix_sum(v);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 // Coarse bin range of a box: fine index / K (K fine cells per bin side).
 struct BinRange {
   uint32_t x0, x1, y0, y1;
@@ -430,12 +449,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     uint32_t acc = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScatterRowsPer; ++k) acc += rh[k];  // zero past this thread's bins
-    uint32_t v = acc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t o = __shfl_up(v, d, 64);
-      if (lane >= (uint32_t)d) v += o;
-    }
+    const uint32_t v = wave_prefix_sum(acc);
     if (lane == 63) L.wsum[w] = v;
     __syncthreads();
     uint32_t ex = v - acc;
@@ -735,12 +749,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
         if (w == 0) {
           const uint32_t cls = kSizeClasses - 1u - lane;  // lane 0: the largest class
           const uint32_t own = L.csum[cls];
-          uint32_t inc = own;
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(inc, d, 64);
-            if (lane >= (uint32_t)d) inc += o;
-          }
+          const uint32_t inc = wave_prefix_sum(own);
           L.cstart[cls] = inc - own;
           L.csum[cls] = 0u;  // cleared for the next bin
           if (cls == 1u) L.npair = inc - own;
@@ -769,21 +778,11 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
           if (c < kk) L.ccount[c] = (uint16_t)v[k];
           acc += v[k];
         }
-        uint32_t inc = acc;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const uint32_t o = __shfl_up(inc, d, 64);
-          if (lane >= (uint32_t)d) inc += o;
-        }
+        const uint32_t inc = wave_prefix_sum(acc);
         if (lane == 63) L.wsum[w] = inc;
         __syncthreads();
         uint32_t ex = inc - acc;
-        {  // sum of the lower waves' totals: one read per lane, then a reduction
-          uint32_t ws = lane < w ? L.wsum[lane] : 0u;
-#pragma unroll
-          for (int d = 1; d < 16; d <<= 1) ws += __shfl_xor(ws, d, 64);
-          ex += __builtin_amdgcn_readfirstlane(ws);
-        }
+        ex += __builtin_amdgcn_readlane(row_prefix_sum(lane < w ? L.wsum[lane] : 0u), 15);  // waves before mine
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t c = tid * 4 + k;
@@ -1077,21 +1076,11 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
           v[k] = c < ne ? L.cnt[c] : 0u;
           acc += v[k];
         }
-        uint32_t inc = acc;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const uint32_t o = __shfl_up(inc, d, 64);
-          if (lane >= (uint32_t)d) inc += o;
-        }
+        const uint32_t inc = wave_prefix_sum(acc);
         if (lane == 63) L.wsum[w] = inc;
         __syncthreads();
         uint32_t exs = inc - acc;
-        {
-          uint32_t ws = lane < w ? L.wsum[lane] : 0u;
-#pragma unroll
-          for (int d = 1; d < 16; d <<= 1) ws += __shfl_xor(ws, d, 64);
-          exs += __builtin_amdgcn_readfirstlane(ws);
-        }
+        exs += __builtin_amdgcn_readlane(row_prefix_sum(lane < w ? L.wsum[lane] : 0u), 15);  // waves before mine
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           const uint32_t c = tid * 5 + k;
