@@ -406,3 +406,22 @@ def test_narrow_kernel_on_a_touching_lattice(torch_cuda, binned, orc, bin_target
     got = run_device(torch_cuda, binned, wide, L, cs)
     want = orc.sort_pairs(orc.brute_pairs(wide))
     assert np.array_equal(got, want), (len(got), len(want))
+
+
+@pytest.mark.parametrize("bin_target", [16, 200, 3500])
+def test_narrow_kernel_on_crowded_cells(torch_cuda, binned, orc, bin_target):
+    """20,000 boxes of side up to one cell in a 256 x 256 domain at cell 16:
+    about 78 boxes per cell, so every box has hundreds of partners in its own
+    cell and the row below, and the pair stage fills and flushes many times
+    inside one partner loop.  A bin target of 16 plans bins of one cell per
+    side (the division by the bin side is then the identity)."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    c = rng.uniform(0, 256, size=(n, 2)).astype(np.float32)
+    w = rng.uniform(1, 16, size=(n, 2)).astype(np.float32)
+    boxes = np.stack([c[:, 0] - w[:, 0] / 2, c[:, 0] + w[:, 0] / 2, c[:, 1] - w[:, 1] / 2, c[:, 1] + w[:, 1] / 2],
+                     1).astype(np.float32)
+    binned.configure(PATHS["binned"], bin_target)
+    got = run_device(torch_cuda, binned, boxes, 256.0, 16.0)
+    want = orc.sort_pairs(orc.brute_pairs(boxes))
+    assert np.array_equal(got, want), (len(got), len(want))
