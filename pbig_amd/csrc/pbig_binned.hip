@@ -971,7 +971,7 @@ struct NarrowLds {
   float4 box[kMaxCoarse];          // the bin's coarse memberships in cell order
   uint32_t cell[kMaxCoarse];       // extended cell of each position
   uint32_t id[kMaxCoarse];         // box id of each position
-  uint32_t cnt[kMaxExt];           // counts, then scatter cursors
+  uint32_t cnt[kMaxExt];           // counts (each membership's rank in its cell is the count before it)
   uint32_t start[kMaxExt + 1];     // first position of each extended cell
   uint2 stage[kTestWaves][kBinStage];
   uint2 binfo[kMaxBinsPerGroup];
@@ -1052,18 +1052,20 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     const uint32_t nf = bg.w;
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
-      // A. each membership's extended cell; counts
-      uint32_t ek[kRegs];
+      // A. each membership's extended cell, and its rank among the cell's
+      //    memberships (the count before it)
+      uint32_t ek[kRegs], rk[kRegs];
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         ek[k] = 0u;
+        rk[k] = 0u;
         if (j < m) {
           int rx = (int)fine_of(rb[k].x, inv_f, nf) - gx, ry = (int)fine_of(rb[k].z, inv_f, nf) - gy;
           rx = rx < -1 ? -1 : (rx > (int)Kx - 1 ? (int)Kx - 1 : rx);
           ry = ry < -1 ? -1 : (ry > (int)Ky - 1 ? (int)Ky - 1 : ry);
           ek[k] = (uint32_t)(rx + 1) + (uint32_t)(ry + 1) * ex;
-          atomicAdd(&L.cnt[ek[k]], 1u);
+          rk[k] = atomicAdd(&L.cnt[ek[k]], 1u);
         }
       }
       __syncthreads();
@@ -1084,10 +1086,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           const uint32_t c = tid * 5 + k;
-          if (c < ne) {
-            L.start[c] = exs;
-            L.cnt[c] = exs;  // scatter cursor
-          }
+          if (c < ne) L.start[c] = exs;
           exs += v[k];
         }
         if (tid == kTestThreads - 1) L.start[ne] = exs;
@@ -1098,7 +1097,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         if (j < m) {
-          const uint32_t pos = atomicAdd(&L.cnt[ek[k]], 1u);
+          const uint32_t pos = L.start[ek[k]] + rk[k];
           L.box[pos] = rb[k];
           L.cell[pos] = ek[k];
           L.id[pos] = ri[k];
