@@ -886,11 +886,10 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
               const uint64_t mask = __ballot(act[u]) & __ballot(a.x <= o[u].y) & __ballot(o[u].x <= a.y) &
                                     __ballot(a.z <= o[u].w) & __ballot(o[u].z <= a.w) &
                                     __ballot((aw | ow[u]) >= (kFlagX | kFlagY));
-              const bool hit = (mask >> lane) & 1ull;
               if (mask) {
                 const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                if (hit) L.stage[w][pos] = make_uint2(ja, ow[u] & kWordIdxMask);
+                if (__builtin_amdgcn_inverse_ballot_w64(mask)) L.stage[w][pos] = make_uint2(ja, ow[u] & kWordIdxMask);
                 sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
                 if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
               }
@@ -1150,11 +1149,12 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
           const float4 o = L.box[qq];
           const uint64_t mask = act & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) & __ballot(a.z <= o.w) &
                                 __ballot(o.z <= a.w);
-          const bool hit = (mask >> lane) & 1ull;
           if (mask) {
             const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            if (hit) L.stage[w][pos] = make_uint2(p, qq);  // positions; ids resolved at the flush
+            // the ballot itself as the lane mask of the hits
+            if (__builtin_amdgcn_inverse_ballot_w64(mask))
+              L.stage[w][pos] = make_uint2(p, qq);  // positions; ids resolved at the flush
             sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
             if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
           }
