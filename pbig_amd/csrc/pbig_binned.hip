@@ -996,7 +996,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
                                                                   uint2* __restrict__ out, uint64_t cap,
                                                                   uint32_t direct, Meta* __restrict__ meta) {
   NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
-  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
   if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
