@@ -432,12 +432,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     # Cell 32 is the autotuner table's one box per cell at this density
     # (pbig_amd/tuning.json); every box then spans at most two cells per axis
-    # and the narrow-box test kernel runs, whose best bin target here is
-    # 3,500 (the general test kernel's best is cell 64, bin target 2,000).
+    # and the narrow-box test schedule runs.  Its best bin target here is
+    # 4,096: bins of 64 x 64 cells, 1,024 of them, four on each of the 256
+    # test workgroups, about 4,200 memberships each (the schedule takes up to
+    # 5,120).  The general test schedule's best is cell 64, bin target 2,000.
     ap.add_argument("--cell-size", type=float, default=32.0,
                     help="fine cell side; 0 = the reference rule (256 on this instance)")
     ap.add_argument("--path", choices=("auto", "general", "binned"), default="auto")
-    ap.add_argument("--bin-target", type=int, default=3500)
+    ap.add_argument("--bin-target", type=int, default=4096)
     ap.add_argument("--no-batched", action="store_true", help="skip the 4,096-instance batch leg")
     ap.add_argument("--no-batched-baseline", action="store_true",
                     help="batch leg without the per-instance comparison loops (for profiling)")

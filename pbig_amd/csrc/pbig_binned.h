@@ -12,6 +12,7 @@ namespace pbig {
 constexpr uint32_t kMaxBins = 6400;          // coarse bins per instance (LDS cursors of the scatter)
 constexpr uint32_t kMaxFine = 4096;          // fine memberships of one strip held in LDS
 constexpr uint32_t kMaxCoarse = 4096;        // coarse memberships of one bin (registers + LDS ids)
+constexpr uint32_t kNarrowMax = 5120;        // the same under the narrow-box test schedule
 constexpr uint32_t kMaxCellsPerBin = 4096;   // k*k fine cells per coarse bin (k <= 64)
 constexpr uint32_t kMaxK = 64;
 constexpr uint32_t kBinStage = 128;          // staged pairs per wave

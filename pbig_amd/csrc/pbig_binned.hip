@@ -319,9 +319,10 @@ __device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get
 
 // Bucket offset of every bin (exclusive scan of all bins' totals), the
 // grand total and the workspace check.  With dense_exit (a launch of one
-// instance), an instance whose heavy bins (over kMaxCoarse memberships) hold
-// more than a quarter of its memberships is flagged kErrDense, and the
-// scatter and test kernels then do nothing.
+// instance), an instance whose heavy bins (over what the launch's test
+// schedule holds: kNarrowMax memberships when no box is wide, else
+// kMaxCoarse) hold more than a quarter of its memberships is flagged
+// kErrDense, and the scatter and test kernels then do nothing.
 __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
                                                      uint32_t* __restrict__ bin_start, uint64_t cap,
                                                      uint32_t dense_exit, Meta* __restrict__ meta) {
@@ -332,10 +333,11 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
       n_bins, L, [&](uint32_t i) { return (uint64_t)total[i]; },
       [&](uint32_t i, uint64_t ex) { bin_start[i] = (uint32_t)(ex < cap ? ex : cap); });
   if (dense_exit) {
+    const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
     uint64_t h = 0;
     for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
       const uint32_t t = total[i];
-      h += t > kMaxCoarse ? t : 0u;
+      h += t > limit ? t : 0u;
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
@@ -965,13 +967,17 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
 // contiguous position ranges.  The partner loop reads only the partner's
 // box; a hit stages the two positions, whose ids the flush looks up.
 constexpr uint32_t kMaxExt = (kMaxK + 1) * (kMaxK + 1);  // extended cells of a bin
+// Coarse memberships of one bin under this schedule: kNarrowMax (a position
+// costs 22 bytes of LDS here), kNarrowRegs per thread in registers.
+constexpr uint32_t kNarrowRegs = kNarrowMax / kTestThreads;
+static_assert(kNarrowMax <= 65535u, "positions fit 16 bits");
 
 struct NarrowLds {
-  float4 box[kMaxCoarse];          // the bin's coarse memberships in cell order
-  uint32_t cell[kMaxCoarse];       // extended cell of each position
-  uint32_t id[kMaxCoarse];         // box id of each position
+  float4 box[kNarrowMax];          // the bin's coarse memberships in cell order
+  uint16_t cell[kNarrowMax];       // extended cell of each position
+  uint32_t id[kNarrowMax];         // box id of each position
   uint32_t cnt[kMaxExt];           // counts (each membership's rank in its cell is the count before it)
-  uint32_t start[kMaxExt + 1];     // first position of each extended cell
+  uint16_t start[kMaxExt + 1];     // first position of each extended cell
   uint2 stage[kTestWaves][kBinStage];
   uint2 binfo[kMaxBinsPerGroup];
   uint4 bingrid[kMaxBinsPerGroup];
@@ -1010,8 +1016,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
   if (tid == 0) L.out_n = 0u;
 
   const uint64_t last = cap_bucket - 1;
-  float4 rb[kRegs];
-  uint32_t ri[kRegs];
+  float4 rb[kNarrowRegs];
+  uint32_t ri[kNarrowRegs];
   for (uint32_t t = 0; t < my_bins; ++t) {
     const uint32_t slot = t % kMaxBinsPerGroup;
     if (slot == 0) {
@@ -1032,7 +1038,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         const uint64_t s0 = L.binfo[0].x;
         const uint32_t j_hi = L.binfo[0].y ? L.binfo[0].y - 1u : 0u;
 #pragma unroll
-        for (uint32_t k = 0; k < kRegs; ++k) {
+        for (uint32_t k = 0; k < kNarrowRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads < j_hi ? tid + k * kTestThreads : j_hi;
           const uint64_t at = s0 + j < last ? s0 + j : last;
           rb[k] = bbox[at];
@@ -1051,12 +1057,12 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     const float inv_f = L.bininv[slot];
     const uint32_t nf = bg.w;
     bool prefetched = false;
-    if (m >= 2 && m <= kMaxCoarse) {
+    if (m >= 2 && m <= kNarrowMax) {
       // A. each membership's extended cell, and its rank among the cell's
       //    memberships (the count before it)
-      uint32_t ek[kRegs], rk[kRegs];
+      uint32_t ek[kNarrowRegs], rk[kNarrowRegs];
 #pragma unroll
-      for (uint32_t k = 0; k < kRegs; ++k) {
+      for (uint32_t k = 0; k < kNarrowRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         ek[k] = 0u;
         rk[k] = 0u;
@@ -1086,20 +1092,20 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           const uint32_t c = tid * 5 + k;
-          if (c < ne) L.start[c] = exs;
+          if (c < ne) L.start[c] = (uint16_t)exs;
           exs += v[k];
         }
-        if (tid == kTestThreads - 1) L.start[ne] = exs;
+        if (tid == kTestThreads - 1) L.start[ne] = (uint16_t)exs;
       }
       __syncthreads();
       // C. one position per membership, in cell order: box, cell and id
 #pragma unroll
-      for (uint32_t k = 0; k < kRegs; ++k) {
+      for (uint32_t k = 0; k < kNarrowRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
         if (j < m) {
           const uint32_t pos = L.start[ek[k]] + rk[k];
           L.box[pos] = rb[k];
-          L.cell[pos] = ek[k];
+          L.cell[pos] = (uint16_t)ek[k];
           L.id[pos] = ri[k];
         }
       }
@@ -1107,7 +1113,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
       for (uint32_t c = tid; c < ne; c += kTestThreads) L.cnt[c] = 0u;  // for the next bin
       if (t + 1 < my_bins) {
 #pragma unroll
-        for (uint32_t k = 0; k < kRegs; ++k) {
+        for (uint32_t k = 0; k < kNarrowRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
           const uint64_t at = sN + j < last ? sN + j : last;
           rb[k] = bbox[at];
@@ -1146,7 +1152,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         for (uint32_t k = 0;; ++k) {
           const uint64_t act = __ballot(k < nall);
           if (!act) break;
-          const uint32_t qq = min(k + (k < n1 ? o1 : o3), kMaxCoarse - 1u);
+          const uint32_t qq = min(k + (k < n1 ? o1 : o3), kNarrowMax - 1u);
           const float4 o = L.box[qq];
           const uint64_t mask = act & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) & __ballot(a.z <= o.w) &
                                 __ballot(o.z <= a.w);
@@ -1163,7 +1169,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
       }
       if (sn) stage_flush(L, w, sn, region, cap_wg);
       __syncthreads();  // the positions are reused by the next bin
-    } else if (m > kMaxCoarse) {
+    } else if (m > kNarrowMax) {
       if (tid == 0) {
         overfull[atomicAdd(&meta->max_per_cell, 1u)] = b;
         atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrOverfull);
@@ -1177,7 +1183,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     if (!prefetched && t + 1 < my_bins) {
       __syncthreads();
 #pragma unroll
-      for (uint32_t k = 0; k < kRegs; ++k) {
+      for (uint32_t k = 0; k < kNarrowRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads < jN ? tid + k * kTestThreads : jN;
         const uint64_t at = sN + j < last ? sN + j : last;
         rb[k] = bbox[at];

@@ -21,8 +21,8 @@ def test_alg_bytes_of_the_4m_instance():
 
 
 def test_committed_traffic_profile_is_read():
-    traffic, src = bench.read_traffic("binned", 32.0, 3500)  # bench.py's defaults
-    assert src == os.path.join("profiles", "traffic_plain_uniform_4m_binned_cs32_bt3500.txt")
+    traffic, src = bench.read_traffic("binned", 32.0, 4096)  # bench.py's defaults
+    assert src == os.path.join("profiles", "traffic_plain_uniform_4m_binned_cs32_bt4096.txt")
     for k in ("bin_hist", "bin_scan", "bin_scatter", "bin_test"):
         assert traffic[k] > 0, k
     # the test kernel reads at least its bucket: 20 bytes per coarse membership

@@ -268,11 +268,11 @@ def test_binned_c1_cell_sizes_and_bin_sizes(torch_cuda, binned, orc, cs, bin_tar
     assert np.array_equal(got, np.load(os.path.join(GOLDEN, "c1_pairs.npy")))
 
 
-@pytest.mark.parametrize("cs", [32, 64, 128])
-def test_binned_c2_full_size(torch_cuda, binned, orc, manifest, cs):
+@pytest.mark.parametrize("cs,bin_target", [(32, 2000), (64, 2000), (128, 2000), (32, 4096)])
+def test_binned_c2_full_size(torch_cuda, binned, orc, manifest, cs, bin_target):
     from pbig_amd import workloads
 
-    binned.configure(PATHS["binned"], 2000)
+    binned.configure(PATHS["binned"], bin_target)
     boxes, L = workloads.c2_large_uniform()
     m = manifest["c2_uniform_4m"]
     got = run_device(torch_cuda, binned, boxes, L, cs, cap=4 * m["pairs"])
@@ -314,11 +314,12 @@ def test_dense_spot_resolved_on_binned_path(torch_cuda, orc):
 
 
 def test_auto_falls_back_on_unsplittable_stack(torch_cuda, orc):
-    """5000 identical boxes share every cell at any refinement: AUTO must
-    return the exact set through the general path."""
+    """6000 identical boxes share every cell at any refinement, more than a
+    bin holds under either test schedule (4,096 general, 5,120 narrow): AUTO
+    must return the exact set through the general path."""
     import pbig_amd
 
-    boxes = np.tile(np.array([[20.0, 20.5, 30.0, 30.25]], np.float32), (5000, 1))
+    boxes = np.tile(np.array([[20.0, 20.5, 30.0, 30.25]], np.float32), (6000, 1))
     boxes = np.concatenate([boxes, np.array([[1, 2, 1, 2], [20.5, 21, 30, 31]], np.float32)])
     with pbig_amd.Engine(0) as e:
         got = run_device(torch_cuda, e, boxes, 64.0, 16)
@@ -408,13 +409,14 @@ def test_narrow_kernel_on_a_touching_lattice(torch_cuda, binned, orc, bin_target
     assert np.array_equal(got, want), (len(got), len(want))
 
 
-@pytest.mark.parametrize("bin_target", [16, 200, 3500])
+@pytest.mark.parametrize("bin_target", [16, 200, 3500, 5000])
 def test_narrow_kernel_on_crowded_cells(torch_cuda, binned, orc, bin_target):
     """20,000 boxes of side up to one cell in a 256 x 256 domain at cell 16:
     about 78 boxes per cell, so every box has hundreds of partners in its own
     cell and the row below, and the pair stage fills and flushes many times
     inside one partner loop.  A bin target of 16 plans bins of one cell per
-    side (the division by the bin side is then the identity)."""
+    side (the division by the bin side is then the identity); 5,000 plans
+    bins near the narrow schedule's capacity, some over it (re-binned)."""
     rng = np.random.default_rng(11)
     n = 20000
     c = rng.uniform(0, 256, size=(n, 2)).astype(np.float32)
@@ -425,3 +427,24 @@ def test_narrow_kernel_on_crowded_cells(torch_cuda, binned, orc, bin_target):
     got = run_device(torch_cuda, binned, boxes, 256.0, 16.0)
     want = orc.sort_pairs(orc.brute_pairs(boxes))
     assert np.array_equal(got, want), (len(got), len(want))
+
+
+def test_auto_keeps_large_narrow_bins_binned(torch_cuda, orc, manifest):
+    """At cell 32 and bin target 4,096 the 4M instance's bins hold about 4,200
+    memberships: over what the general test schedule takes (4,096), within
+    the narrow-box schedule's 5,120.  The automatic choice must not count
+    them as heavy; the pairs are the manifest's."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    eng = pbig_amd.Engine(0)
+    try:
+        eng.configure(PATHS["auto"], 4096)
+        boxes, L = workloads.c2_large_uniform()
+        m = manifest["c2_uniform_4m"]
+        got = run_device(torch_cuda, eng, boxes, L, 32.0, cap=4 * m["pairs"])
+        assert eng.stats()["path"] == PATHS["binned"]
+        assert len(got) == m["pairs"]
+        assert orc.pair_hash(got) == m["sha256"]
+    finally:
+        eng.close()
