@@ -983,6 +983,7 @@ struct NarrowLds {
   uint4 bingrid[kMaxBinsPerGroup];
   float bininv[kMaxBinsPerGroup];
   uint32_t wsum[kTestWaves];
+  uint32_t dnext;                  // next chunk of positions to claim, past the first 16
   uint32_t out_n;
   unsigned long long out_base;
 };
@@ -1096,6 +1097,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
           exs += v[k];
         }
         if (tid == kTestThreads - 1) L.start[ne] = (uint16_t)exs;
+        if (tid == 0) L.dnext = 0u;
       }
       __syncthreads();
       // C. one position per membership, in cell order: box, cell and id
@@ -1125,7 +1127,10 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
       //    (W's positions end where my cell's begin: one range), then
       //    SW..S(..SE) of the row below.  Each pair inside a cell is tested
       //    once, by its later member.
-      for (uint32_t p0 = w * 64; p0 < m; p0 += kTestThreads) {
+      //    Chunks of 64 positions: a wave's first by its index, the rest from
+      //    a shared counter, so that waves whose chunks had fewer partners
+      //    take more of them.
+      for (uint32_t p0 = w * 64; p0 < m;) {
         const uint32_t p = p0 + lane;
         uint32_t b1 = 0, n1 = 0, b3 = 0, n3 = 0;
         float4 a = make_float4(0.f, -1.f, 0.f, -1.f);
@@ -1166,6 +1171,9 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
             if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
           }
         }
+        uint32_t claim = 0;
+        if (lane == 0) claim = atomicAdd(&L.dnext, 64u);
+        p0 = kTestThreads + __builtin_amdgcn_readfirstlane(claim);
       }
       if (sn) stage_flush(L, w, sn, region, cap_wg);
       __syncthreads();  // the positions are reused by the next bin
