@@ -73,6 +73,13 @@ __device__ __forceinline__ uint32_t row_prefix_sum(uint32_t v) {
   return v;
 }
 
+// Two independent inclusive prefix sums, over lanes 0-31 and over lanes 32-63.
+__device__ __forceinline__ uint32_t half_prefix_sum(uint32_t v) {
+  v = row_prefix_sum(v);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  return v;
+}
+
 __device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t v) {
   v = row_prefix_sum(v);
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
@@ -191,6 +198,7 @@ __global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ bo
 // the registers after a prefix over the groups.  Instances of more than
 // 1,024 chunks take several tiles, the running total carried.
 constexpr uint32_t kScanBins = 32, kScanGroups = 32, kScanPer = 32;
+static_assert(kScanBins * kScanGroups == 1024 && kScanGroups == 32, "the group prefix is a 32-lane scan per bin");
 
 __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ hist,
                                                     const InstDesc* __restrict__ inst,
@@ -201,6 +209,30 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
   __shared__ uint32_t part[kScanGroups][kScanBins + 1];
   __shared__ uint32_t carry[kScanBins];
   __shared__ uint32_t tiles;
+  if (blockIdx.x == gridDim.x - 1) {
+    // The last block: the result words of the call, the pair counter and the
+    // boxes' validity and width flags, from every chunk's histogram row.
+    uint32_t e = 0;
+    for (uint32_t w = threadIdx.x; w < n_chunks; w += blockDim.x) {
+      const InstDesc I = inst[chunk_inst[w]];
+      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * hist_pitch(I.nb) + I.nb];
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
+    __shared__ uint32_t es[16];
+    if ((threadIdx.x & 63) == 0) es[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t any = 0;
+      for (int k = 0; k < 16; ++k) any |= es[k];
+      meta->n_pairs = 0ull;
+      meta->error = (any & 1u) ? kErrInvalidBox : 0ull;
+      meta->items = 0ull;
+      meta->max_per_cell = 0u;
+      meta->wide = (any & 2u) ? 1u : 0u;
+    }
+    return;
+  }
   const uint32_t bl = threadIdx.x % kScanBins, g = threadIdx.x / kScanBins;
   const uint32_t gb = blockIdx.x * kScanBins + bl;
   uint32_t b = 0, nch = 0, hp = 0, op = 0;
@@ -230,14 +262,15 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
     }
     part[g][bl] = s;
     __syncthreads();
-    if (g == 0) {  // exclusive prefix over the groups, after the earlier tiles
-      uint32_t acc = carry[bl];
-      for (uint32_t k = 0; k < kScanGroups; ++k) {
-        const uint32_t x = part[k][bl];
-        part[k][bl] = acc;
-        acc += x;
-      }
-      carry[bl] = acc;
+    {  // exclusive prefix over the groups, after the earlier tiles: each
+       // 32-lane half-wave takes one bin, a lane per group
+      const uint32_t tb = threadIdx.x / kScanGroups, tg = threadIdx.x % kScanGroups;
+      const uint32_t x = part[tg][tb];
+      const uint32_t inc = half_prefix_sum(x);
+      const uint32_t c = carry[tb];
+      part[tg][tb] = c + inc - x;
+      __syncthreads();  // every lane has read carry[tb]
+      if (tg == kScanGroups - 1) carry[tb] = c + inc;
     }
     __syncthreads();
     uint32_t base = part[g][bl];
@@ -249,28 +282,6 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
     __syncthreads();  // part[] is rewritten by the next tile
   }
   if (g == 0 && gb < n_bins) total[gb] = carry[bl];
-  if (blockIdx.x == 0) {
-    // Result words of the call: pair counter and the boxes' validity.
-    uint32_t e = 0;
-    for (uint32_t w = threadIdx.x; w < n_chunks; w += blockDim.x) {
-      const InstDesc I = inst[chunk_inst[w]];
-      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * hist_pitch(I.nb) + I.nb];
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
-    __shared__ uint32_t es[16];
-    if ((threadIdx.x & 63) == 0) es[threadIdx.x >> 6] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t any = 0;
-      for (int k = 0; k < 16; ++k) any |= es[k];
-      meta->n_pairs = 0ull;
-      meta->error = (any & 1u) ? kErrInvalidBox : 0ull;
-      meta->items = 0ull;
-      meta->max_per_cell = 0u;
-      meta->wide = (any & 2u) ? 1u : 0u;
-    }
-  }
 }
 
 // Exclusive scan of n values by one 1024-thread workgroup, in tiles of 4096:
@@ -1357,7 +1368,8 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(0)
   k_bin_hist<<<nc, 1024, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
   PBIG_MARK(1)
-  k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
+  // one block more than the bins need: the last one makes the call's result words
+  k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
   k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
