@@ -87,6 +87,25 @@ __device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t v) {
   return v;
 }
 
+// The same for 64-bit values: each lane move takes both halves from the same
+// source lane (or zero for both), and the add carries.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, kCtrl, kRowMask, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), kCtrl, kRowMask, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_prefix_sum64(uint64_t v) {
+  v += dpp_u64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_u64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_u64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_u64<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_u64<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_u64<0x143, 0xc>(v);  // row_bcast:31
+  return v;
+}
+
 // Coarse bin range of a box: fine index / K (K fine cells per bin side).
 struct BinRange {
   uint32_t x0, x1, y0, y1;
@@ -304,12 +323,7 @@ __device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get
       x[k] = i0 + k < n ? get(i0 + k) : 0ull;
       acc += x[k];
     }
-    uint64_t inc = acc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint64_t o = __shfl_up(inc, d, 64);
-      if (lane >= (uint32_t)d) inc += o;
-    }
+    const uint64_t inc = wave_prefix_sum64(acc);
     if (lane == 63) L.w[w] = inc;
     __syncthreads();
     uint64_t ex = carry + inc - acc, tile = 0;
@@ -350,9 +364,8 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
       const uint32_t t = total[i];
       h += t > limit ? t : 0u;
     }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
-    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&heavy, (unsigned long long)h);
+    h = wave_prefix_sum64(h);
+    if ((threadIdx.x & 63) == 63 && h) atomicAdd(&heavy, (unsigned long long)h);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -385,7 +398,14 @@ struct ScatterLds {
   uint32_t id[kStageEntries];
   uint32_t dst[kStageEntries];
   uint32_t wsum[16];
+  ScanLds scan;              // the bucket starts of a single instance (fold)
+  unsigned long long heavy;  // its memberships in bins over the test schedule's capacity
 };
+
+// A single instance with at most kFoldBins bins: the scatter scans the bins'
+// totals itself (bucket starts in LDS after the cursors, nb_cap more words),
+// and no k_bin_starts launch precedes it.
+constexpr uint32_t kFoldBins = 2048;
 
 // kIds: the boxes' ids come from ids_in (virtual instances), else they are
 // the index inside the instance.
@@ -397,14 +417,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
                                                                  uint32_t n_chunks,
                                                                  const uint32_t* __restrict__ hist,
                                                                  const uint32_t* __restrict__ off,
-                                                                 const uint32_t* __restrict__ bin_start,
+                                                                 uint32_t* __restrict__ bin_start,
                                                                  float4* __restrict__ bbox,
                                                                  uint32_t* __restrict__ bid, uint64_t cap,
-                                                                 uint32_t nb_cap, Meta* __restrict__ meta) {
+                                                                 uint32_t nb_cap, const uint32_t* __restrict__ total,
+                                                                 uint32_t fold, uint32_t dense_exit,
+                                                                 Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   ScatterLds& L = *reinterpret_cast<ScatterLds*>(lds_raw);
   uint32_t* const cur = reinterpret_cast<uint32_t*>(lds_raw + sizeof(ScatterLds));
   int32_t* const woff = reinterpret_cast<int32_t*>(cur + nb_cap);
+  uint32_t* const bst = reinterpret_cast<uint32_t*>(woff + nb_cap);  // fold: the bins' bucket starts
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // Persistent workgroups, XCD-aware.  Workgroups go to the 8 XCDs
   // round-robin; the gridDim / 8 workgroups of XCD x walk that XCD's
@@ -413,8 +436,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   // bucket come from the same XCD at about the same time and merge in its
   // L2.  The next chunk's boxes and table rows are loaded while the current
   // chunk's staged memberships are written out.
-  if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
-  if (meta->error & kErrDense) return;  // the host runs the general path
+  if (!fold) {
+    if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
+    if (meta->error & kErrDense) return;  // the host runs the general path
+  }
   const uint32_t xper = (n_chunks + 7u) / 8u;
   const uint32_t xnext = gridDim.x / 8u;  // workgroups per XCD (the grid is a multiple of 8)
   uint32_t r = blockIdx.x >> 3;
@@ -443,10 +468,42 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       const bool in = k < per && b0 + k < nbj;
       rh[k] = in ? jh[b0 + k] : 0u;
       ro[k] = in ? jo[b0 + k] : 0u;
-      rs[k] = in ? js[b0 + k] : 0u;
+      rs[k] = in && !fold ? js[b0 + k] : 0u;
     }
   };
   if (wc < wend) load_chunk(wc);
+  if (fold) {
+    // One instance: every workgroup scans its bins' totals into LDS (the
+    // bucket starts) and checks them as k_bin_starts would, while its first
+    // chunk loads; workgroup 0 also publishes the starts, the membership
+    // count and the error bits.  Every workgroup reaches the same verdict.
+    const InstDesc I0 = inst[0];
+    const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
+    if (tid == 0) L.heavy = 0ull;
+    uint64_t hv = 0;
+    const uint64_t grand = block_exscan(
+        I0.nb, L.scan,
+        [&](uint32_t i) {
+          const uint32_t t = total[I0.bin0 + i];
+          hv += t > limit ? t : 0u;
+          return (uint64_t)t;
+        },
+        [&](uint32_t i, uint64_t ex) {
+          const uint32_t v = (uint32_t)(ex < cap ? ex : cap);
+          bst[i] = v;
+          if (blockIdx.x == 0) bin_start[I0.bin0 + i] = v;
+        });
+    hv = wave_prefix_sum64(hv);
+    if (lane == 63 && hv) atomicAdd(&L.heavy, hv);
+    __syncthreads();
+    const bool dense = dense_exit && 4ull * L.heavy > grand;
+    if (blockIdx.x == 0 && tid == 0) {
+      meta->memberships = grand;
+      if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
+      if (dense) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrDense);
+    }
+    if (grand > cap || dense) return;
+  }
   for (; wc < wend; wc += xnext) {
   const InstDesc I = inst[chunk_inst[wc]];
   const uint32_t nb = I.nb;
@@ -474,7 +531,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     for (uint32_t k = 0; k < kScatterRowsPer; ++k) {
       const uint32_t b = b0 + k;
       if (k < per && b < nb) {
-        const uint32_t dst0 = rs[k] + ro[k];
+        const uint32_t dst0 = (fold ? bst[b] : rs[k]) + ro[k];
         cur[b] = dst0;
         woff[b] = (int32_t)(ex - dst0);
         ex += rh[k];
@@ -1372,10 +1429,11 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
-  k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
+  const uint32_t fold = bp.n_inst == 1u && bp.nb_max <= kFoldBins ? 1u : 0u;
+  if (!fold) k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
-  const size_t scatter_lds = sizeof(ScatterLds) + 8u * nb_cap;
+  const size_t scatter_lds = sizeof(ScatterLds) + (fold ? 12u : 8u) * nb_cap;
   // A single instance's chunks are all alike: persistent workgroups, one per
   // CU (a multiple of 8, one share per XCD).  A batch's chunks differ in
   // size: one workgroup per chunk, balanced by the dispatcher.
@@ -1384,11 +1442,11 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   if (ids_in)
     k_bin_scatter<true><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
-        ws.cap_bucket, nb_cap, ws.meta);
+        ws.cap_bucket, nb_cap, ws.total, fold, bp.dense_exit, ws.meta);
   else
     k_bin_scatter<false><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
-        ws.cap_bucket, nb_cap, ws.meta);
+        ws.cap_bucket, nb_cap, ws.total, fold, bp.dense_exit, ws.meta);
   PBIG_MARK(4)
   k_bin_test<<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
