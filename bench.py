@@ -538,9 +538,11 @@ def main():
     value = n_pairs / (ms_per_step / 1e3)
     avg = {k: v / calls for k, v in per_kernel.items()}
     if path_name == "binned":
-        # A single instance's test kernel writes the output itself; the pair
-        # scan and compaction kernels are not launched (their slots would
-        # only time two back-to-back events).
+        # A single instance's test kernel writes the output itself and its
+        # scatter scans the bucket offsets: the offsets, pair scan and
+        # compaction kernels are not launched (their slots would only time
+        # two back-to-back events).
+        avg.pop("bin_starts", None)
         avg.pop("bin_pair_scan", None)
         avg.pop("bin_compact", None)
     # B_alg terms by kernel (the binned kernels do the same four jobs)
