@@ -197,10 +197,19 @@ __global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ bo
         continue;
       }
       const uint4 f = fine_corners(b, I);
-      if (f.y - f.x > 1u || f.w - f.z > 1u) atomicOr(&err, 2u);  // spans more than two fine cells
       const BinRange r = bin_range_of(f, I);
-      for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-        for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * I.nbx + cx], 1u);
+      if (f.y - f.x > 1u || f.w - f.z > 1u) {  // spans more than two fine cells
+        atomicOr(&err, 2u);
+        for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
+          for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * I.nbx + cx], 1u);
+      } else {  // at most two bins per axis: the first, then east, north, north-east
+        const uint32_t bb = r.y0 * I.nbx + r.x0;
+        const bool ex = r.x1 != r.x0, ey = r.y1 != r.y0;
+        atomicAdd(&h[bb], 1u);
+        if (ex) atomicAdd(&h[bb + 1u], 1u);
+        if (ey) atomicAdd(&h[bb + I.nbx], 1u);
+        if (ex && ey) atomicAdd(&h[bb + I.nbx + 1u], 1u);
+      }
     }
   }
   __syncthreads();
@@ -471,6 +480,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       rs[k] = in && !fold ? js[b0 + k] : 0u;
     }
   };
+  const bool narrow = !meta->wide;  // the histogram's flag, uniform over the launch
   if (wc < wend) load_chunk(wc);
   if (fold) {
     // One instance: every workgroup scans its bins' totals into LDS (the
@@ -540,7 +550,22 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   }
   __syncthreads();
   // 3. rank every membership; stage it at its rank (or write it directly
-  //    past the stage)
+  //    past the stage).  When no box of the launch spans more than two fine
+  //    cells per axis, none spans more than two bins per axis: a box's bins
+  //    are its first one and, when it crosses into them, the next one east,
+  //    north and north-east, placed without loops.
+  auto place = [&](uint32_t bb, const float4& b, uint32_t i) {
+    const uint32_t d = atomicAdd(&cur[bb], 1u);
+    const uint32_t at = (uint32_t)((int32_t)d + woff[bb]);
+    if (at < kStageEntries) {
+      L.box[at] = b;
+      L.id[at] = i;
+      L.dst[at] = d;
+    } else if (d < cap) {
+      bbox[d] = b;
+      bid[d] = i;
+    }
+  };
 #pragma unroll
   for (uint32_t k = 0; k < kScatterPer; ++k) {
     const uint64_t gi = start + tid + k * kScatterThreads;
@@ -549,20 +574,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     const BinRange rg = bin_range(b, I);
     uint32_t i = (uint32_t)(gi - I.box0);  // id inside the instance
     if (kIds) i = ids_in[gi];
-    for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
-      for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) {
-        const uint32_t bb = cy * I.nbx + cx;
-        const uint32_t d = atomicAdd(&cur[bb], 1u);
-        const uint32_t at = (uint32_t)((int32_t)d + woff[bb]);
-        if (at < kStageEntries) {
-          L.box[at] = b;
-          L.id[at] = i;
-          L.dst[at] = d;
-        } else if (d < cap) {
-          bbox[d] = b;
-          bid[d] = i;
-        }
-      }
+    if (narrow) {
+      const uint32_t bb = rg.y0 * I.nbx + rg.x0;
+      const bool ex = rg.x1 != rg.x0, ey = rg.y1 != rg.y0;
+      place(bb, b, i);
+      if (ex) place(bb + 1u, b, i);
+      if (ey) place(bb + I.nbx, b, i);
+      if (ex && ey) place(bb + I.nbx + 1u, b, i);
+    } else {
+      for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
+        for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) place(cy * I.nbx + cx, b, i);
+    }
   }
   __syncthreads();
   if (wc + xnext < wend) load_chunk(wc + xnext);  // in flight during the write-out
