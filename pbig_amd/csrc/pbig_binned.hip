@@ -1052,10 +1052,11 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
 //                                                   (R = (x+1, y)).
 // Case by case (B = A, W, SW, S or SE of A, and the mirror cases), every
 // overlapping pair with R in this bin is tested exactly once, and only
-// there: the overlap test alone decides, with no reference-cell flags.  SW,
-// S and SE are adjacent cells of one row, so a box's partners are three
-// contiguous position ranges.  The partner loop reads only the partner's
-// box; a hit stages the two positions, whose ids the flush looks up.
+// there: the overlap test alone decides, with no reference-cell flags.  W's
+// positions end where C's begin, and SW, S and SE are adjacent cells of one
+// row, so a box's partners are two contiguous position ranges.  The partner
+// loop reads only the partner's box; a hit stages the two positions, whose
+// ids the flush looks up.
 constexpr uint32_t kMaxExt = (kMaxK + 1) * (kMaxK + 1);  // extended cells of a bin
 // Coarse memberships of one bin under this schedule: kNarrowMax (a position
 // costs 22 bytes of LDS here), kNarrowRegs per thread in registers.
