@@ -17,7 +17,8 @@ constexpr uint32_t kMaxCellsPerBin = 4096;   // k*k fine cells per coarse bin (k
 constexpr uint32_t kMaxK = 64;
 constexpr uint32_t kBinStage = 128;          // staged pairs per wave
 constexpr uint32_t kMaxBinsPerGroup = 128;   // bins walked by one persistent test workgroup
-constexpr uint32_t kBatch = 4;               // boxes per thread loaded before use (hist): a 4,096-box chunk in one pass
+constexpr uint32_t kBatch = 4;               // boxes per thread loaded before use (hist)
+constexpr uint32_t kHistThreads = 512;       // histogram workgroup: a 4,096-box chunk in two passes, four per CU
 constexpr uint32_t kScatterChunk = 4096;     // boxes per histogram/scatter workgroup (at most)
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
 constexpr unsigned long long kErrDense = 16ull;    // heavy bins hold over a quarter of the memberships

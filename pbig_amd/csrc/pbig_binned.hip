@@ -166,7 +166,7 @@ __device__ __forceinline__ void chunk_range(const InstDesc& I, uint32_t w, uint6
   *end = *start + I.chunk < inst_end ? *start + I.chunk : inst_end;
 }
 
-__global__ __launch_bounds__(1024) void k_bin_hist(const float4* __restrict__ boxes,
+__global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restrict__ boxes,
                                                     const InstDesc* __restrict__ inst,
                                                     const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
                                                     uint32_t* __restrict__ hist) {
@@ -1446,7 +1446,7 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 #define PBIG_MARK(k) \
   if (ev && (e = hipEventRecord(ev[k], s)) != hipSuccess) return e;
   PBIG_MARK(0)
-  k_bin_hist<<<nc, 1024, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
+  k_bin_hist<<<nc, kHistThreads, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
   PBIG_MARK(1)
   // one block more than the bins need: the last one makes the call's result words
   k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
