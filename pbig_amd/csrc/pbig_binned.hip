@@ -14,7 +14,7 @@
 //                  counts its coarse-bin memberships in an LDS histogram,
 //                  written as one row of that instance's hist[chunk][bin]
 //   k_bin_scan     per bin, exclusive scan over the instance's chunks (the
-//                  table read once, in whole lines)
+//                  table read once)
 //   k_bin_starts   exclusive scan of all bins' totals (bucket offsets)
 //   k_bin_scatter  the chunk is held in registers; every membership is
 //                  ranked among the chunk's memberships in bin order and
@@ -70,13 +70,6 @@ __device__ __forceinline__ uint32_t row_prefix_sum(uint32_t v) {
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  return v;
-}
-
-// Two independent inclusive prefix sums, over lanes 0-31 and over lanes 32-63.
-__device__ __forceinline__ uint32_t half_prefix_sum(uint32_t v) {
-  v = row_prefix_sum(v);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
   return v;
 }
 
@@ -220,13 +213,14 @@ __global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restr
 
 // ---------------------------------------------------------------------------
 // Per bin, exclusive scan of its histogram column over the instance's
-// chunks.  A block takes 32 bins (one 128-byte line of a row) and 32 groups
-// of chunks; each thread holds the 32 counts of its chunks in registers, so
-// the table is read once, in whole lines, and the offsets are written from
-// the registers after a prefix over the groups.  Instances of more than
+// chunks.  A block takes 16 bins (half a 128-byte line of a row; the other
+// half is the next block's, an L2 hit) and 64 groups of chunks; each thread
+// holds the 16 counts of its chunks in registers, so the table is read
+// once, and the offsets are written from the registers after a prefix over
+// the groups (one wave per bin, a lane per group).  Instances of more than
 // 1,024 chunks take several tiles, the running total carried.
-constexpr uint32_t kScanBins = 32, kScanGroups = 32, kScanPer = 32;
-static_assert(kScanBins * kScanGroups == 1024 && kScanGroups == 32, "the group prefix is a 32-lane scan per bin");
+constexpr uint32_t kScanBins = 16, kScanGroups = 64, kScanPer = 16;
+static_assert(kScanBins * kScanGroups == 1024 && kScanGroups == 64, "the group prefix is one wave's scan per bin");
 
 __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ hist,
                                                     const InstDesc* __restrict__ inst,
@@ -291,10 +285,10 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
     part[g][bl] = s;
     __syncthreads();
     {  // exclusive prefix over the groups, after the earlier tiles: each
-       // 32-lane half-wave takes one bin, a lane per group
+       // wave takes one bin, a lane per group
       const uint32_t tb = threadIdx.x / kScanGroups, tg = threadIdx.x % kScanGroups;
       const uint32_t x = part[tg][tb];
-      const uint32_t inc = half_prefix_sum(x);
+      const uint32_t inc = wave_prefix_sum(x);
       const uint32_t c = carry[tb];
       part[tg][tb] = c + inc - x;
       __syncthreads();  // every lane has read carry[tb]
