@@ -101,7 +101,6 @@ struct BinnedWorkspace {
 // hist, scan, starts, scatter, test, pair_scan, compact
 constexpr int kBinnedKernels = 7;
 
-size_t binned_test_lds_bytes();
 hipError_t binned_init();
 
 // Enqueue the binned pipeline over the batch described by the workspace's

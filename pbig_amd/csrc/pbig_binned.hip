@@ -1401,7 +1401,6 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 }
 
 // ---------------------------------------------------------------------------
-size_t binned_test_lds_bytes() { return kTestLdsBytes; }
 
 hipError_t binned_init() {
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
