@@ -1239,22 +1239,30 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         // partner k is position k + o1 (k < n1) or k + o3; a lane past its
         // partners reads a clamped position and its bit is masked off
         const uint32_t o1 = b1, o3 = b3 - n1;
-        for (uint32_t k = 0;; ++k) {
-          const uint64_t act = __ballot(k < nall);
-          if (!act) break;
-          const uint32_t qq = min(k + (k < n1 ? o1 : o3), kNarrowMax - 1u);
-          const float4 o = L.box[qq];
-          const uint64_t mask = act & __ballot(a.x <= o.y) & __ballot(o.x <= a.y) & __ballot(a.z <= o.w) &
-                                __ballot(o.z <= a.w);
-          if (mask) {
-            const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            // the ballot itself as the lane mask of the hits
-            if (__builtin_amdgcn_inverse_ballot_w64(mask))
-              L.stage[w][pos] = make_uint2(p, qq);  // positions; ids resolved at the flush
-            sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
-            if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
-          }
+        auto partner = [&](uint32_t k) { return min(k + (k < n1 ? o1 : o3), kNarrowMax - 1u); };
+        auto stage_hits = [&](uint64_t mask, uint32_t qq) {
+          const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+          // the ballot itself as the lane mask of the hits
+          if (__builtin_amdgcn_inverse_ballot_w64(mask))
+            L.stage[w][pos] = make_uint2(p, qq);  // positions; ids resolved at the flush
+          sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
+          if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
+        };
+        // two partners per iteration: two independent test chains per wave
+        for (uint32_t k = 0;; k += 2) {
+          const uint64_t act0 = __ballot(k < nall);
+          if (!act0) break;
+          const uint64_t act1 = __ballot(k + 1u < nall);
+          const uint32_t q0 = partner(k), q1 = partner(k + 1u);
+          const float4 x0 = L.box[q0];
+          const float4 x1 = L.box[q1];
+          const uint64_t m0 = act0 & __ballot(a.x <= x0.y) & __ballot(x0.x <= a.y) & __ballot(a.z <= x0.w) &
+                              __ballot(x0.z <= a.w);
+          const uint64_t m1 = act1 & __ballot(a.x <= x1.y) & __ballot(x1.x <= a.y) & __ballot(a.z <= x1.w) &
+                              __ballot(x1.z <= a.w);
+          if (m0) stage_hits(m0, q0);
+          if (m1) stage_hits(m1, q1);
         }
         uint32_t claim = 0;
         if (lane == 0) claim = atomicAdd(&L.dnext, 64u);
