@@ -201,6 +201,17 @@ int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t 
                           uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
                           uint64_t* n_pairs, void* stream);
 
+/* After a pbig_hip_batch_launch (on this stream) whose error word holds only
+ * the bins-beyond-LDS bit: h_flags[i] = 1 (n_inst bytes) for each instance
+ * that had such a bin, *n_flagged their number.  Every other instance's
+ * (offset, count) entry and pair run from that launch is already complete;
+ * a flagged instance's count is short, and pbig_hip_batch_finish, or a call
+ * on that instance alone, completes it.  Fails when the error word holds
+ * anything else.  No counterpart in the reference (one instance per call,
+ * src/pbig/pbig.h:10); it lets a server hand over the finished instances of
+ * a batch without waiting for the others' host work. */
+int pbig_hip_batch_unresolved(pbig_hip_batch* batch, uint8_t* h_flags, uint32_t* n_flagged, void* stream);
+
 /* Asynchronous batch launch: enqueues the seven kernels and returns; no host
  * synchronisation, no allocation, no memset, so it may be captured in a HIP
  * graph.  d_inst_pairs (optional, device) receives n_inst x {offset, count};

@@ -46,6 +46,7 @@ EXPORTS = (
     "pbig_hip_batch_update",
     "pbig_hip_batch_check",
     "pbig_hip_batch_finish",
+    "pbig_hip_batch_unresolved",
     "pbig_hip_batch_launch",
     "pbig_hip_batch_capture",
     "pbig_hip_batch_replay",
@@ -149,6 +150,7 @@ def load_library(path: str | None = None):
         "pbig_hip_batch_update": (i32, [P, ctypes.POINTER(Instance), u32, P]),
         "pbig_hip_batch_check": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
         "pbig_hip_batch_finish": (i32, [P, P, u64, P, u64, P, ctypes.POINTER(u64), P]),
+        "pbig_hip_batch_unresolved": (i32, [P, P, ctypes.POINTER(u32), P]),
         "pbig_hip_batch_launch": (i32, [P, P, u64, P, u64, P, P, P]),
         "pbig_hip_batch_capture": (i32, [P, P, u64, P, u64, P, P]),
         "pbig_hip_batch_replay": (i32, [P, P]),
@@ -496,6 +498,18 @@ class Batch:
         if rc != PBIG_OK:
             self._eng._raise(rc)
         return int(out.value), io[: self.n_inst]
+
+    def unresolved(self, stream=None) -> np.ndarray:
+        """After a ``launch`` whose error word is only the bins-beyond-LDS bit (8): a bool
+        array, True for the instances that had a bin beyond LDS (their counts
+        are short); the other instances' (offset, count) entries and pairs
+        from that launch are complete."""
+        flags = np.zeros(max(self.n_inst, 1), dtype=np.uint8)
+        n = ctypes.c_uint32(0)
+        rc = self._lib.pbig_hip_batch_unresolved(self._h, flags.ctypes.data, ctypes.byref(n), _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._eng._raise(rc)
+        return flags[: self.n_inst].astype(bool)
 
     def launch(self, boxes, pairs, inst_pairs=None, result=None, stream=None):
         """Enqueue only.  inst_pairs: (k, 2) int64 device tensor; result: (2,)

@@ -1374,6 +1374,41 @@ int pbig_hip_batch_finish(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   return batch_check_impl(b, d_boxes, n_boxes_total, d_pairs, cap, h_inst_pairs, n_pairs, stream, true);
 }
 
+int pbig_hip_batch_unresolved(pbig_hip_batch* b, uint8_t* h_flags, uint32_t* n_flagged, void* stream) {
+  if (!b) return PBIG_E_ARG;
+  pbig_hip_ctx* c = b->ctx;
+  if (!n_flagged) return fail(c, PBIG_E_ARG, "n_flagged is null");
+  const uint32_t n_inst = (uint32_t)b->desc.size();
+  if (n_inst && !h_flags) return fail(c, PBIG_E_ARG, "h_flags is null");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = pick_stream(c, stream);
+  BinnedState& st = b->st;
+  HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const Meta m = *c->h_meta;
+  if (m.error & ~(unsigned long long)pbig::kErrOverfull)
+    return fail(c, PBIG_E_ARG, "the last launch reported more than bins beyond LDS (error word %#llx)", m.error);
+  std::fill(h_flags, h_flags + n_inst, (uint8_t)0);
+  uint32_t nf = 0;
+  const uint32_t n_bins = st.bp.n_bins;
+  const uint32_t n_over = (uint32_t)std::min<uint64_t>(m.max_per_cell, n_bins);
+  if (n_over) {
+    std::vector<uint32_t> over(n_over);
+    HIP_TRY(c, hipMemcpy(over.data(), st.ws.overfull, n_over * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t x : over) {
+      const uint32_t bin = x & ~pbig::kOverfullCell;
+      if (bin >= n_bins) continue;
+      const uint32_t i = st.bin_inst[bin];
+      if (i < n_inst && !h_flags[i]) {
+        h_flags[i] = 1;
+        ++nf;
+      }
+    }
+  }
+  *n_flagged = nf;
+  return PBIG_OK;
+}
+
 int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
                      uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream, bool after_launch) {
   if (!b) return PBIG_E_ARG;

@@ -27,7 +27,12 @@ per launch:
   work (re-binning dense bins, a dense plain request, regrowth) is finished
   by a synchronous call in a worker thread; every slot has its own context,
   so the loop keeps admitting, launching and delivering for the other slots
-  meanwhile.
+  meanwhile.  Under light load (no more requests waiting than slots), when
+  a batch's only trouble is bins beyond LDS, its requests without such a bin
+  are delivered at once and the others are queued again, each to run alone
+  on the plain path (whose dense exit is a short general-path call), so one
+  clustered request does not hold up its batch; under a backlog the batch
+  is finished whole, which costs less device time.
 * Completion.  With ``results="device"`` a request's pairs stay in HBM (a
   view of the slot's pair buffer, valid while the consumer runs); with
   ``results="host"`` they are copied to pinned host memory first.  The
@@ -48,6 +53,8 @@ import numpy as np
 
 import pbig_amd
 from pbig_amd import autotune
+
+_ERR_OVERFULL = 8  # error word: a coarse bin did not fit in LDS (re-binned on the host)
 
 
 @dataclass
@@ -71,6 +78,7 @@ class Request:
     t_done: float = 0.0
     batch_id: int = -1
     _off: int = -1
+    _alone: bool = False             # rerun alone: its batch left it incomplete
 
     @property
     def n(self) -> int:
@@ -194,6 +202,7 @@ class Server:
         self.done: list[Request] = []
         self.n_batches = 0
         self.batch_sizes: list[int] = []
+        self.n_requeued = 0                  # requests a batch left incomplete, rerun alone
 
     # -- client side ---------------------------------------------------------
     def submit(self, req: Request):
@@ -201,6 +210,7 @@ class Server:
             raise ValueError(f"request {req.rid}: {req.n} boxes exceed the pool of {self.pool.capacity}")
         if req.cell_size <= 0:
             req.cell_size = float(autotune.cell_sizes([0, req.n], [req.L], self.table)[0])
+        req._alone = False
         self.queue.append(req)
 
     def close(self):
@@ -261,8 +271,15 @@ class Server:
     def _launch(self, s: _Slot, now: float):
         torch = self.torch
         reqs, rest, total = [], [], 0
+        # Requests a batch left incomplete run alone under light load; under
+        # a backlog they form a batch of their own (finished as a whole), so
+        # they are never split off again.
+        redo = bool(self.admitted) and self.admitted[0]._alone
+        alone = redo and self._light()
         for r in self.admitted:
-            if not reqs or total + r.n <= self.max_batch_boxes:
+            if r._alone != redo or (alone and reqs):
+                rest.append(r)
+            elif not reqs or total + r.n <= self.max_batch_boxes:
                 reqs.append(r)
                 total += r.n
             else:
@@ -349,6 +366,32 @@ class Server:
         s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
         return total
 
+    def _light(self) -> bool:
+        """Light load: no more requests waiting than there are slots."""
+        return len(self.queue) + len(self.admitted) <= len(self.slots)
+
+    def _split(self, s: _Slot) -> bool:
+        """A batch whose only trouble was bins beyond LDS: its requests
+        without such a bin are complete and stay in the slot for delivery;
+        the others go back to the head of the queue, each to run alone on
+        the plain path, instead of the whole batch waiting for their host
+        work.  False when every request is affected."""
+        if any(r._alone for r in s.reqs):
+            return False  # a batch of requests split off before: finished whole
+        flags = s.batch.unresolved(stream=s.stream)
+        if flags.all() or not flags.any():
+            return False
+        io = s.h_io[:len(s.reqs)].clone()
+        keep = [i for i, f in enumerate(flags) if not f]
+        back = [r for r, f in zip(s.reqs, flags) if f]
+        s.h_io[:len(keep)].copy_(io[keep])
+        s.reqs = [s.reqs[i] for i in keep]
+        for r in back:
+            r._alone = True
+        self.admitted[:0] = back
+        self.n_requeued += len(back)
+        return True
+
     def _results_ready(self, s: _Slot, total: int, now: float) -> bool:
         torch = self.torch
         s.total = total
@@ -368,6 +411,9 @@ class Server:
             total, err = (int(v) for v in s.h_res.tolist())
             for r in s.reqs:
                 r.t_ready = now
+            if (err == _ERR_OVERFULL and s.mode == "batch" and len(s.reqs) > 1 and
+                    total <= s.pairs.shape[0] and self._light() and self._split(s)):
+                return self._results_ready(s, total, now)
             if err != 0 or total > s.pairs.shape[0]:
                 s.future = self.pool_exec.submit(self._finish, s, err)
                 s.state = "finishing"
@@ -425,7 +471,9 @@ class Server:
             elif not progressed and any(s.state == "finishing" for s in self.slots):
                 time.sleep(5e-5)  # let a worker thread take the interpreter lock
         t_end = time.perf_counter() - t0
-        return summarize(order, t_end, self.batch_sizes)
+        out = summarize(order, t_end, self.batch_sizes)
+        out["requeued"] = self.n_requeued
+        return out
 
 
 def summarize(reqs: list[Request], t_end: float, batch_sizes=None) -> dict:

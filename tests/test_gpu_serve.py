@@ -112,3 +112,36 @@ def test_serve_device_resident(torch_cuda):
     assert stats["requests"] == 30
     assert all(r.pairs.shape[0] == r.n_pairs for r in reqs)
     check_requests(torch_cuda, reqs)
+
+
+def test_serve_batch_with_overfull_bins_delivers_the_rest(torch_cuda):
+    """Light load: a batch whose clustered request has bins beyond LDS hands
+    over its other requests at once and reruns that one alone; every
+    request's pairs still equal the oracle's and its own run."""
+    from pbig_amd import workloads
+    from pbig_amd.serve import Request, Server
+
+    reqs = []
+    for i in range(10):
+        if i % 3 == 0:
+            n = 150_000
+            b = workloads.clustered(n, 65536.0 * (n / 2097152) ** 0.5, 11 + i)
+        else:
+            n = 5_000 + 3_000 * i
+            b = workloads.uniform(n, 65536.0 * (n / 4194304) ** 0.5, 1.0, 32.0, 11 + i)
+        reqs.append(Request(rid=i, boxes=b, L=65536.0 * (n / 2097152) ** 0.5 if i % 3 == 0
+                            else 65536.0 * (n / 4194304) ** 0.5))
+    srv = Server(hbm_budget=1 << 30, slots=3)
+    try:
+        srv.run(reqs, realtime=False)  # sizes the slots' workspaces
+        for r in reqs:
+            r.pairs = None
+        # arrivals 2 ms apart, in threes: light load, batches of up to three
+        for r in reqs:
+            r.arrival = 0.002 * (r.rid // 3)
+        stats = srv.run(reqs, realtime=True)
+    finally:
+        srv.close()
+    assert stats["requests"] == 10
+    assert stats["requeued"] >= 1, "no batch was left incomplete: the case is not exercised"
+    check_requests(torch_cuda, reqs)
