@@ -29,10 +29,11 @@ per launch:
   so the loop keeps admitting, launching and delivering for the other slots
   meanwhile.  Under light load (no more requests waiting than slots), when
   a batch's only trouble is bins beyond LDS, its requests without such a bin
-  are delivered at once and the others are queued again, each to run alone
-  on the plain path (whose dense exit is a short general-path call), so one
-  clustered request does not hold up its batch; under a backlog the batch
-  is finished whole, which costs less device time.
+  are delivered at once and the others are queued again as a batch of
+  their own (one request alone runs on the plain path, whose dense exit is a
+  short general-path call), so one clustered request does not hold up its
+  batch; under a backlog the batch is finished whole, which costs less
+  device time.
 * Completion.  With ``results="device"`` a request's pairs stay in HBM (a
   view of the slot's pair buffer, valid while the consumer runs); with
   ``results="host"`` they are copied to pinned host memory first.  The
@@ -271,13 +272,12 @@ class Server:
     def _launch(self, s: _Slot, now: float):
         torch = self.torch
         reqs, rest, total = [], [], 0
-        # Requests a batch left incomplete run alone under light load; under
-        # a backlog they form a batch of their own (finished as a whole), so
-        # they are never split off again.
+        # Requests a batch left incomplete form a batch of their own: one
+        # alone runs on the plain path, several are finished as a whole, and
+        # none is split off again.
         redo = bool(self.admitted) and self.admitted[0]._alone
-        alone = redo and self._light()
         for r in self.admitted:
-            if r._alone != redo or (alone and reqs):
+            if r._alone != redo:
                 rest.append(r)
             elif not reqs or total + r.n <= self.max_batch_boxes:
                 reqs.append(r)
@@ -373,9 +373,9 @@ class Server:
     def _split(self, s: _Slot) -> bool:
         """A batch whose only trouble was bins beyond LDS: its requests
         without such a bin are complete and stay in the slot for delivery;
-        the others go back to the head of the queue, each to run alone on
-        the plain path, instead of the whole batch waiting for their host
-        work.  False when every request is affected."""
+        the others go back to the head of the queue, to run as a batch of
+        their own, instead of the whole batch waiting for their host work.
+        False when every request is affected."""
         if any(r._alone for r in s.reqs):
             return False  # a batch of requests split off before: finished whole
         flags = s.batch.unresolved(stream=s.stream)
