@@ -201,13 +201,16 @@ int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t 
                           uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
                           uint64_t* n_pairs, void* stream);
 
-/* After a pbig_hip_batch_launch (on this stream) whose error word holds only
- * the bins-beyond-LDS bit: h_flags[i] = 1 (n_inst bytes) for each instance
- * that had such a bin, *n_flagged their number.  Every other instance's
- * (offset, count) entry and pair run from that launch is already complete;
- * a flagged instance's count is short, and pbig_hip_batch_finish, or a call
- * on that instance alone, completes it.  Fails when the error word holds
- * anything else.  No counterpart in the reference (one instance per call,
+/* After a pbig_hip_batch_launch (or replay, on this stream) whose error word
+ * holds only the bins-beyond-LDS bit and whose pair total fits the launch's
+ * pair_capacity: h_flags[i] = 1 (n_inst bytes) for each instance that had
+ * such a bin, *n_flagged their number.  Every other instance's (offset,
+ * count) entry and pair run from that launch is then complete; a flagged
+ * instance's count is short, and pbig_hip_batch_finish, or a call on that
+ * instance alone, completes it.  Fails with PBIG_E_ARG when the error word
+ * holds anything else, and with PBIG_E_OVERFLOW when the launch found more
+ * pairs than its capacity (the pairs past it were dropped, so later runs
+ * are incomplete: finish with a larger buffer).  No counterpart in the reference (one instance per call,
  * src/pbig/pbig.h:10); it lets a server hand over the finished instances of
  * a batch without waiting for the others' host work. */
 int pbig_hip_batch_unresolved(pbig_hip_batch* batch, uint8_t* h_flags, uint32_t* n_flagged, void* stream);

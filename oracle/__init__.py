@@ -24,6 +24,15 @@ Pinning (each function below is checked in ``tests/test_oracle.py``):
     cell, jobs, compared pairs and 1,121,815,231 pairs).
   * ``brute_u32`` -- the same predicate by brute force; must equal
     ``grid_u32``'s count on in-range inputs.
+  * ``brute_count`` -- closed forms (k identical boxes, a chain, a lattice),
+    the 16k instance's 10,409 pairs and ``len(brute_pairs)`` on random
+    inputs, on one thread and several.
+  * ``grid_counts`` / ``grid_stats`` -- a geometric count (a closed box
+    touches a half-open cell, border cells extended for clamping), on
+    integer boxes that end exactly on cell edges and stick out of the
+    domain; ``grid_pairs``' own statistics must agree.
+  * ``decide_cell_size`` also on constructed inputs where a float mean, or
+    an int sum divided in float, changes the answer (64 against 128).
 """
 from __future__ import annotations
 

@@ -503,7 +503,8 @@ class Batch:
         """After a ``launch`` whose error word is only the bins-beyond-LDS bit (8): a bool
         array, True for the instances that had a bin beyond LDS (their counts
         are short); the other instances' (offset, count) entries and pairs
-        from that launch are complete."""
+        from that launch are complete.  Raises PbigError(PBIG_E_OVERFLOW)
+        when the launch found more pairs than its capacity."""
         flags = np.zeros(max(self.n_inst, 1), dtype=np.uint8)
         n = ctypes.c_uint32(0)
         rc = self._lib.pbig_hip_batch_unresolved(self._h, flags.ctypes.data, ctypes.byref(n), _stream_ptr(stream))

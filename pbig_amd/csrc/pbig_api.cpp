@@ -22,6 +22,7 @@ unsigned int g_n = 0;
 unsigned int g_range = 0;
 unsigned int g_cs = 0;
 std::vector<unsigned int> g_pairs;
+unsigned long long g_last_pairs = 0;  // pairs of the last pbig_check (in g_pairs)
 
 [[noreturn]] void die(const char* what, int rc) {
   printf("ERROR:%s failed (status %d): %s\n", what, rc, g_ctx ? pbig_hip_last_error(g_ctx) : "no context");
@@ -33,7 +34,11 @@ std::vector<unsigned int> g_pairs;
 }  // namespace
 
 report::report() : sum(0) {}
+#if defined(PBIG_COMPAT_U32)
+void report::print() { printf("Number of pairs = %d\n", sum); }
+#else
 void report::print() { printf("Number of pairs = %llu\n", sum); }
+#endif
 void report::found(const RectInfo* a, const RectInfo* b) {
   (void)a;
   (void)b;
@@ -79,6 +84,7 @@ void pbig_check(RectInfo* input, report* r) {
     rc = run(g_pairs.data(), cap);
   }
   if (rc) die("pbig_check", rc);
+  g_last_pairs = np;
   for (unsigned long long k = 0; k < np; ++k) (*r)(g_pairs[2 * k], g_pairs[2 * k + 1]);
 }
 
@@ -89,8 +95,7 @@ void pbig_check_stream(RectInfo* input, std::vector<unsigned long long>* words, 
   {
     report r;
     pbig_check(input, &r);
-    const unsigned long long np = r.count();
-    flat.assign(g_pairs.begin(), g_pairs.begin() + 2 * np);
+    flat.assign(g_pairs.begin(), g_pairs.begin() + 2 * g_last_pairs);
   }
   uint64_t nw = 0, nc = 0, nr = 0;
   const uint64_t np = flat.size() / 2;

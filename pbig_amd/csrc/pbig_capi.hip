@@ -438,6 +438,12 @@ int ensure_state_(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket) {
   }
   const uint64_t scratch = (uint64_t)bp.test_blocks * w.cap_wg;
   if ((rc = grow(c, &w.scratch, &w.cap_scratch, std::max<uint64_t>(scratch, 1)))) return rc;
+  // Bucket offsets, scatter cursors and test ranges are 32-bit: a launch with
+  // more memberships than 2^32-1 exceeds every bucket (the scan flags
+  // kErrWorkspace) and its regrowth stops here.
+  if (bucket > 0xFFFFFFFFull)
+    return fail(c, PBIG_E_NOMEM, "%llu bin memberships exceed the binned path's 2^32-1",
+                (unsigned long long)bucket);
   if (w.cap_bucket < bucket || !w.bbox) {
     if (w.bbox) HIP_TRY(c, hipFree(w.bbox));
     if (w.bid) HIP_TRY(c, hipFree(w.bid));
@@ -458,7 +464,7 @@ int ensure_state_(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket) {
 // are pageable memory.
 int size_state(pbig_hip_ctx* c, BinnedState& st) {
   const uint64_t n = st.bp.n_boxes;
-  const uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, n + n / 4 + 4096);
+  const uint64_t bucket = std::min<uint64_t>(std::max<uint64_t>(st.ws.cap_bucket, n + n / 4 + 4096), 0xFFFFFFFFull);
   const uint64_t per = std::max<uint64_t>(8192, (n + st.bp.test_blocks - 1) / st.bp.test_blocks * 2);
   const uint32_t cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, std::max<uint64_t>(st.ws.cap_wg, per));
   if (cap_wg != st.ws.cap_wg) ++st.generation;
@@ -595,7 +601,8 @@ int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, cons
       if (attempt == 2) return fail(c, PBIG_E_NOMEM, "bin workspace did not fit after regrowth");
       if (m->items > st.ws.cap_wg)
         st.ws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, m->items + m->items / 4 + 1024);
-      const uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, m->memberships + m->memberships / 16 + 4096);
+      uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, m->memberships + m->memberships / 16 + 4096);
+      if (m->memberships <= 0xFFFFFFFFull) bucket = std::min<uint64_t>(bucket, 0xFFFFFFFFull);
       if ((rc = ensure_state(c, st, bucket))) return rc;
       continue;
     }
@@ -1159,6 +1166,7 @@ struct pbig_hip_batch {
   hipGraphExec_t exec = nullptr;
   uint64_t graph_generation = 0;
   hipStream_t capture_stream = nullptr;
+  uint64_t last_cap = 0;            // pair capacity of the last launch (or of the captured graph)
 };
 
 namespace {
@@ -1316,6 +1324,7 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   if ((rc = after_upload(c, b->st, s))) return rc;
   HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
+  b->last_cap = cap;
   if (d_result)
     HIP_TRY(c, pbig::launch_result(b->st.ws.meta, d_result, s));
   return PBIG_OK;
@@ -1347,6 +1356,7 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
   b->graph = g;
   HIP_TRY(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   b->graph_generation = b->st.generation;
+  b->last_cap = cap;
   return PBIG_OK;
 }
 
@@ -1386,8 +1396,14 @@ int pbig_hip_batch_unresolved(pbig_hip_batch* b, uint8_t* h_flags, uint32_t* n_f
   HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   const Meta m = *c->h_meta;
+  *n_flagged = 0;
   if (m.error & ~(unsigned long long)pbig::kErrOverfull)
     return fail(c, PBIG_E_ARG, "the last launch reported more than bins beyond LDS (error word %#llx)", m.error);
+  // Pairs past the launch's capacity were dropped (k_bin_compact keeps only
+  // d < cap), so the runs of the instances there are not complete.
+  if (m.n_pairs > b->last_cap)
+    return fail(c, PBIG_E_OVERFLOW, "the last launch found %llu pairs, more than its capacity of %llu",
+                (unsigned long long)m.n_pairs, (unsigned long long)b->last_cap);
   std::fill(h_flags, h_flags + n_inst, (uint8_t)0);
   uint32_t nf = 0;
   const uint32_t n_bins = st.bp.n_bins;

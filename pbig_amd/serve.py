@@ -352,10 +352,14 @@ class Server:
             view = self.d_pool[r._off:r._off + r.n]
             if err & 16:
                 s.eng.configure(pbig_amd.PATH_GENERAL, int(self.table["bin_target"]))
-            total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream, allow_overflow=True)
-            if total > s.pairs.shape[0]:
-                self._grow_pairs(s, total)
-                total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
+            try:
+                total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream, allow_overflow=True)
+                if total > s.pairs.shape[0]:
+                    self._grow_pairs(s, total)
+                    total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
+            finally:
+                # the slot's later batches choose their paths automatically again
+                s.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
             s.h_io[0, 0] = 0
             s.h_io[0, 1] = total
             return total

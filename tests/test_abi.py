@@ -77,3 +77,18 @@ def test_rect_layout_matches_reference_record():
     assert pbig_amd.RECT_DTYPE.itemsize == 20  # RectInfo, src/pbig/rect.h:15-18 (2-D)
     r = pbig_amd.as_rects(np.arange(8, dtype=np.float32).reshape(2, 4))
     assert r["rid"].tolist() == [0, 1]
+
+
+def test_report_layout_per_build(tmp_path):
+    """libpbig_u32.so is the reference's shipped build: its report object
+    keeps the reference's layout (one int, src/pbig/report.h:15), so a
+    caller compiled against the reference's header can link to it.  The
+    float build's report holds a 64-bit counter.  (The header checks both
+    with static_assert; this compiles it each way.)"""
+    import subprocess
+
+    src = tmp_path / "r.cpp"
+    src.write_text('#include "report.h"\nint size_of_report() { return (int)sizeof(report); }\n')
+    inc = os.path.join(ROOT, "include")
+    for flags in (["-DPBIG_COMPAT_U32", "-DDIMS=3"], ["-DPBIG_COMPAT_U32", "-DDIMS=2"], []):
+        subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{inc}", *flags, str(src)], check=True)

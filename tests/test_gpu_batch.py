@@ -233,24 +233,48 @@ def test_batch_rejects_bad_arguments(torch_cuda, eng):
 
 @pytest.mark.slow
 def test_batch_c4_full(torch_cuda, eng):
-    """The 4,096-instance batch (129M boxes): every instance against the
-    oracle's grid restatement at the same cell size."""
+    """The 4,096-instance batch (120M boxes) as the bench runs it: cell sizes
+    from the autotuner's table, the table's bin target, one HIP-graph capture
+    replayed twice (with other work between the replays); and once more
+    through the synchronous entry point at cell 32.  Every instance's sorted
+    pair set against the oracle's grid restatement, bit for bit."""
     import oracle
     import pbig_amd
-    from pbig_amd import workloads
+    from pbig_amd import autotune, workloads
 
     boxes, offsets, ranges = workloads.batched(4096, workloads.SEEDS["c4"])
+    table = autotune.load_table()
+    cells = autotune.cell_sizes(offsets, ranges, table)
+    eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
     d = to_dev(torch_cuda, boxes)
     cap = int(0.2 * boxes.shape[0])
     pairs = torch_cuda.empty((cap, 2), dtype=torch_cuda.int32, device="cuda:0")
+    io_dev = torch_cuda.zeros((4096, 2), dtype=torch_cuda.int64, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    runs = []
+    with pbig_amd.Batch(eng, offsets, ranges.astype(np.float64), cells) as b:
+        n0, io0 = b.check(d, pairs)  # sizes the workspace
+        b.capture(d, pairs, io_dev, res)
+        for rep in range(2):
+            pairs.fill_(-1)
+            io_dev.zero_()
+            torch_cuda.cuda.synchronize()
+            b.replay()
+            torch_cuda.cuda.synchronize()
+            assert res.tolist() == [n0, 0], rep
+            io = io_dev.cpu().numpy()
+            assert np.array_equal(io.astype(np.uint64), io0)
+            runs.append((pairs[:n0].cpu().numpy().view(np.uint32).copy(), io))
+    eng.configure(pbig_amd.PATH_AUTO)
     with pbig_amd.Batch(eng, offsets, ranges.astype(np.float64), 32.0) as b:
         n, io = b.check(d, pairs)
-    got = pairs[:n].cpu().numpy().view(np.uint32)
+        runs.append((pairs[:n].cpu().numpy().view(np.uint32).copy(), io))
     for i in range(4096):
         sub = boxes[offsets[i]:offsets[i + 1]]
         w = oracle.sort_pairs(oracle.grid_pairs(sub, float(ranges[i]), 32, threads=16))
-        g = oracle.sort_pairs(got[int(io[i, 0]):int(io[i, 0] + io[i, 1])])
-        assert np.array_equal(g, w), i
+        for k, (got, io) in enumerate(runs):
+            g = oracle.sort_pairs(got[int(io[i, 0]):int(io[i, 0] + io[i, 1])])
+            assert np.array_equal(g, w), (i, k)
 
 
 @pytest.mark.parametrize("path", ["binned", "auto"])
@@ -313,3 +337,42 @@ def test_batch_finish_after_launch(torch_cuda, eng):
         n, io = b.check(d, pairs, finish=True)
         assert n == total
         check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+
+
+def test_batch_unresolved_refuses_after_overflow(torch_cuda, eng):
+    """A launch whose only error is bins beyond LDS but whose pairs exceed the
+    launch's capacity: the pairs past the capacity were dropped, so
+    unresolved() must not vouch for the other instances' runs."""
+    import oracle
+    import pbig_amd
+    from pbig_amd import workloads
+
+    eng.configure(pbig_amd.PATH_AUTO)
+    inst = [(workloads.uniform(50000, 3000.0, 1.0, 8.0, 311), 3000.0),
+            (workloads.clustered(150000, 17000.0, 310), 17000.0)]
+    boxes, offsets, ranges = concat(inst)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    big = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    small = torch_cuda.empty((want[0].shape[0] // 2, 2), dtype=torch_cuda.int32, device="cuda:0")
+    with pbig_amd.Batch(eng, offsets, ranges, [13.0, 40.0]) as b:
+        b.check(d, big)  # sizes the workspace
+        res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+        b.launch(d, small, None, res)
+        torch_cuda.cuda.synchronize()
+        n_launch, err = res.tolist()
+        assert err == 8 and n_launch > small.shape[0]
+        with pytest.raises(pbig_amd.PbigError) as e:
+            b.unresolved()
+        assert e.value.status == pbig_amd.PBIG_E_OVERFLOW
+        # with room for the launch's pairs, the first instance is complete
+        io = torch_cuda.zeros((2, 2), dtype=torch_cuda.int64, device="cuda:0")
+        b.launch(d, big, io, res)
+        torch_cuda.cuda.synchronize()
+        assert res[1].item() == 8
+        flags = b.unresolved()
+        assert flags.tolist() == [False, True]
+        o, c = (int(v) for v in io[0].tolist())
+        got = big[o:o + c].cpu().numpy().view(np.uint32)
+        check_all([oracle.sort_pairs(got)], want[:1])

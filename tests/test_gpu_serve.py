@@ -145,3 +145,32 @@ def test_serve_batch_with_overfull_bins_delivers_the_rest(torch_cuda):
     assert stats["requests"] == 10
     assert stats["requeued"] >= 1, "no batch was left incomplete: the case is not exercised"
     check_requests(torch_cuda, reqs)
+
+
+def test_serve_c5_stream_full_sizes(torch_cuda):
+    """The serving configuration at its stated request sizes: the first 16
+    requests of the Poisson stream the bench serves (seed 5, n log-uniform
+    in [1,000, 1,000,000], 50/50 uniform and clustered), arriving at 2,000
+    requests/s, HBM budget 8 GB.  Three requests hold more than 500,000
+    boxes.  Every request's pairs equal the oracle's sweep-and-prune set and
+    its own plain-path run."""
+    from pbig_amd import autotune, workloads
+    from pbig_amd.serve import Request, Server
+
+    table = autotune.load_table()
+    spec = workloads.serving_requests(400, workloads.SEEDS["c5"], 2000.0)[:16]
+    reqs = []
+    for q in spec:
+        b = workloads.request_boxes(q)
+        reqs.append(Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
+                            cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table)[0])))
+    assert sum(r.n > 500_000 for r in reqs) >= 2
+    assert {q["kind"] for q in spec} == {"uniform", "clustered"}
+    assert max(r.n for r in reqs) > 900_000
+    srv = Server(hbm_budget=8 << 30, slots=3, table=table)
+    try:
+        stats = srv.run(reqs, realtime=True)
+    finally:
+        srv.close()
+    assert stats["requests"] == 16
+    check_requests(torch_cuda, reqs)

@@ -182,3 +182,128 @@ def test_planted_mistakes_are_caught(orc):
     # dropping the y tests would add pairs
     xo = (x0[:, None] <= x1[None, :]) & (x0[None, :] <= x1[:, None])
     assert int(np.triu(xo[:2000, :2000], 1).sum()) > int(((i < 2000) & (j < 2000)).sum())
+
+
+def test_brute_count_pinned(orc):
+    """orc_brute2f_count (the timed brute-force checker) against closed forms,
+    the golden 16k count and the pair list it counts."""
+    from pbig_amd import workloads
+
+    assert orc.brute_count(np.tile(box(1, 2, 1, 2), (37, 1))) == 37 * 36 // 2
+    assert orc.brute_count(np.stack([box(i, i + 1, 0, 1) for i in range(20)])) == 19
+    g = [box(3 * i, 3 * i + 1, 3 * j, 3 * j + 1) for i in range(10) for j in range(10)]
+    assert orc.brute_count(np.stack(g)) == 0
+    assert orc.brute_count(np.zeros((0, 4), np.float32)) == 0
+    rng = np.random.default_rng(11)
+    for n, L, smax in ((700, 64.0, 6.0), (1500, 300.0, 40.0)):
+        b = _random_boxes(rng, n, L, 0.5, smax)
+        want = len(orc.brute_pairs(b))
+        assert orc.brute_count(b, threads=1) == want
+        assert orc.brute_count(b, threads=4) == want
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        man = json.load(f)["c1_uniform_16k"]
+    b1, _ = workloads.c1_small_uniform()
+    assert orc.brute_count(b1) == man["pairs"] == 10409
+
+
+def _touch_counts(b, L, cs):
+    """Cell membership counts by geometry, independent of the floor map: with
+    nrc = ceil(L / cs) columns, column c covers [c cs, (c+1) cs), the first one
+    extended to -inf and the last to +inf (clamping); a closed box [x0, x1]
+    touches column c iff x0 < hi(c) and x1 >= lo(c).  Row-major, nrc*cy + cx."""
+    nrc = max(1, int(np.ceil(L / cs)))
+    lo = np.arange(nrc, dtype=np.float64) * cs
+    hi = lo + cs
+    lo[0], hi[-1] = -np.inf, np.inf
+    x = (b[:, 0:1] < hi[None, :]) & (b[:, 1:2] >= lo[None, :])
+    y = (b[:, 2:3] < hi[None, :]) & (b[:, 3:4] >= lo[None, :])
+    return (y.T.astype(np.int64) @ x.astype(np.int64)).ravel(), nrc
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_grid_stats_and_counts_pinned(orc, case):
+    """orc_grid2f_counts and orc_grid2f_stats against the geometric count:
+    integer-valued boxes land on cell edges (a box ending exactly on an edge
+    touches the next cell), some boxes stick out of the domain on every side
+    (clamped into the border cells), and L is not a multiple of the cell."""
+    rng = np.random.default_rng(100 + case)
+    L, cs, n = [(100.0, 4, 900), (1000.0, 64, 3000), (257.0, 16, 2000), (50.0, 8, 400)][case]
+    c = rng.integers(-8, int(L) + 8, size=(n, 2)).astype(np.float32)
+    w = rng.integers(0, [3, 90, 20, 30][case], size=(n, 2)).astype(np.float32)
+    b = np.stack([c[:, 0], c[:, 0] + w[:, 0], c[:, 1], c[:, 1] + w[:, 1]], 1).astype(np.float32)
+    b[: n // 10] = np.tile(box(5, 6, 5, 6), (n // 10, 1))  # one crowded cell
+    want, nrc = _touch_counts(b, L, cs)
+    got = orc.grid_counts(b, L, cs)
+    assert got.shape == (nrc * nrc,)
+    assert np.array_equal(got.astype(np.int64), want)
+    st = orc.grid_stats(b, L, cs)
+    assert st["n_cells"] == nrc * nrc
+    assert st["memberships"] == int(want.sum())
+    assert st["max_per_cell"] == int(want.max()) and st["max_cell"] == int(np.argmax(want))
+    assert st["nonempty_cells"] == int((want > 0).sum())
+    k = want[want > 1]
+    assert st["compared_pairs"] == int((k * k // 2).sum())
+    assert st["jobs256"] == int(((k + 255) // 256).sum())
+    # the grid pipeline reports the same binning statistics with its pairs
+    pairs, st2 = orc.grid_pairs(b, L, cs, with_stats=True)
+    assert st2["pairs"] == len(pairs)
+    assert {k2: v for k2, v in st2.items() if k2 != "pairs"} == {k2: v for k2, v in st.items() if k2 != "pairs"}
+
+
+def _extent_boxes(ex, ey, n):
+    """n boxes whose first len(ex) x/y extents are given, spread over [0, 1024)."""
+    b = np.zeros((n, 4), np.float32)
+    xs = (np.arange(n) * 37 % 1000).astype(np.float32)
+    ys = (np.arange(n) * 91 % 1000).astype(np.float32)
+    ex = np.resize(np.asarray(ex, np.float32), n)
+    ey = np.resize(np.asarray(ey, np.float32), n)
+    b[:, 0], b[:, 1], b[:, 2], b[:, 3] = xs, xs + ex, ys, ys + ey
+    return b
+
+
+def _rule_with(b, n, range_, accumulate):
+    """The cell-size loop of pbig.cu:1316-1331 with the mean extent given by
+    `accumulate` (for planting mistakes in the mean)."""
+    x, y = accumulate(b[:1000, 1] - b[:1000, 0]), accumulate(b[:1000, 3] - b[:1000, 2])
+    cs = 1
+    while cs * 2 < range_:
+        cs *= 2
+    while True:
+        nrc = int(np.ceil(range_ / cs))
+        nmr = float(np.float32((1 + np.float32(x) / np.float32(cs)) * (1 + np.float32(y) / np.float32(cs))))
+        per_cell = int(nmr * n / (nrc * nrc))
+        if cs < x or per_cell < 32 or nmr > 4 or cs == 1:
+            break
+        cs //= 2
+    return cs * 2
+
+
+def test_decide_cell_size_int_truncated_mean(orc):
+    """pbig.cu:1303-1308 accumulates the extents with an int init value
+    (each step truncates) and divides by the sample count in integers.  On
+    these inputs a float mean, or an int sum with a float division, crosses
+    the cell side 64 and gives 128; the reference's arithmetic gives 64."""
+    import pbig_amd
+
+    n = 4096
+    # x: 999 extents of 65.5 (each step truncates: +65) and one of 64: int sum
+    # 64,999 -> 64 by integer division, float mean 65.4985, float division
+    # 64.999; the loop stops where cs < x (pbig.cu:1322).
+    # y (x extents 0): 999 of 193.5 and one of 192 -> 192 against 193.4985 or
+    # 192.999; at cs 64, nmr = 1 + y/64 is exactly 4 (go on) or above (stop).
+    ex = [65.5] * 999 + [64.0]
+    ey = [193.5] * 999 + [192.0]
+    for bx, by in ((ex, [8.0]), ([0.0], ey)):
+        b = _extent_boxes(bx, by, n)
+        assert orc.decide_cell_size(b, 1024.0) == 64
+        assert pbig_amd.reference_cell_size(b, 1024.0) == 64
+
+        def int_trunc(e):
+            acc = 0
+            for v in e:
+                acc = int(np.float32(acc) + np.float32(v))
+            return float(acc // len(e))
+
+        assert _rule_with(b, n, 1024.0, int_trunc) == 64
+        assert _rule_with(b, n, 1024.0, lambda e: float(np.mean(e.astype(np.float64)))) == 128
+        assert _rule_with(b, n, 1024.0, lambda e: sum(int(np.float32(v)) for v in e) / len(e)) == 128
