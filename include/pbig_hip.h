@@ -115,6 +115,26 @@ int pbig_hip_check_device_async(pbig_hip_ctx* ctx, const float* d_boxes, uint32_
                                 float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
                                 uint64_t* d_result, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the asynchronous plain path on the binned pipeline
+ * (same kernels and result words as pbig_hip_check_device_async), with its
+ * whole workspace in a fixed 48 MiB device array that the library holds on
+ * each device until the process ends.  It calls no hipMalloc, hipFree or
+ * memset; the caller passes the boxes, the pair buffer and d_result (2 x
+ * uint64 device words: pair count, error word).  n up to about 2^20 boxes
+ * (the exact need depends on the plan; a larger call returns PBIG_E_ARG
+ * before launching anything); cell_size must be > 0.  The error word's bit
+ * 1 (more memberships or pairs per test workgroup than the array holds) or
+ * bit 3 (a coarse bin beyond LDS) means the result is incomplete: use the
+ * allocating entry points for that input.  Every call on one device shares
+ * that device's array and leaves its contents behind (each launch clears
+ * what it reads before use); calls that overlap on one device, from any
+ * context or stream, race. */
+int pbig_hip_check_device_noalloc(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
+                                  float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
+                                  uint64_t* d_result, void* stream);
+
 /* Grow the context's workspace for up to n_boxes boxes, n_cells cells and
  * n_memberships cell memberships (used by the async path). */
 int pbig_hip_reserve(pbig_hip_ctx* ctx, uint64_t n_boxes, uint64_t n_cells, uint64_t n_memberships);

@@ -35,6 +35,7 @@ EXPORTS = (
     "pbig_hip_reference_cell_size",
     "pbig_hip_check_device",
     "pbig_hip_check_device_async",
+    "pbig_hip_check_device_noalloc",
     "pbig_hip_reserve",
     "pbig_hip_check_host",
     "pbig_hip_last_stats",
@@ -138,6 +139,7 @@ def load_library(path: str | None = None):
         "pbig_hip_reference_cell_size": (u32, [P, u32, f64]),
         "pbig_hip_check_device": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64), P]),
         "pbig_hip_check_device_async": (i32, [P, P, u32, f64, f32, P, u64, P, P]),
+        "pbig_hip_check_device_noalloc": (i32, [P, P, u32, f64, f32, P, u64, P, P]),
         "pbig_hip_reserve": (i32, [P, u64, u64, u64]),
         "pbig_hip_check_host": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64)]),
         "pbig_hip_last_stats": (i32, [P, ctypes.POINTER(Stats)]),
@@ -335,6 +337,18 @@ class Engine:
         rc = self._lib.pbig_hip_check_device_async(self._h, _ptr(boxes) if n else None, n, float(range_),
                                                    float(cell_size), _ptr(pairs), cap, _ptr(result),
                                                    _stream_ptr(stream))
+        if rc != PBIG_OK:
+            self._raise(rc)
+
+    def check_device_noalloc(self, boxes, range_: float, cell_size: float, pairs, result, stream=None):
+        """Enqueue only, with no device allocation: the library's fixed scratch
+        array holds the workspace (up to about 2^20 boxes).  result: (2,)
+        int64 device tensor <- {n_pairs, error}."""
+        n = int(boxes.shape[0])
+        cap = int(pairs.shape[0])
+        rc = self._lib.pbig_hip_check_device_noalloc(self._h, _ptr(boxes) if n else None, n, float(range_),
+                                                     float(cell_size), _ptr(pairs), cap, _ptr(result),
+                                                     _stream_ptr(stream))
         if rc != PBIG_OK:
             self._raise(rc)
 

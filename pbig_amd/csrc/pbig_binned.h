@@ -137,6 +137,11 @@ hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, 
 // to the caller's two device words by one small kernel.
 hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s);
 
+// Tables of a launch of one instance written by a kernel from its argument:
+// the instance record, and zero in the chunk -> instance and bin -> instance
+// tables (no copy and no memset, so the sequence may be captured).
+hipError_t launch_single_tables(const BinnedWorkspace& ws, const InstDesc& d, hipStream_t s);
+
 }  // namespace pbig
 
 #endif  // PBIG_BINNED_H_

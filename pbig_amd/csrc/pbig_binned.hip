@@ -1427,6 +1427,22 @@ hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_single_tables(InstDesc d, InstDesc* __restrict__ inst,
+                                                       uint32_t* __restrict__ chunk_inst,
+                                                       uint32_t* __restrict__ bin_inst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) inst[0] = d;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.nchunks; i += stride) chunk_inst[i] = 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.nb; i += stride) bin_inst[i] = 0u;
+}
+
+hipError_t launch_single_tables(const BinnedWorkspace& ws, const InstDesc& d, hipStream_t s) {
+  const uint32_t m = d.nchunks > d.nb ? d.nchunks : d.nb;
+  const uint32_t blocks = m ? (m + 255u) / 256u : 1u;
+  k_single_tables<<<blocks < 64u ? blocks : 64u, 256, 0, s>>>(d, ws.inst, ws.chunk_inst, ws.bin_inst);
+  return hipGetLastError();
+}
+
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
                                 uint64_t* inst_out, hipStream_t s) {
   const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;

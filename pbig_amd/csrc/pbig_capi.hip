@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -83,6 +84,7 @@ struct pbig_hip_ctx {
   InstDesc* h_ring = nullptr;
   hipEvent_t ring_ev[16] = {};
   uint32_t ring_next = 0;
+  unsigned char* noalloc = nullptr;  // this device's address of the no-allocation scratch
   pbig_hip_config cfg{PBIG_PATH_AUTO, 2000, 0, 0};
   int last_path = 0;
   pbig_hip_stats stats{};
@@ -96,6 +98,32 @@ struct pbig_hip_ctx {
 
 namespace {
 constexpr int kEventsPerCall = 8;
+
+// Device memory of pbig_hip_check_device_noalloc: the whole workspace of a
+// single-instance binned launch, carved per call from this array.  Each
+// device that loads the library holds its own copy for the life of the
+// process.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device (noalloc_setup).
+constexpr uint64_t kNoAllocBytes = 48ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kMaxDevices] = {};
+
+// The scratch array's address on `device`, which must be current.
+hipError_t noalloc_setup(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = (unsigned char*)p;
+  }
+  *out = g_noalloc_ptr[device];
+  return hipSuccess;
+}
 }
 
 namespace {
@@ -890,6 +918,7 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
   if (hipGetDeviceProperties(&prop, device_id) != hipSuccess) { delete c; return PBIG_E_HIP; }
   c->n_cu = prop.multiProcessorCount;
   if (pbig::binned_init() != hipSuccess) { delete c; return PBIG_E_HIP; }
+  if (noalloc_setup(device_id, &c->noalloc) != hipSuccess) { delete c; return PBIG_E_HIP; }
   c->ws.test_blocks = (uint32_t)c->n_cu * 8;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PBIG_E_HIP; }
   if (hipMalloc((void**)&c->ws.meta, sizeof(Meta)) != hipSuccess) { delete c; return PBIG_E_NOMEM; }
@@ -1039,6 +1068,98 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
   }
   if (d_result)
     HIP_TRY(c, pbig::launch_result(c->ws.meta, d_result, s));
+  return PBIG_OK;
+}
+
+int pbig_hip_check_device_noalloc(pbig_hip_ctx* c, const float* d_boxes, uint32_t n, double range,
+                                  float cell_size, uint32_t* d_pairs, uint64_t cap, uint64_t* d_result,
+                                  void* stream) {
+  if (!c) return PBIG_E_ARG;
+  if (n > pbig::kMaxBoxes) return fail(c, PBIG_E_ARG, "n = %u exceeds the limit of 2^30-1 boxes", n);
+  if (n && !d_boxes) return fail(c, PBIG_E_ARG, "d_boxes is null");
+  if (((uintptr_t)d_boxes & 15) != 0) return fail(c, PBIG_E_ARG, "d_boxes must be 16-byte aligned");
+  if (cap && !d_pairs) return fail(c, PBIG_E_ARG, "d_pairs is null with a nonzero capacity");
+  if (!d_result) return fail(c, PBIG_E_ARG, "d_result is null");
+  if (!(cell_size > 0.0f)) return fail(c, PBIG_E_ARG, "the no-allocation path needs cell_size > 0");
+  if (!c->noalloc) return fail(c, PBIG_E_HIP, "no scratch address for device %d", c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = pick_stream(c, stream);
+  GridParams gp;
+  uint32_t n_cells;
+  int rc;
+  if ((rc = grid_of(c, range, cell_size, &gp, &n_cells))) return rc;
+  InstDesc d;
+  if (!plan_instance(n, range, cell_size, c->cfg.bin_target, &d, (uint32_t)c->n_cu))
+    return fail(c, PBIG_E_ARG, "the binned path cannot plan this grid (more than %u bins)", pbig::kMaxBins);
+  // The plan of a batch of one (what layout_tables computes), with no host
+  // tables: k_single_tables writes them on the device.
+  BatchPlan bp;
+  bp.n_inst = 1;
+  bp.n_chunks = d.nchunks;
+  bp.n_bins = n ? d.nb : 0u;
+  bp.n_boxes = n;
+  bp.nb_max = d.nchunks ? d.nb : 0u;
+  bp.hist_entries = (uint64_t)d.nchunks * pbig::hist_pitch(d.nb);
+  bp.off_entries = (uint64_t)d.nchunks * pbig::off_pitch(d.nb);
+  bp.test_blocks = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)c->n_cu, std::max<uint32_t>(bp.n_bins, 1)));
+  bp.n_cu = (uint32_t)c->n_cu;
+  bp.direct = 1u;
+  bp.dense_exit = 0u;
+  // Carve the workspace: every array at a 256-byte boundary; the pair
+  // regions sized as the allocating path sizes them first; the rest of the
+  // array is the bucket (20 bytes per membership, at least 1.25 per box).
+  uint64_t at = 0;
+  auto take = [&](uint64_t bytes) {
+    const uint64_t p = at;
+    at = (at + bytes + 255u) & ~255ull;
+    return p;
+  };
+  const uint64_t nb1 = (uint64_t)bp.n_bins + 1;
+  const uint64_t o_inst = take(sizeof(InstDesc)), o_meta = take(sizeof(Meta));
+  const uint64_t o_ci = take(4ull * std::max<uint32_t>(bp.n_chunks, 1)), o_bi = take(4ull * nb1);
+  const uint64_t o_hist = take(4 * std::max<uint64_t>(bp.hist_entries, 1));
+  const uint64_t o_off = take(4 * std::max<uint64_t>(bp.off_entries, 1));
+  const uint64_t o_total = take(4 * nb1), o_start = take(4 * nb1), o_over = take(4 * nb1);
+  const uint64_t o_seg = take(8 * nb1), o_out = take(8 * nb1);
+  const uint64_t cap_wg = std::max<uint64_t>(8192, ((uint64_t)n + bp.test_blocks - 1) / bp.test_blocks * 2);
+  const uint64_t o_reg = take(8 * cap_wg * bp.test_blocks);
+  const uint64_t want_bucket = (uint64_t)n + n / 4 + 4096;
+  const uint64_t room = at < kNoAllocBytes ? (kNoAllocBytes - at) / 20u : 0u;
+  if (room < want_bucket)
+    return fail(c, PBIG_E_ARG,
+                "%u boxes need %llu bytes of scratch; the no-allocation entry point holds %llu (about 2^20 boxes)",
+                n, (unsigned long long)(at + 20 * want_bucket), (unsigned long long)kNoAllocBytes);
+  const uint64_t cap_bucket = std::min<uint64_t>(room & ~63ull, 0xFFFFFFFFull);
+  const uint64_t o_bbox = take(16 * cap_bucket), o_bid = take(4 * cap_bucket);
+  unsigned char* base = c->noalloc;
+  pbig::BinnedWorkspace ws;
+  ws.inst = (InstDesc*)(base + o_inst);
+  ws.meta = (Meta*)(base + o_meta);
+  ws.chunk_inst = (uint32_t*)(base + o_ci);
+  ws.bin_inst = (uint32_t*)(base + o_bi);
+  ws.hist = (uint32_t*)(base + o_hist);
+  ws.off = (uint32_t*)(base + o_off);
+  ws.total = (uint32_t*)(base + o_total);
+  ws.bin_start = (uint32_t*)(base + o_start);
+  ws.overfull = (uint32_t*)(base + o_over);
+  ws.bin_seg = (uint2*)(base + o_seg);
+  ws.bin_out = (uint64_t*)(base + o_out);
+  ws.scratch = (uint2*)(base + o_reg);
+  ws.cap_wg = (uint32_t)cap_wg;
+  ws.cap_scratch = cap_wg * bp.test_blocks;
+  ws.bbox = (float4*)(base + o_bbox);
+  ws.bid = (uint32_t*)(base + o_bid);
+  ws.cap_bucket = cap_bucket;
+  ws.cap_hist = std::max<uint64_t>(bp.hist_entries, 1);
+  ws.cap_off = std::max<uint64_t>(bp.off_entries, 1);
+  ws.cap_bins = nb1;
+  ws.cap_inst = 1;
+  ws.cap_chunks = std::max<uint32_t>(bp.n_chunks, 1);
+  ws.cap_bin_inst = nb1;
+  d.box0 = 0;
+  HIP_TRY(c, pbig::launch_single_tables(ws, d, s));
+  HIP_TRY(c, pbig::launch_binned(ws, (const float4*)d_boxes, nullptr, bp, (uint2*)d_pairs, cap, nullptr, s, nullptr));
+  HIP_TRY(c, pbig::launch_result(ws.meta, d_result, s));
   return PBIG_OK;
 }
 
