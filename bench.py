@@ -559,10 +559,12 @@ def main():
     # B_alg terms by kernel (the binned kernels do the same four jobs)
     # (bin_starts moves a few KB: counted in bin_scan; the single-instance test kernel writes the
     # output itself, so it carries the 8P term)
+    # (the single-pass launch's place kernel does the count, scan and scatter
+    # jobs in one pass over the boxes)
     kernel_alg = {"count": B["count"], "scatter": B["scatter"], "test": B["test"],
                   "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref,
                   "bin_hist": B["count"], "bin_scan": B["scan"], "bin_starts": 0.0, "bin_scatter": B["scatter"],
-                  "bin_test": B["test"]}
+                  "bin_test": B["test"], "bin_place": B["count"] + B["scan"] + B["scatter"]}
     dominant = max(avg, key=lambda k: avg[k])
     traffic, traffic_src = read_traffic(path_name, cs, args.bin_target)
     achieved = kernel_alg[dominant] / (avg[dominant] / 1e3) / 1e9

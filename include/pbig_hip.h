@@ -107,10 +107,11 @@ int pbig_hip_check_device(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, d
  * d_result[1] an error word (0 = ok; bit 0 invalid box, bit 1 workspace too
  * small, bit 2 cell too dense, bit 3 a coarse bin did not fit in LDS, bit 4
  * the instance is mostly dense clusters, which the automatic path choice
- * sends to the general path).  d_result is 2 x uint64 in device memory.  The
- * path (and the workspace size) follow the context's last synchronous call;
- * when bit 1, 3 or 4 is set, run the synchronous entry point once and call
- * again. */
+ * sends to the general path, bit 5 a coarse bin outgrew its fixed share of
+ * the single-pass launch's bucket).  d_result is 2 x uint64 in device memory.
+ * The path (and the workspace size) follow the context's last synchronous
+ * call; when bit 1, 3, 4 or 5 is set, run the synchronous entry point once
+ * (it settles the path for this instance shape) and call again. */
 int pbig_hip_check_device_async(pbig_hip_ctx* ctx, const float* d_boxes, uint32_t n, double range,
                                 float cell_size, uint32_t* d_pairs, uint64_t pair_capacity,
                                 uint64_t* d_result, void* stream);
@@ -162,7 +163,8 @@ int pbig_hip_configure(pbig_hip_ctx* ctx, const pbig_hip_config* cfg);
  * kernel in launch order, the number of calls recorded and the path they
  * took (n_kernels >= 7).  GENERAL: clear, count, scan_reduce, scan_apply,
  * scatter, test.  BINNED: hist, scan, starts, scatter, test, pair_scan,
- * compact.  Batch launches are timed the same way (path BINNED). */
+ * compact.  Path 3, the binned path's single-pass launch of one instance:
+ * place, test.  Batch launches are timed as BINNED. */
 int pbig_hip_timing_start(pbig_hip_ctx* ctx, int max_calls);
 int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls, int* path);
 

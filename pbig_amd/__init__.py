@@ -60,12 +60,16 @@ EXPORTS = (
 )
 
 PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
-PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned"}
+# pbig_hip_timing_read also reports the binned path's single-pass launch of
+# one instance (no histogram or scan pass) as path 3.
+TIMING_ONEPASS = 3
+PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned", TIMING_ONEPASS: "onepass"}
 # Kernels of each path in launch order (pbig_hip_timing_read).
 KERNELS = {
     PATH_GENERAL: ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test"),
     PATH_BINNED: ("bin_hist", "bin_scan", "bin_starts", "bin_scatter", "bin_test", "bin_pair_scan",
                   "bin_compact"),
+    TIMING_ONEPASS: ("bin_place", "bin_test"),
 }
 
 # numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)

@@ -23,6 +23,8 @@ constexpr uint32_t kScatterChunk = 4096;     // boxes per histogram/scatter work
 constexpr unsigned long long kErrOverfull = 8ull;  // a coarse bin did not fit in LDS
 constexpr unsigned long long kErrDense = 16ull;    // heavy bins hold over a quarter of the memberships
                                                    // (checked only when the plan asks: dense_exit)
+constexpr unsigned long long kErrBinCap = 32ull;   // single-pass launch: a bin's memberships exceed its fixed
+                                                   // share of the bucket (the two-pass launch handles it)
 // Overfull list entries: the bin, with this bit when a single fine cell
 // exceeded kMaxFine (re-binning cannot help; the instance needs the general
 // path) rather than the bin exceeding kMaxCoarse memberships.
@@ -136,6 +138,28 @@ hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, 
 // The pair count and error word of a call (meta's first two words) copied
 // to the caller's two device words by one small kernel.
 hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s);
+
+// Single-pass launch of one instance (no histogram or scan pass): every bin
+// owns bin_cap consecutive bucket entries (bin b at b * bin_cap); each place
+// workgroup counts its boxes' memberships per bin in LDS and reserves its
+// share of every bin with one device atomic on fill[b].  The counters and
+// result words alternate between two sets from call to call: a call zeroes
+// the other set (fill_next, meta_next) for the next one.
+struct OnePass {
+  uint32_t* fill = nullptr;       // per bin: memberships reserved (this call)
+  uint32_t* fill_next = nullptr;  // the next call's counters, zeroed by this one
+  uint32_t fill_words = 0;        // entries of each fill array
+  uint32_t* bin_start = nullptr;  // per bin: b * bin_cap
+  uint32_t bin_cap = 0;
+  Meta* meta = nullptr;           // this call's result words
+  Meta* meta_next = nullptr;      // the next call's, zeroed by this one
+};
+
+constexpr int kOnePassKernels = 2;  // place, test
+
+hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, const float4* boxes,
+                                 const BatchPlan& bp, uint2* out, uint64_t cap, hipStream_t s,
+                                 hipEvent_t* events = nullptr);
 
 // Tables of a launch of one instance written by a kernel from its argument:
 // the instance record, and zero in the chunk -> instance and bin -> instance

@@ -595,6 +595,239 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
 }
 
 // ---------------------------------------------------------------------------
+// Single-pass placement of one instance's boxes into fixed-capacity bins.
+//
+// One persistent 1024-thread workgroup per CU takes a contiguous range of
+// boxes, 16,384 at a time (16 per thread, held in registers):
+//   1. counts the round's memberships per coarse bin with LDS atomics: a
+//      box's first bin returns the box's rank there (kept, 16 bits, in a
+//      register); its other bins (east, north, north-east of a box that
+//      crosses a bin edge, or every bin of a wide box) are counted apart;
+//   2. reserves the round's share of every bin it touches with one device
+//      atomic on fill[b] (about 1,000 per round instead of one per
+//      membership): first memberships, then the others.  A share past the
+//      bin's capacity flags kErrBinCap and the workgroup stops placing;
+//   3. writes the other memberships straight to slots handed out by LDS
+//      atomics, from a list of the boxes that have any (kept in the stage's
+//      space during the count: a few percent of narrow boxes; a round with
+//      more than the stage holds fails over like a bin over capacity);
+//   4. places the first memberships through an LDS stage, in passes of
+//      `stage` entries in the workgroup's bin order (a block scan of the
+//      first counts), so that a wave's stores cover whole runs of a bin.
+// The boxes are read once (no histogram pass and no scan kernel).  The call
+// also zeroes the next call's counters and result words.
+constexpr uint32_t kPlaceThreads = 1024;
+constexpr uint32_t kPlacePer = 16;
+constexpr uint32_t kPlaceRound = kPlaceThreads * kPlacePer;
+// A box's first bin and its rank there share one register word.
+constexpr uint32_t kRankShift = 13;
+static_assert(kMaxBins < (1u << kRankShift) && kPlaceRound <= (1u << (32 - kRankShift)), "bin | rank << 13");
+// Register word of a box in the round: its first bin, and bits for "also
+// the bin east" / "also the bin north"; kSlotWide: the box spans more than
+// two fine cells on an axis (all its bins go the direct way);
+// kSlotNone: no box (past the range or invalid).
+constexpr uint32_t kSlotEast = 1u << 29, kSlotNorth = 1u << 30, kSlotWide = 1u << 31;
+constexpr uint32_t kSlotBin = kSlotEast - 1u;
+constexpr uint32_t kSlotNone = 0xFFFFFFFFu;
+static_assert(kMaxBins <= kSlotBin, "bin field");
+
+// Dynamic LDS of the place kernel: three words per bin, then the stage.
+__host__ __device__ inline uint32_t place_stage_entries(uint32_t nb, uint32_t lds_bytes) {
+  const uint32_t fixed = 12u * nb + 64u;
+  const uint32_t e = lds_bytes > fixed ? (lds_bytes - fixed) / 24u : 0u;
+  return (e > 8192u ? 8192u : e) & ~63u;
+}
+
+__global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __restrict__ boxes,
+                                                             const InstDesc* __restrict__ inst,
+                                                             uint32_t* __restrict__ fill,
+                                                             uint32_t* __restrict__ fill_next, uint32_t fill_words,
+                                                             float4* __restrict__ bbox, uint32_t* __restrict__ bid,
+                                                             uint32_t bin_cap, uint32_t stage,
+                                                             Meta* __restrict__ meta,
+                                                             Meta* __restrict__ meta_next) {
+  extern __shared__ __align__(16) unsigned char place_raw[];
+  __shared__ uint32_t flags;  // 1 invalid box, 2 wide box, 4 bin over capacity (or too many boxes listed)
+  __shared__ uint32_t wsum[kPlaceThreads / 64];
+  __shared__ uint32_t npend;  // boxes of the round with memberships besides a first one
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const InstDesc I = inst[0];
+  const uint32_t nb = I.nb, nbx = I.nbx;
+  float4* const sbox = reinterpret_cast<float4*>(place_raw);     // stage: boxes,
+  uint32_t* const sid = reinterpret_cast<uint32_t*>(sbox + stage);  // ids,
+  uint32_t* const sdst = sid + stage;                              // destinations
+  uint32_t* const cnt1 = sdst + stage;  // per bin: first memberships, then their position in the stage order
+  uint32_t* const cnt2 = cnt1 + nb;     // per bin: other memberships, then their next bucket slot
+  uint32_t* const base1 = cnt2 + nb;    // per bin: bucket slot of the first memberships' run
+  for (uint32_t i = blockIdx.x * kPlaceThreads + tid; i < fill_words; i += gridDim.x * kPlaceThreads)
+    fill_next[i] = 0u;
+  if (blockIdx.x == 0 && tid < sizeof(Meta) / sizeof(unsigned long long))
+    reinterpret_cast<unsigned long long*>(meta_next)[tid] = 0ull;
+  if (tid == 0) flags = 0u;
+  const uint64_t n = I.n;
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t g0 = (uint64_t)blockIdx.x * per;
+  const uint64_t g1 = g0 + per < n ? g0 + per : n;
+  const uint32_t bper = (nb + kPlaceThreads - 1) / kPlaceThreads;  // bins per thread in the scan
+  for (uint64_t r0 = g0; r0 < g1; r0 += kPlaceRound) {
+    const uint64_t r1 = r0 + kPlaceRound < g1 ? r0 + kPlaceRound : g1;
+    for (uint32_t b = tid; b < nb; b += kPlaceThreads) {
+      cnt1[b] = 0u;
+      cnt2[b] = 0u;
+    }
+    if (tid == 0) npend = 0u;
+    float4 q[kPlacePer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPlacePer; ++k) {
+      const uint64_t i = r0 + tid + k * kPlaceThreads;
+      q[k] = boxes[I.box0 + (i < r1 ? i : r1 - 1)];
+    }
+    __syncthreads();  // counters zeroed (and the last round placed)
+    // 1. count; a first membership's bin and rank in it are kept
+    uint32_t slot[kPlacePer];
+    uint32_t fl = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPlacePer; ++k) {
+      const uint64_t i = r0 + tid + k * kPlaceThreads;
+      const float4 b = q[k];
+      slot[k] = kSlotNone;
+      uint32_t rk = 0u;
+      if (i < r1) {
+        if (!((b.x <= b.y) && (b.z <= b.w))) {
+          fl |= 1u;
+        } else {
+          const uint4 f = fine_corners(b, I);
+          const BinRange rg = bin_range_of(f, I);
+          const uint32_t bb = rg.y0 * nbx + rg.x0;
+          uint32_t sw;
+          if (f.y - f.x > 1u || f.w - f.z > 1u) {
+            fl |= 2u;
+            sw = bb | kSlotWide;
+            for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
+              for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) atomicAdd(&cnt2[cy * nbx + cx], 1u);
+          } else {
+            const bool ex = rg.x1 != rg.x0, ey = rg.y1 != rg.y0;
+            sw = bb | (ex ? kSlotEast : 0u) | (ey ? kSlotNorth : 0u);
+            rk = atomicAdd(&cnt1[bb], 1u);
+            if (ex) atomicAdd(&cnt2[bb + 1u], 1u);
+            if (ey) atomicAdd(&cnt2[bb + nbx], 1u);
+            if (ex && ey) atomicAdd(&cnt2[bb + nbx + 1u], 1u);
+          }
+          slot[k] = sw & kSlotWide ? kSlotNone : bb | (rk << kRankShift);  // the staged passes' boxes
+          if (sw & (kSlotWide | kSlotEast | kSlotNorth)) {  // listed for the direct writes
+            const uint32_t at = atomicAdd(&npend, 1u);
+            if (at < stage) {
+              sbox[at] = b;
+              sid[at] = (uint32_t)i;
+              sdst[at] = sw;
+            } else {
+              fl |= 4u;
+            }
+          }
+        }
+      }
+    }
+    if (fl) atomicOr(&flags, fl);
+    __syncthreads();
+    // 2. reserve every touched bin's share, and the stage order of the first
+    //    memberships (exclusive scan of their counts over the bins)
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < bper; ++k) {
+      const uint32_t b = tid * bper + k;
+      if (b < nb) {
+        const uint32_t c1 = cnt1[b], c2 = cnt2[b];
+        uint32_t base = 0u;
+        if (c1 + c2) {
+          base = atomicAdd(&fill[b], c1 + c2);
+          if (base + c1 + c2 > bin_cap) atomicOr(&flags, 4u);
+        }
+        base1[b] = b * bin_cap + base;
+        cnt2[b] = b * bin_cap + base + c1;
+        acc += c1;
+      }
+    }
+    const uint32_t inc = wave_prefix_sum(acc);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t ex = inc - acc, total1 = 0;
+    for (uint32_t k = 0; k < kPlaceThreads / 64; ++k) {
+      if (k < w) ex += wsum[k];
+      total1 += wsum[k];
+    }
+    for (uint32_t k = 0; k < bper; ++k) {
+      const uint32_t b = tid * bper + k;
+      if (b < nb) {
+        const uint32_t c1 = cnt1[b];
+        cnt1[b] = ex;
+        ex += c1;
+      }
+    }
+    __syncthreads();
+    if (flags & 4u) break;  // the launch fails over to the two-pass pipeline
+    // 3. the listed boxes' other memberships (every one of a wide box)
+    {
+      const uint32_t np = npend;
+      for (uint32_t j = tid; j < np; j += kPlaceThreads) {
+        const float4 b = sbox[j];
+        const uint32_t id = sid[j], sw = sdst[j];
+        auto put = [&](uint32_t bin) {
+          const uint32_t d = atomicAdd(&cnt2[bin], 1u);
+          bbox[d] = b;
+          bid[d] = id;
+        };
+        if (sw & kSlotWide) {
+          const BinRange rg = bin_range_of(fine_corners(b, I), I);
+          for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
+            for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) put(cy * nbx + cx);
+        } else {
+          const uint32_t bb = sw & kSlotBin;
+          if (sw & kSlotEast) put(bb + 1u);
+          if (sw & kSlotNorth) put(bb + nbx);
+          if ((sw & kSlotEast) && (sw & kSlotNorth)) put(bb + nbx + 1u);
+        }
+      }
+      __syncthreads();  // the stage is reused
+    }
+    // 4. first memberships, staged in passes in the workgroup's bin order:
+    //    each box's word becomes (bin, position in that order); the writer
+    //    finds the bucket slot from the staged bin
+    const uint32_t kBinMask = (1u << kRankShift) - 1u;
+#pragma unroll
+    for (uint32_t k = 0; k < kPlacePer; ++k) {
+      const uint32_t sl = slot[k];
+      if (sl != kSlotNone) slot[k] = (sl & kBinMask) | ((cnt1[sl & kBinMask] + (sl >> kRankShift)) << kRankShift);
+    }
+    for (uint32_t v0 = 0; v0 < total1; v0 += stage) {
+      const uint32_t lim = total1 - v0 < stage ? total1 - v0 : stage;
+#pragma unroll
+      for (uint32_t k = 0; k < kPlacePer; ++k) {
+        const uint32_t sl = slot[k];
+        const uint32_t j = (sl >> kRankShift) - v0;
+        if (sl != kSlotNone && j < lim) {
+          sbox[j] = q[k];
+          sid[j] = (uint32_t)(r0 + tid + k * kPlaceThreads);
+          sdst[j] = sl & kBinMask;
+        }
+      }
+      __syncthreads();
+      for (uint32_t j = tid; j < lim; j += kPlaceThreads) {
+        const uint32_t bb = sdst[j];
+        const uint32_t d = base1[bb] + (v0 + j - cnt1[bb]);
+        bbox[d] = sbox[j];
+        bid[d] = sid[j];
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && flags) {
+    if (flags & 1u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrInvalidBox);
+    if (flags & 2u) atomicOr(&meta->wide, 1u);
+    if (flags & 4u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrBinCap);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fine binning + per-cell test of coarse bins in LDS.
 //
 // One persistent 1024-thread workgroup per CU walks bins b = blockIdx.x,
@@ -734,7 +967,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
+  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
   }
@@ -1091,7 +1324,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense)) {
+  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
     return;
   }
@@ -1410,7 +1643,16 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 
 // ---------------------------------------------------------------------------
 
+// Dynamic LDS of the place kernel (its static part is a few words).
+constexpr uint32_t kPlaceLdsBytes = 156u * 1024u;
+static_assert(12u * kMaxBins + 64u + 24u * 1024u <= kPlaceLdsBytes, "a stage of 1,024 entries at the most bins");
+
 hipError_t binned_init() {
+  {
+    const hipError_t e0 = hipFuncSetAttribute((const void*)k_bin_place, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kPlaceLdsBytes);
+    if (e0 != hipSuccess) return e0;
+  }
   hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kTestLdsBytes);
   if (e != hipSuccess) return e;
@@ -1449,6 +1691,32 @@ hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, 
   k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
   k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
                                       ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, const float4* boxes,
+                                 const BatchPlan& bp, uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev) {
+  hipError_t e;
+#define PBIG_MARK(k) \
+  if (ev && (e = hipEventRecord(ev[k], s)) != hipSuccess) return e;
+  PBIG_MARK(0)
+  // one workgroup per CU, each with at least 2,048 boxes (device atomics:
+  // workgroups x bins touched)
+  const uint64_t want = (bp.n_boxes + 2047u) / 2048u;
+  uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
+  if (want < grid) grid = want ? (uint32_t)want : 1u;
+  const uint32_t nbp = bp.nb_max ? bp.nb_max : 1u;
+  const uint32_t stage = place_stage_entries(nbp, kPlaceLdsBytes);
+  if (stage < 64u) return hipErrorInvalidValue;  // too many bins for the stage (the planner keeps nb <= kMaxBins)
+  k_bin_place<<<grid, kPlaceThreads, 24u * stage + 12u * nbp, s>>>(boxes, ws.inst, op.fill, op.fill_next,
+                                                                  op.fill_words, ws.bbox, ws.bid, op.bin_cap, stage,
+                                                                  op.meta, op.meta_next);
+  PBIG_MARK(1)
+  k_bin_test<<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(
+      ws.bbox, ws.bid, op.bin_start, op.fill, ws.inst, ws.bin_inst, bp.n_bins, (uint64_t)bp.n_bins * op.bin_cap,
+      ws.scratch, ws.cap_wg, ws.bin_seg, ws.overfull, out, cap, 1u, op.meta);
+  PBIG_MARK(2)
+#undef PBIG_MARK
   return hipGetLastError();
 }
 

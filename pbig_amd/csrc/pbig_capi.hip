@@ -59,6 +59,19 @@ struct BinnedState {
   pbig::SpliceRun* runs = nullptr; // device table of the runs to place
   uint64_t cap_runs = 0;
   bool own_meta = false;
+  // Single-pass launch of the plain path (no histogram or scan): every bin
+  // owns bin_cap bucket entries; counters and result words alternate
+  // between two sets by call parity (each call zeroes the other set).
+  uint32_t bin_cap = 0;          // 0: this plan runs the two-pass launch
+  bool onepass_failed = false;   // this plan's single-pass launch reported trouble: two passes from now on
+  bool onepass_dirty = true;     // the counter sets must be cleared before the next single-pass launch
+  uint32_t* fill[2] = {nullptr, nullptr};
+  uint64_t cap_fill = 0;
+  Meta* meta1 = nullptr;         // two result-word sets
+  uint32_t* start1 = nullptr;    // per bin: b * bin_cap
+  uint64_t cap_start1 = 0;
+  uint32_t start1_nb = 0, start1_cap = 0;
+  uint32_t parity = 0;
 };
 
 struct pbig_hip_ctx {
@@ -98,6 +111,9 @@ struct pbig_hip_ctx {
 
 namespace {
 constexpr int kEventsPerCall = 8;
+// Timing path id of the single-pass launch (place, test), beside
+// PBIG_PATH_GENERAL and PBIG_PATH_BINNED.
+constexpr int kPathOnePass = 3;
 
 // Device memory of pbig_hip_check_device_noalloc: the whole workspace of a
 // single-instance binned launch, carved per call from this array.  Each
@@ -525,6 +541,13 @@ void free_state(BinnedState& st) {
     if (p) (void)hipFree(p);
   if (st.own_meta && b.meta) (void)hipFree(b.meta);
   st.ws = BinnedWorkspace{};
+  for (void* p : {(void*)st.fill[0], (void*)st.fill[1], (void*)st.meta1, (void*)st.start1})
+    if (p) (void)hipFree(p);
+  st.fill[0] = st.fill[1] = st.start1 = nullptr;
+  st.meta1 = nullptr;
+  st.cap_fill = st.cap_start1 = 0;
+  st.start1_nb = st.start1_cap = 0;
+  st.onepass_dirty = true;
 }
 
 int upload_tables(pbig_hip_ctx* c, BinnedState& st, hipStream_t s) {
@@ -577,6 +600,18 @@ int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_
   if (!layout_tables(st, (uint32_t)c->n_cu, nullptr)) {
     *ok = false;
     return PBIG_OK;
+  }
+  // Single-pass eligibility: a full bin's expected memberships (boxes per
+  // bin area, 10% for boxes crossing into neighbouring bins) plus eight
+  // standard deviations fit what one test workgroup holds.
+  {
+    const double nf0 = std::max(1.0, std::ceil(range / (double)cell));
+    const double e = 1.1 * (double)n * (double)d.k * d.k / (nf0 * nf0);
+    const double want = e + 8.0 * std::sqrt(e) + 64.0;
+    st.bin_cap = want <= (double)pbig::kNarrowMax
+                     ? std::min<uint32_t>(pbig::kNarrowMax, ((uint32_t)std::ceil(want) + 63u) & ~63u)
+                     : 0u;
+    st.onepass_failed = false;
   }
   st.ws.meta = c->ws.meta;
   int rc;
@@ -637,6 +672,112 @@ int run_state_sync(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, cons
     return PBIG_OK;
   }
   return fail(c, PBIG_E_HIP, "unreachable");
+}
+
+// Buffers of the plain state's single-pass launch for its current plan.
+// *ok is false when the plan runs the two-pass launch.
+int prepare_onepass(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, bool* ok) {
+  *ok = st.bin_cap != 0 && !st.onepass_failed && st.bp.n_inst == 1 && st.bp.n_bins > 0;
+  if (!*ok) return PBIG_OK;
+  const uint32_t nb = st.bp.n_bins;
+  int rc;
+  if ((rc = ensure_state(c, st, std::max<uint64_t>(st.ws.cap_bucket, (uint64_t)nb * st.bin_cap)))) return rc;
+  if (st.cap_fill < nb || !st.fill[0] || !st.fill[1]) {
+    for (int k = 0; k < 2; ++k) {
+      if (st.fill[k]) HIP_TRY(c, hipFree(st.fill[k]));
+      st.fill[k] = nullptr;
+    }
+    st.cap_fill = 0;
+    const uint64_t want = std::max<uint64_t>(nb, 1024);
+    uint64_t d0 = 0, d1 = 0;
+    if ((rc = grow(c, &st.fill[0], &d0, want)) || (rc = grow(c, &st.fill[1], &d1, want))) return rc;
+    st.cap_fill = want;
+    st.onepass_dirty = true;
+  }
+  if (!st.meta1) {
+    HIP_TRY(c, hipMalloc((void**)&st.meta1, 2 * sizeof(Meta)));
+    st.onepass_dirty = true;
+  }
+  if (st.start1_nb != nb || st.start1_cap != st.bin_cap) {
+    if ((rc = grow(c, &st.start1, &st.cap_start1, nb))) return rc;
+    std::vector<uint32_t> v(nb);
+    for (uint32_t b = 0; b < nb; ++b) v[b] = b * st.bin_cap;
+    HIP_TRY(c, hipMemcpyAsync(st.start1, v.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));  // v is pageable and goes out of scope
+    st.start1_nb = nb;
+    st.start1_cap = st.bin_cap;
+  }
+  if (st.onepass_dirty) {
+    HIP_TRY(c, hipMemsetAsync(st.fill[0], 0, st.cap_fill * sizeof(uint32_t), s));
+    HIP_TRY(c, hipMemsetAsync(st.fill[1], 0, st.cap_fill * sizeof(uint32_t), s));
+    HIP_TRY(c, hipMemsetAsync(st.meta1, 0, 2 * sizeof(Meta), s));
+    st.parity = 0;
+    st.onepass_dirty = false;
+  }
+  return PBIG_OK;
+}
+
+// Enqueue the single-pass launch; *dm receives the device address of its
+// result words.
+int launch_onepass(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
+                   hipStream_t s, Meta** dm) {
+  const uint32_t p = st.parity & 1u;
+  pbig::OnePass op;
+  op.fill = st.fill[p];
+  op.fill_next = st.fill[p ^ 1u];
+  op.fill_words = (uint32_t)st.cap_fill;
+  op.bin_start = st.start1;
+  op.bin_cap = st.bin_cap;
+  op.meta = st.meta1 + p;
+  op.meta_next = st.meta1 + (p ^ 1u);
+  const hipError_t e = pbig::launch_binned_onepass(st.ws, op, d_boxes, st.bp, d_out, cap, s,
+                                                   next_events(c, kPathOnePass));
+  if (e != hipSuccess) {
+    st.onepass_dirty = true;  // the counter sets may not have been cleared for the next call
+    HIP_TRY(c, e);
+  }
+  st.parity ^= 1u;
+  *dm = op.meta;
+  return PBIG_OK;
+}
+
+// The single-pass launch, synchronously.  0: done, *m holds the result
+// words (memberships summed from the bins' counters); 1: the plan runs the
+// two-pass launch instead (not eligible, or the launch reported trouble,
+// which also marks the plan); < 0: an error.
+int run_onepass_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
+  BinnedState& st = c->plain;
+  int rc;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    bool ok = false;
+    if ((rc = prepare_onepass(c, st, s, &ok))) return rc;
+    if (!ok) return 1;
+    Meta* dm = nullptr;
+    if ((rc = launch_onepass(c, st, d_boxes, d_out, cap, s, &dm))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, dm, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    *m = *c->h_meta;
+    if (m->error & pbig::kErrInvalidBox)
+      return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    if (m->error == pbig::kErrWorkspace && m->items > st.ws.cap_wg) {  // a pair region overflowed
+      st.ws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, m->items + m->items / 4 + 1024);
+      if ((rc = ensure_state(c, st, st.ws.cap_bucket))) return rc;
+      continue;
+    }
+    if (m->error) {
+      st.onepass_failed = true;
+      return 1;
+    }
+    // the counters of this call were left in the set it used
+    const uint32_t nb = st.bp.n_bins;
+    std::vector<uint32_t> f(nb);
+    HIP_TRY(c, hipMemcpy(f.data(), st.fill[(st.parity ^ 1u) & 1u], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    m->memberships = 0;
+    for (uint32_t v : f) m->memberships += v;
+    return 0;
+  }
+  st.onepass_failed = true;
+  return 1;
 }
 
 // Pairs of a launch's instances found outside it (by re-binning their
@@ -836,12 +977,15 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
                     hipStream_t s, uint64_t* n_pairs) {
   BinnedState& st = c->plain;
   Meta m;
-  Splice sp;
-  std::vector<uint8_t> failed;
-  int rc;
-  if ((rc = resolve(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, 0, &m, &sp, &failed))) return rc;
-  if (failed[0]) return 1;
-  if (!sp.empty() && (rc = apply_splice(c, st, sp, d_out, cap, nullptr, s, &m))) return rc;
+  int rc = run_onepass_sync(c, d_boxes, d_out, cap, s, &m);
+  if (rc < 0) return rc;
+  if (rc == 1) {  // the two-pass launch, with re-binning of bins beyond LDS
+    Splice sp;
+    std::vector<uint8_t> failed;
+    if ((rc = resolve(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, 0, &m, &sp, &failed))) return rc;
+    if (failed[0]) return 1;
+    if (!sp.empty() && (rc = apply_splice(c, st, sp, d_out, cap, nullptr, s, &m))) return rc;
+  }
   const InstDesc& d = st.inst[0];
   c->last_path = PBIG_PATH_BINNED;
   c->stats.n_boxes = n;
@@ -1059,6 +1203,14 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
     binned = ok;
   }
   if (binned) {
+    bool one = false;
+    if ((rc = prepare_onepass(c, c->plain, s, &one))) return rc;
+    if (one) {
+      Meta* dm = nullptr;
+      if ((rc = launch_onepass(c, c->plain, (const float4*)d_boxes, (uint2*)d_pairs, cap, s, &dm))) return rc;
+      if (d_result) HIP_TRY(c, pbig::launch_result(dm, d_result, s));
+      return PBIG_OK;
+    }
     HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, nullptr, c->plain.bp, (uint2*)d_pairs, cap,
                                    nullptr, s, next_events(c, PBIG_PATH_BINNED)));
   } else {
@@ -1236,7 +1388,9 @@ int pbig_hip_timing_start(pbig_hip_ctx* c, int max_calls) {
 int pbig_hip_timing_read(pbig_hip_ctx* c, double* ms_per_kernel, int n_kernels, int* n_calls, int* path) {
   if (!c || !ms_per_kernel || n_kernels < pbig::kPlainKernels || !n_calls) return PBIG_E_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  const int nk = c->timing_path == PBIG_PATH_BINNED ? pbig::kBinnedKernels : pbig::kPlainKernels;
+  const int nk = c->timing_path == PBIG_PATH_BINNED ? pbig::kBinnedKernels
+                 : c->timing_path == kPathOnePass ? pbig::kOnePassKernels
+                                                   : pbig::kPlainKernels;
   for (int k = 0; k < n_kernels; ++k) ms_per_kernel[k] = 0.0;
   for (int call = 0; call < c->timing_used; ++call) {
     hipEvent_t* ev = &c->events[(size_t)call * kEventsPerCall];
