@@ -119,12 +119,17 @@ def read_traffic(path_name, cs, bin_target):
     return out, os.path.relpath(fn, ROOT)
 
 
-def clustered_leg(torch, eng_cls, reps=5):
-    """The clustered 2,097,152-box instance (seed 3), boxes resident: one
-    synchronous call per step (its heavy coarse bins are re-binned, which
-    needs the host between levels), binned path vs general path, and the
-    automatic choice (which sends an instance whose heavy bins hold most of
-    its memberships to the general path)."""
+def clustered_leg(torch, eng_cls, steps=50, reps=5):
+    """The clustered 2,097,152-box instance (seed 3), boxes resident, at the
+    autotuner's cell size.
+    * One synchronous call per step, per path: the binned path forced (its
+      heavy coarse bins are re-binned, which needs the host between levels),
+      the general path forced, and the automatic choice.
+    * The automatic choice as a stream of asynchronous calls (after the
+      synchronous call that settles its path): `steps` calls timed with two
+      events, then again with HIP events around every kernel; the roofline
+      of the dominant kernel and of the whole call, with the algorithmic
+      bytes priced at the reference rule's cell size like the 4M leg's."""
     import pbig_amd
     from pbig_amd import autotune, workloads
 
@@ -134,14 +139,22 @@ def clustered_leg(torch, eng_cls, reps=5):
     d = torch.from_numpy(boxes).to(dev)
     pairs = torch.empty((40 * n, 2), dtype=torch.int32, device=dev)
     cell = float(autotune.cell_sizes([0, n], [L])[0])
+    ref_cs = pbig_amd.reference_cell_size(boxes, L)
     out = {"workload": "2,097,152 float32 rects, 90% in 64 Gaussian clusters (sigma 327.68) + 10% uniform "
                        "in [0,65536)^2, log-normal sides, seed 3, boxes resident in HBM",
-           "cell_size": cell}
+           "cell_size": cell, "reference_cell_size": ref_cs}
+    e = eng_cls(0)
+    e.configure(pbig_amd.PATH_GENERAL)
+    k_ref = e.check_device(d, L, ref_cs, pairs)
+    st = e.stats()
+    e.close()
+    B = alg_bytes(n, st["n_cells"], st["memberships"], k_ref)
     for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL),
                        ("auto", pbig_amd.PATH_AUTO)):
         e = eng_cls(0)
         e.configure(path)
         k = e.check_device(d, L, cell, pairs)
+        assert k == k_ref, (name, k, k_ref)
         torch.cuda.synchronize()
         t = []
         for _ in range(reps):
@@ -150,9 +163,50 @@ def clustered_leg(torch, eng_cls, reps=5):
             t.append(time.perf_counter() - t0)
             assert k2 == k
         st = e.stats()
-        e.close()
         ms = statistics.median(t) * 1e3
         out[name] = {"ms_per_call": ms, "pairs": k, "overlaps_per_s": k / (ms / 1e3), "path": st["path"]}
+        if name == "auto":
+            stream = torch.cuda.Stream(device=dev)
+            res = torch.zeros(2, dtype=torch.int64, device=dev)
+            with torch.cuda.stream(stream):
+                for _ in range(3):
+                    e.check_device_async(d, L, cell, pairs, res, stream=stream)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            with torch.cuda.stream(stream):
+                a.record(stream)
+                for _ in range(steps):
+                    e.check_device_async(d, L, cell, pairs, res, stream=stream)
+                b.record(stream)
+            torch.cuda.synchronize()
+            assert res.cpu().tolist() == [k, 0], res.cpu().tolist()
+            ms_async = a.elapsed_time(b) / steps
+            e.timing_start(steps)
+            with torch.cuda.stream(stream):
+                for _ in range(steps):
+                    e.check_device_async(d, L, cell, pairs, res, stream=stream)
+            torch.cuda.synchronize()
+            per_kernel, calls, path_name = e.timing_read()
+            assert res.cpu().tolist() == [k, 0]
+            avg = {kk: v / calls for kk, v in per_kernel.items() if v > 0}
+            kernel_alg = {"count": B["count"], "scatter": B["scatter"], "test": B["test"],
+                          "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * st["n_cells"],
+                          "bin_hist": B["count"], "bin_scan": B["scan"], "bin_starts": 0.0,
+                          "bin_scatter": B["scatter"], "bin_test": B["test"],
+                          "bin_place": B["count"] + B["scan"] + B["scatter"]}
+            dom = max(avg, key=lambda kk: avg[kk])
+            achieved = kernel_alg.get(dom, 0.0) / (avg[dom] / 1e3) / 1e9
+            out["auto_async"] = {
+                "ms_per_call": ms_async, "overlaps_per_s": k / (ms_async / 1e3), "path": path_name,
+                "kernel_avg_ms": avg,
+                "alg_bytes_per_call": B["total"],
+                "alg_GBps": B["total"] / (ms_async / 1e3) / 1e9,
+                "frac_of_peak": B["total"] / (ms_async / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                             "alg_bytes_per_launch": kernel_alg.get(dom, 0.0), "avg_launch_ms": avg[dom]},
+            }
+        e.close()
     del d, pairs
     torch.cuda.empty_cache()
     return out

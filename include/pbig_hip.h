@@ -164,7 +164,9 @@ int pbig_hip_configure(pbig_hip_ctx* ctx, const pbig_hip_config* cfg);
  * took (n_kernels >= 7).  GENERAL: clear, count, scan_reduce, scan_apply,
  * scatter, test.  BINNED: hist, scan, starts, scatter, test, pair_scan,
  * compact.  Path 3, the binned path's single-pass launch of one instance:
- * place, test.  Batch launches are timed as BINNED. */
+ * place, test.  Path 4, the two-pass launch of one instance followed by
+ * re-binning levels planned on the device: one duration per level (level 0
+ * is the two-pass launch).  Batch launches are timed as BINNED. */
 int pbig_hip_timing_start(pbig_hip_ctx* ctx, int max_calls);
 int pbig_hip_timing_read(pbig_hip_ctx* ctx, double* ms_per_kernel, int n_kernels, int* n_calls, int* path);
 

@@ -63,13 +63,17 @@ PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
 # pbig_hip_timing_read also reports the binned path's single-pass launch of
 # one instance (no histogram or scan pass) as path 3.
 TIMING_ONEPASS = 3
-PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned", TIMING_ONEPASS: "onepass"}
+# ... and the two-pass launch followed by re-binning levels planned on the
+# device as path 4 (one duration per level).
+TIMING_DEEP = 4
+PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned", TIMING_ONEPASS: "onepass", TIMING_DEEP: "deep"}
 # Kernels of each path in launch order (pbig_hip_timing_read).
 KERNELS = {
     PATH_GENERAL: ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test"),
     PATH_BINNED: ("bin_hist", "bin_scan", "bin_starts", "bin_scatter", "bin_test", "bin_pair_scan",
                   "bin_compact"),
     TIMING_ONEPASS: ("bin_place", "bin_test"),
+    TIMING_DEEP: ("level0", "level1", "level2", "level3"),
 }
 
 # numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)

@@ -61,6 +61,68 @@ struct InstDesc {
 };
 static_assert(sizeof(InstDesc) == 112, "InstDesc layout");
 
+// Plan of a virtual instance (host and device): coarse bin lb of instance
+// P, holding m memberships at box0 of P's launch's bin-ordered arrays.  Its
+// fine cells are P's cells split 2^j ways per side, j chosen so that a cell
+// holds about two memberships at the bin's density (dense cells cost
+// quadratically): the cell side is P's divided by 2^j, and inv_f * 2^j is
+// exact, so a point's cell at this level lies inside its cell at P's level
+// and the reference-cell rule claims every pair in the same region as at
+// P's level.  j is capped so that the cells stay at most 2^max_refine
+// times finer than the instance's own (root_inv_f): past the boxes' own size
+// finer cells only multiply the memberships of every box; the bins shrink
+// instead.  Bins of about `target` memberships, at most kMaxK cells per side
+// and max_bins bins.  False when the bin cannot be split.
+__host__ __device__ inline bool plan_virtual_bin(const InstDesc& P, uint32_t parent, uint32_t lb, uint32_t m,
+                                                 uint64_t box0, uint32_t target, uint32_t max_bins, InstDesc* v,
+                                                 float root_inv_f = 0.0f, uint32_t max_refine = 6) {
+  *v = InstDesc{};
+  v->n = m;
+  v->box0 = box0;
+  v->parent = parent;
+  const uint32_t fx0 = P.fx0 + (lb % P.nbx) * P.k, fy0 = P.fy0 + (lb / P.nbx) * P.k;
+  const uint32_t rx = P.fx0 + P.span_x - fx0, ry = P.fy0 + P.span_y - fy0;
+  const uint32_t sx = P.k < rx ? P.k : rx, sy = P.k < ry ? P.k : ry;
+  const double per_cell = (double)m / ((double)sx * sy);
+  uint32_t done = 0;  // refinement already applied above this level (powers of two: exact)
+  if (root_inv_f > 0.0f)
+    while (done < 24 && root_inv_f * (float)(1u << done) < P.inv_f) ++done;
+  const uint32_t jmax = max_refine > done ? max_refine - done : 0u;
+  uint32_t j = 0;
+  while (j < 6 && j < jmax && per_cell / (double)(1u << (2 * j)) > 2.0 && ((uint64_t)P.nf << (j + 1)) <= (1u << 21))
+    ++j;
+  const uint32_t f = 1u << j;
+  v->inv_f = P.inv_f * (float)f;
+  v->nf = P.nf * f;
+  v->fx0 = fx0 * f;
+  v->fy0 = fy0 * f;
+  v->span_x = sx * f;
+  v->span_y = sy * f;
+  const uint32_t side = v->span_x > v->span_y ? v->span_x : v->span_y;
+  if (side <= 1) return false;
+  // bins: about m / target of them, square, k <= kMaxK, at most max_bins
+  const double tq = (double)(target > 16u ? target : 16u);
+  double nbx_want = ceil(sqrt((double)m / tq));
+  if (nbx_want < 2.0) nbx_want = 2.0;
+  uint32_t k = (uint32_t)ceil((double)side / nbx_want);
+  k = k < 1u ? 1u : (k > kMaxK ? kMaxK : k);
+  if (j == 0 && k > side - 1u) k = side - 1u;  // same cells: the bins must shrink
+  while (((side + k - 1) / k) * ((side + k - 1) / k) > max_bins) {
+    if (k >= kMaxK) return false;
+    ++k;
+  }
+  v->k = k;
+  v->k_magic = ((1ull << 32) + k - 1) / k;
+  v->nbx = (side + k - 1) / k;
+  v->nb = v->nbx * v->nbx;
+  uint64_t nw = ((uint64_t)m + kScatterChunk - 1) / kScatterChunk;
+  if (nw > 8) nw = (nw + 7) / 8 * 8;
+  const uint64_t ch = ((uint64_t)m + nw - 1) / nw;
+  v->chunk = (uint32_t)(ch < kScatterChunk ? ch : kScatterChunk);
+  v->nchunks = m ? (uint32_t)(((uint64_t)m + v->chunk - 1) / v->chunk) : 0u;
+  return true;
+}
+
 // Totals of a batch (host side).
 // Row pitches of the histogram (nb counts + the chunk's invalid-box word)
 // and offset tables, in entries: whole 128-byte lines, so that the scan's
@@ -160,6 +222,34 @@ constexpr int kOnePassKernels = 2;  // place, test
 hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, const float4* boxes,
                                  const BatchPlan& bp, uint2* out, uint64_t cap, hipStream_t s,
                                  hipEvent_t* events = nullptr);
+
+// Workspace bounds of a re-binning level planned on the device (its plan
+// kernel flags plan_err when the plan exceeds them).
+struct DeepCaps {
+  uint32_t inst = 0, chunks = 0, bins = 0, nb_max = 0;
+  uint64_t hist = 0, off = 0;
+};
+
+// The result words of every level of a launch (level 0 first), for the
+// final result kernel.
+constexpr uint32_t kDeepLevels = 4;     // level 0 and up to three re-binning levels
+constexpr uint32_t kDeepMaxInst = 2048;  // virtual instances of one level (LDS of its plan kernel)
+struct DeepMetas {
+  const Meta* meta[kDeepLevels];
+  uint32_t levels;
+};
+
+// One re-binning level: its plan kernel reads the level above's list of bins
+// beyond LDS (parent, parent_bins bins; parent_dev: that level was itself
+// planned on the device), then hist, scan, starts, scatter and test run over
+// its virtual instances with device counts; its test kernel writes pairs
+// through the shared counter in its meta's out_count.
+// root_inv_f, max_refine: the cap on the cells' refinement below the
+// instance's own (plan_virtual_bin).
+hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
+                             const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
+                             uint32_t max_refine, uint32_t n_cu, uint2* out, uint64_t cap, hipStream_t s);
+hipError_t launch_deep_result(const DeepMetas& m, uint64_t* result, hipStream_t s);
 
 // Tables of a launch of one instance written by a kernel from its argument:
 // the instance record, and zero in the chunk -> instance and bin -> instance

@@ -159,12 +159,17 @@ __device__ __forceinline__ void chunk_range(const InstDesc& I, uint32_t w, uint6
   *end = *start + I.chunk < inst_end ? *start + I.chunk : inst_end;
 }
 
+// kDev (all kernels below): a level planned on the device; its chunk and bin
+// counts are read from the level's result words (meta->dev_*), and the grid
+// is an upper bound.
+template <bool kDev>
 __global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restrict__ boxes,
                                                     const InstDesc* __restrict__ inst,
                                                     const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
-                                                    uint32_t* __restrict__ hist) {
+                                                    uint32_t* __restrict__ hist, const Meta* __restrict__ meta) {
   __shared__ uint32_t h[kMaxBins];
   __shared__ uint32_t err;
+  if (kDev) n_chunks = meta->plan_err ? 0u : meta->dev_chunks;
   if (blockIdx.x >= n_chunks) return;
   const InstDesc I = inst[chunk_inst[blockIdx.x]];
   for (uint32_t b = threadIdx.x; b < I.nb; b += blockDim.x) h[b] = 0u;
@@ -222,6 +227,7 @@ __global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restr
 constexpr uint32_t kScanBins = 16, kScanGroups = 64, kScanPer = 16;
 static_assert(kScanBins * kScanGroups == 1024 && kScanGroups == 64, "the group prefix is one wave's scan per bin");
 
+template <bool kDev>
 __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ hist,
                                                     const InstDesc* __restrict__ inst,
                                                     const uint32_t* __restrict__ bin_inst, uint32_t n_bins,
@@ -231,6 +237,10 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
   __shared__ uint32_t part[kScanGroups][kScanBins + 1];
   __shared__ uint32_t carry[kScanBins];
   __shared__ uint32_t tiles;
+  if (kDev) {
+    n_bins = meta->plan_err ? 0u : meta->dev_bins;
+    n_chunks = meta->plan_err ? 0u : meta->dev_chunks;
+  }
   if (blockIdx.x == gridDim.x - 1) {
     // The last block: the result words of the call, the pair counter and the
     // boxes' validity and width flags, from every chunk's histogram row.
@@ -351,11 +361,13 @@ __device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get
 // schedule holds: kNarrowMax memberships when no box is wide, else
 // kMaxCoarse) hold more than a quarter of its memberships is flagged
 // kErrDense, and the scatter and test kernels then do nothing.
+template <bool kDev>
 __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
                                                      uint32_t* __restrict__ bin_start, uint64_t cap,
                                                      uint32_t dense_exit, Meta* __restrict__ meta) {
   __shared__ ScanLds L;
   __shared__ unsigned long long heavy;
+  if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (threadIdx.x == 0) heavy = 0ull;
   const uint64_t grand = block_exscan(
       n_bins, L, [&](uint32_t i) { return (uint64_t)total[i]; },
@@ -412,7 +424,7 @@ constexpr uint32_t kFoldBins = 2048;
 
 // kIds: the boxes' ids come from ids_in (virtual instances), else they are
 // the index inside the instance.
-template <bool kIds>
+template <bool kIds, bool kDev>
 __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* __restrict__ boxes,
                                                                  const uint32_t* __restrict__ ids_in,
                                                                  const InstDesc* __restrict__ inst,
@@ -439,6 +451,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
   // bucket come from the same XCD at about the same time and merge in its
   // L2.  The next chunk's boxes and table rows are loaded while the current
   // chunk's staged memberships are written out.
+  if (kDev) n_chunks = meta->plan_err ? 0u : meta->dev_chunks;
   if (!fold) {
     if (meta->memberships > cap) return;  // the bucket would overflow; the host regrows it
     if (meta->error & kErrDense) return;  // the host runs the general path
@@ -962,8 +975,8 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
                                                            uint64_t cap_bucket, uint2* __restrict__ scratch,
                                                            uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
-                                                           uint2* __restrict__ out, uint64_t cap, uint32_t direct,
-                                                           Meta* __restrict__ meta) {
+                                                           uint2* __restrict__ out, uint64_t cap,
+                                                           unsigned long long* out_count, Meta* __restrict__ meta) {
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
@@ -1249,13 +1262,13 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
     atomicMax((unsigned long long*)&meta->items, (unsigned long long)seg0);
     atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
   }
-  if (direct) {
+  if (out_count) {
     // A single instance needs no bin order: the region (every bin's pairs,
     // dropped ones rewound) goes to a range of the output reserved with one
     // device atomic on the pair counter, which ends as the total.
     __syncthreads();
     const uint32_t n = L.out_n < cap_wg ? L.out_n : cap_wg;
-    if (tid == 0) L.out_base = L.out_n ? atomicAdd((unsigned long long*)&meta->n_pairs, (unsigned long long)L.out_n) : 0ull;
+    if (tid == 0) L.out_base = L.out_n ? atomicAdd(out_count, (unsigned long long)L.out_n) : 0ull;
     __syncthreads();
     const uint64_t base = L.out_base;
     for (uint32_t k = tid; k < n; k += kTestThreads)
@@ -1319,7 +1332,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
                                                                   uint2* __restrict__ bin_seg,
                                                                   uint32_t* __restrict__ overfull,
                                                                   uint2* __restrict__ out, uint64_t cap,
-                                                                  uint32_t direct, Meta* __restrict__ meta) {
+                                                                  unsigned long long* out_count,
+                                                                  Meta* __restrict__ meta) {
   NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
@@ -1529,10 +1543,10 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     atomicMax((unsigned long long*)&meta->items, (unsigned long long)seg0);
     atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
   }
-  if (direct) {
+  if (out_count) {
     __syncthreads();
     const uint32_t n = L.out_n < cap_wg ? L.out_n : cap_wg;
-    if (tid == 0) L.out_base = L.out_n ? atomicAdd((unsigned long long*)&meta->n_pairs, (unsigned long long)L.out_n) : 0ull;
+    if (tid == 0) L.out_base = L.out_n ? atomicAdd(out_count, (unsigned long long)L.out_n) : 0ull;
     __syncthreads();
     const uint64_t base = L.out_base;
     for (uint32_t k = tid; k < n; k += kTestThreads)
@@ -1545,6 +1559,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
 // spent on a schedule that has nothing to do.
 constexpr size_t kTestLdsBytes = sizeof(TestLds) > sizeof(NarrowLds) ? sizeof(TestLds) : sizeof(NarrowLds);
 
+template <bool kDev>
 __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restrict__ bbox,
                                                            const uint32_t* __restrict__ bid,
                                                            const uint32_t* __restrict__ bin_start,
@@ -1557,12 +1572,17 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                                                            uint2* __restrict__ out, uint64_t cap, uint32_t direct,
                                                            Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  // direct output: the launch's pair counter (a single instance's own, or
+  // the one its device-planned levels share)
+  unsigned long long* const out_count =
+      kDev ? meta->out_count : (direct ? (unsigned long long*)&meta->n_pairs : nullptr);
+  if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (meta->wide)
     test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                 overfull, out, cap, direct, meta);
+                 overfull, out, cap, out_count, meta);
   else
     test_narrow(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                overfull, out, cap, direct, meta);
+                overfull, out, cap, out_count, meta);
 }
 
 // The pair count and error word, copied to the caller's two result words
@@ -1620,6 +1640,148 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
   }
 }
 
+// ---------------------------------------------------------------------------
+// Re-binning levels planned on the device.
+//
+// After a launch of one instance whose test kernel listed bins beyond LDS,
+// each further level turns the list of the level above into virtual
+// instances (plan_virtual_bin: the bin's region, finer cells, smaller bins)
+// and runs the histogram, scans, scatter and test over them, reading the
+// boxes and ids from the level above's bucket; its test kernel writes its
+// pairs straight to the output through the launch's shared pair counter and
+// lists its own bins beyond LDS for the next level.  No host work between
+// levels: the grids are upper bounds and every count comes from the level's
+// result words, written by its plan kernel (one workgroup).
+
+__global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__ pinst,
+                                                    const uint32_t* __restrict__ pbin_inst,
+                                                    const uint32_t* __restrict__ ptotal,
+                                                    const uint32_t* __restrict__ pstart,
+                                                    const uint32_t* __restrict__ poverfull, uint32_t p_bins,
+                                                    uint32_t p_dev, const Meta* __restrict__ pmeta, uint32_t target,
+                                                    DeepCaps caps, float root_inv_f, uint32_t max_refine,
+                                                    InstDesc* __restrict__ inst, uint32_t* __restrict__ chunk_inst,
+                                                    uint32_t* __restrict__ bin_inst, Meta* __restrict__ meta) {
+  __shared__ ScanLds L;
+  __shared__ uint32_t c0[kDeepMaxInst + 1], b0[kDeepMaxInst + 1];
+  __shared__ uint32_t nbmax, unsplit;
+  const uint32_t tid = threadIdx.x;
+  // (a level planned on the host has no plan error word of its own)
+  uint32_t n_over = p_dev && pmeta->plan_err ? 0u : pmeta->max_per_cell;
+  n_over = n_over < p_bins ? n_over : p_bins;
+  if (tid == 0) {
+    nbmax = 0u;
+    unsplit = 0u;
+  }
+  __syncthreads();
+  const uint32_t n_inst = n_over <= caps.inst && n_over <= kDeepMaxInst ? n_over : 0u;
+  // 1. every listed bin's virtual instance
+  for (uint32_t i = tid; i < n_inst; i += 1024) {
+    const uint32_t b = poverfull[i] & ~kOverfullCell;
+    InstDesc v{};
+    if (b < p_bins) {
+      const uint32_t pi = pbin_inst[b];
+      const InstDesc P = pinst[pi];
+      if (!plan_virtual_bin(P, pi, b - P.bin0, ptotal[b], pstart[b], target, caps.nb_max, &v, root_inv_f,
+                            max_refine)) {
+        v = InstDesc{};
+        atomicOr(&unsplit, 1u);
+      }
+    }
+    inst[i] = v;
+    atomicMax(&nbmax, v.n ? v.nb : 0u);
+  }
+  __syncthreads();
+  // 2. layout: first chunk, bin, histogram and offset row of each instance
+  const uint64_t chunks = block_exscan(
+      n_inst, L, [&](uint32_t i) { return (uint64_t)inst[i].nchunks; },
+      [&](uint32_t i, uint64_t ex) { inst[i].chunk0 = (uint32_t)ex; c0[i] = (uint32_t)ex; });
+  const uint64_t bins = block_exscan(
+      n_inst, L, [&](uint32_t i) { return (uint64_t)(inst[i].n ? inst[i].nb : 0u); },
+      [&](uint32_t i, uint64_t ex) { inst[i].bin0 = (uint32_t)ex; b0[i] = (uint32_t)ex; });
+  const uint64_t hist = block_exscan(
+      n_inst, L, [&](uint32_t i) { return (uint64_t)inst[i].nchunks * hist_pitch(inst[i].nb); },
+      [&](uint32_t i, uint64_t ex) { inst[i].hist0 = ex; });
+  const uint64_t off = block_exscan(
+      n_inst, L, [&](uint32_t i) { return (uint64_t)inst[i].nchunks * off_pitch(inst[i].nb); },
+      [&](uint32_t i, uint64_t ex) { inst[i].off0 = ex; });
+  if (tid == 0) {
+    c0[n_inst] = (uint32_t)chunks;
+    b0[n_inst] = (uint32_t)bins;
+  }
+  __syncthreads();
+  const bool fits = n_inst == n_over && chunks <= caps.chunks && bins <= caps.bins && hist <= caps.hist &&
+                    off <= caps.off && nbmax <= caps.nb_max;
+  // 3. chunk -> instance and bin -> instance (binary search of the starts)
+  if (fits) {
+    auto owner = [&](const uint32_t* st, uint32_t x) {
+      uint32_t lo = 0, hi = n_inst;  // st[lo] <= x < st[hi]
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (st[mid] <= x) lo = mid; else hi = mid;
+      }
+      return lo;
+    };
+    for (uint32_t c = tid; c < (uint32_t)chunks; c += 1024) chunk_inst[c] = owner(c0, c);
+    for (uint32_t b = tid; b < (uint32_t)bins; b += 1024) bin_inst[b] = owner(b0, b);
+  }
+  if (tid == 0) {
+    meta->dev_inst = n_over;
+    meta->dev_chunks = (uint32_t)chunks;
+    meta->dev_bins = (uint32_t)bins;
+    meta->dev_nbmax = nbmax;
+    meta->dev_hist = hist;
+    meta->dev_off = off;
+    meta->plan_err = (fits ? 0u : 1u) | (unsplit ? 2u : 0u);
+  }
+}
+
+// The result words of a launch with device-planned levels: the pair count
+// (the shared counter), and an error word: an invalid box, a workspace that
+// did not fit at any level (a level's plan, bucket or pair regions), or bins
+// beyond LDS that the last level still lists, or that could not be split.
+__global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, uint64_t* __restrict__ result) {
+  if (threadIdx.x != 0) return;
+  unsigned long long e = 0ull;
+  for (uint32_t l = 0; l < m.levels; ++l) {
+    const Meta* x = m.meta[l];
+    e |= x->error & (kErrInvalidBox | kErrWorkspace | kErrTooDense | kErrDense);
+    if (l == 0) continue;  // level 0 is planned on the host
+    if (x->plan_err & 1u) e |= kErrWorkspace;
+    if (x->plan_err & 2u) e |= kErrOverfull;
+  }
+  e |= m.meta[m.levels - 1]->error & kErrOverfull;
+  result[0] = m.meta[0]->n_pairs;
+  result[1] = e;
+}
+
+hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
+                             const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
+                             uint32_t max_refine, uint32_t n_cu, uint2* out, uint64_t cap, hipStream_t s) {
+  k_deep_plan<<<1, 1024, 0, s>>>(parent.inst, parent.bin_inst, parent.total, parent.bin_start, parent.overfull,
+                                 parent_bins, parent_dev ? 1u : 0u, parent.meta, target, caps, root_inv_f, max_refine,
+                                 ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta);
+  const uint32_t nc = caps.chunks ? caps.chunks : 1u, nbins = caps.bins ? caps.bins : 1u;
+  k_bin_hist<true><<<nc, kHistThreads, 0, s>>>(parent.bbox, ws.inst, ws.chunk_inst, 0u, ws.hist, ws.meta);
+  k_bin_scan<true><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(
+      ws.hist, ws.inst, ws.bin_inst, 0u, ws.chunk_inst, 0u, ws.off, ws.total, ws.meta);
+  k_bin_starts<true><<<1, 1024, 0, s>>>(ws.total, 0u, ws.bin_start, ws.cap_bucket, 0u, ws.meta);
+  const uint32_t per_xcd = (nc + 7u) / 8u < n_cu / 8u ? (nc + 7u) / 8u : (n_cu >= 8u ? n_cu / 8u : 1u);
+  const size_t scatter_lds = sizeof(ScatterLds) + 8u * (caps.nb_max ? caps.nb_max : 1u);
+  k_bin_scatter<true, true><<<per_xcd * 8u, kScatterThreads, scatter_lds, s>>>(
+      parent.bbox, parent.bid, ws.inst, ws.chunk_inst, 0u, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
+      ws.cap_bucket, caps.nb_max ? caps.nb_max : 1u, ws.total, 0u, 0u, ws.meta);
+  k_bin_test<true><<<n_cu ? n_cu : 1u, kTestThreads, kTestLdsBytes, s>>>(
+      ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, ws.bin_inst, 0u, ws.cap_bucket, ws.scratch, ws.cap_wg,
+      ws.bin_seg, ws.overfull, out, cap, 1u, ws.meta);
+  return hipGetLastError();
+}
+
+hipError_t launch_deep_result(const DeepMetas& m, uint64_t* result, hipStream_t s) {
+  k_deep_result<<<1, 64, 0, s>>>(m, result);
+  return hipGetLastError();
+}
+
 // Place runs of pairs found outside the launch (block r copies run r).
 __global__ __launch_bounds__(256) void k_splice_copy(const SpliceRun* __restrict__ runs,
                                                       const uint64_t* __restrict__ inst_out,
@@ -1653,15 +1815,17 @@ hipError_t binned_init() {
                                               (int)kPlaceLdsBytes);
     if (e0 != hipSuccess) return e0;
   }
-  hipError_t e = hipFuncSetAttribute((const void*)k_bin_test, hipFuncAttributeMaxDynamicSharedMemorySize,
+  hipError_t e = hipFuncSetAttribute((const void*)k_bin_test<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kTestLdsBytes);
   if (e != hipSuccess) return e;
   const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
-  e = hipFuncSetAttribute((const void*)k_bin_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          scatter_max);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_bin_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             scatter_max);
+  for (const void* f : {(const void*)k_bin_test<true>, (const void*)k_bin_scatter<false, false>,
+                        (const void*)k_bin_scatter<true, false>, (const void*)k_bin_scatter<true, true>}) {
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            f == (const void*)k_bin_test<true> ? (int)kTestLdsBytes : scatter_max);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s) {
@@ -1712,7 +1876,7 @@ hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, c
                                                                   op.fill_words, ws.bbox, ws.bid, op.bin_cap, stage,
                                                                   op.meta, op.meta_next);
   PBIG_MARK(1)
-  k_bin_test<<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(
+  k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(
       ws.bbox, ws.bid, op.bin_start, op.fill, ws.inst, ws.bin_inst, bp.n_bins, (uint64_t)bp.n_bins * op.bin_cap,
       ws.scratch, ws.cap_wg, ws.bin_seg, ws.overfull, out, cap, 1u, op.meta);
   PBIG_MARK(2)
@@ -1731,14 +1895,14 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 #define PBIG_MARK(k) \
   if (ev && (e = hipEventRecord(ev[k], s)) != hipSuccess) return e;
   PBIG_MARK(0)
-  k_bin_hist<<<nc, kHistThreads, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist);
+  k_bin_hist<false><<<nc, kHistThreads, 0, s>>>(boxes, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.meta);
   PBIG_MARK(1)
   // one block more than the bins need: the last one makes the call's result words
-  k_bin_scan<<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
+  k_bin_scan<false><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
   const uint32_t fold = bp.n_inst == 1u && bp.nb_max <= kFoldBins ? 1u : 0u;
-  if (!fold) k_bin_starts<<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
+  if (!fold) k_bin_starts<false><<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + (fold ? 12u : 8u) * nb_cap;
@@ -1748,15 +1912,15 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   const uint32_t xcd_share = bp.n_inst == 1u && bp.n_cu >= 16u ? bp.n_cu / 8u : (nc + 7u) / 8u;
   const uint32_t sgrid = (nc + 7u) / 8u < xcd_share ? (nc + 7u) / 8u * 8u : xcd_share * 8u;
   if (ids_in)
-    k_bin_scatter<true><<<sgrid, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<true, false><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, ids_in, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.total, fold, bp.dense_exit, ws.meta);
   else
-    k_bin_scatter<false><<<sgrid, kScatterThreads, scatter_lds, s>>>(
+    k_bin_scatter<false, false><<<sgrid, kScatterThreads, scatter_lds, s>>>(
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.total, fold, bp.dense_exit, ws.meta);
   PBIG_MARK(4)
-  k_bin_test<<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
+  k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
                                                                    ws.overfull, out, cap, bp.direct, ws.meta);

@@ -72,6 +72,15 @@ struct BinnedState {
   uint64_t cap_start1 = 0;
   uint32_t start1_nb = 0, start1_cap = 0;
   uint32_t parity = 0;
+  // Re-binning levels planned on the device (levels 1.. of the plain path's
+  // two-pass launch), each with its own workspace and bounds.  deep_ok: the
+  // plan's last synchronous call completed on them, so asynchronous calls
+  // take them too; deep_failed: they cannot complete this plan (bins that do
+  // not split, or more levels than there are).
+  std::unique_ptr<BinnedState> dl[pbig::kDeepLevels];
+  pbig::DeepCaps dcaps[pbig::kDeepLevels];
+  bool deep_ok = false, deep_failed = false;
+  uint64_t* d_res = nullptr;  // result words of a synchronous call on them
 };
 
 struct pbig_hip_ctx {
@@ -111,6 +120,9 @@ struct pbig_hip_ctx {
 
 namespace {
 constexpr int kEventsPerCall = 8;
+// Timing path id of the plain path's launch with device re-binning levels:
+// one duration per level (level 0 is the two-pass launch).
+constexpr int kPathDeep = 4;
 // Timing path id of the single-pass launch (place, test), beside
 // PBIG_PATH_GENERAL and PBIG_PATH_BINNED.
 constexpr int kPathOnePass = 3;
@@ -338,54 +350,11 @@ bool plan_instance(uint32_t n, double range, float cell, uint32_t bin_target, In
   return true;
 }
 
-// Plan of a virtual instance: coarse bin lb of instance P, holding m
-// memberships at box0 of P's launch's bin-ordered arrays.  Its fine cells are
-// P's cells split 2^j ways per side, j chosen so that a cell holds about two
-// memberships at the bin's density (dense cells cost quadratically): the
-// cell side is P's divided by 2^j, and inv_f * 2^j is exact, so a point's
-// cell at this level lies inside its cell at P's level and the
-// reference-cell rule claims every pair in the same region as at P's level.
-// Bins of about `target` memberships, at most kMaxK cells per side.  False
-// when the bin cannot be split.
+// Plan of a virtual instance (pbig::plan_virtual_bin, shared with the
+// device planner of re-binning levels).
 bool plan_virtual(const InstDesc& P, uint32_t parent, uint32_t lb, uint32_t m, uint64_t box0, uint32_t target,
                   InstDesc* v) {
-  *v = InstDesc{};
-  v->n = m;
-  v->box0 = box0;
-  v->parent = parent;
-  const uint32_t fx0 = P.fx0 + (lb % P.nbx) * P.k, fy0 = P.fy0 + (lb / P.nbx) * P.k;
-  const uint32_t sx = std::min<uint32_t>(P.k, P.fx0 + P.span_x - fx0);
-  const uint32_t sy = std::min<uint32_t>(P.k, P.fy0 + P.span_y - fy0);
-  const double per_cell = (double)m / ((double)sx * sy);
-  uint32_t j = 0;
-  while (j < 6 && per_cell / (double)(1u << (2 * j)) > 2.0 && ((uint64_t)P.nf << (j + 1)) <= (1u << 21)) ++j;
-  const uint32_t f = 1u << j;
-  v->inv_f = P.inv_f * (float)f;
-  v->nf = P.nf * f;
-  v->fx0 = fx0 * f;
-  v->fy0 = fy0 * f;
-  v->span_x = sx * f;
-  v->span_y = sy * f;
-  const uint32_t side = std::max(v->span_x, v->span_y);
-  if (side <= 1) return false;
-  // bins: about m / target of them, square, k <= kMaxK, at most kMaxBins
-  const double nbx_want = std::max(2.0, std::ceil(std::sqrt((double)m / std::max<uint32_t>(target, 16))));
-  uint32_t k = (uint32_t)std::ceil(side / nbx_want);
-  k = std::min<uint32_t>(std::max<uint32_t>(k, 1), pbig::kMaxK);
-  if (j == 0) k = std::min<uint32_t>(k, side - 1);  // same cells: the bins must shrink
-  while (((side + k - 1) / k) * ((side + k - 1) / k) > pbig::kMaxBins) {
-    if (k >= pbig::kMaxK) return false;
-    ++k;
-  }
-  v->k = k;
-  v->k_magic = ((1ull << 32) + k - 1) / k;
-  v->nbx = (side + k - 1) / k;
-  v->nb = v->nbx * v->nbx;
-  uint64_t nw = ((uint64_t)m + pbig::kScatterChunk - 1) / pbig::kScatterChunk;
-  if (nw > 8) nw = (nw + 7) / 8 * 8;
-  v->chunk = (uint32_t)std::min<uint64_t>(pbig::kScatterChunk, ((uint64_t)m + nw - 1) / nw);
-  v->nchunks = m ? (uint32_t)(((uint64_t)m + v->chunk - 1) / v->chunk) : 0u;
-  return true;
+  return pbig::plan_virtual_bin(P, parent, lb, m, box0, target, pbig::kMaxBins, v);
 }
 
 // Global layout of a planned batch: first chunk, bin, histogram row, offset
@@ -541,8 +510,16 @@ void free_state(BinnedState& st) {
     if (p) (void)hipFree(p);
   if (st.own_meta && b.meta) (void)hipFree(b.meta);
   st.ws = BinnedWorkspace{};
-  for (void* p : {(void*)st.fill[0], (void*)st.fill[1], (void*)st.meta1, (void*)st.start1})
+  for (void* p : {(void*)st.fill[0], (void*)st.fill[1], (void*)st.meta1, (void*)st.start1, (void*)st.d_res})
     if (p) (void)hipFree(p);
+  st.d_res = nullptr;
+  for (auto& lv : st.dl)
+    if (lv) {
+      free_state(*lv);
+      lv.reset();
+    }
+  for (auto& k : st.dcaps) k = pbig::DeepCaps{};
+  st.deep_ok = st.deep_failed = false;
   st.fill[0] = st.fill[1] = st.start1 = nullptr;
   st.meta1 = nullptr;
   st.cap_fill = st.cap_start1 = 0;
@@ -612,6 +589,7 @@ int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_
                      ? std::min<uint32_t>(pbig::kNarrowMax, ((uint32_t)std::ceil(want) + 63u) & ~63u)
                      : 0u;
     st.onepass_failed = false;
+    st.deep_ok = st.deep_failed = false;
   }
   st.ws.meta = c->ws.meta;
   int rc;
@@ -780,6 +758,179 @@ int run_onepass_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint6
   return 1;
 }
 
+// ---------------------------------------------------------------------------
+// The plain path's two-pass launch followed by re-binning levels planned on
+// the device (no host work between levels).
+
+// Workspace of device-planned level l (1..) for its bounds; its result words
+// point at level 0's pair counter.
+int ensure_deep_level(pbig_hip_ctx* c, BinnedState& st, uint32_t l, uint64_t bucket) {
+  if (!st.dl[l]) {
+    st.dl[l] = std::make_unique<BinnedState>();
+    BinnedState& L = *st.dl[l];
+    HIP_TRY(c, hipMalloc((void**)&L.ws.meta, sizeof(Meta)));
+    L.own_meta = true;
+    Meta h{};
+    h.out_count = (unsigned long long*)&st.ws.meta->n_pairs;
+    HIP_TRY(c, hipMemcpy(L.ws.meta, &h, sizeof(Meta), hipMemcpyHostToDevice));
+  }
+  BinnedState& L = *st.dl[l];
+  const pbig::DeepCaps& k = st.dcaps[l];
+  L.bp = BatchPlan{};
+  L.bp.n_inst = k.inst;
+  L.bp.n_chunks = k.chunks;
+  L.bp.n_bins = k.bins;
+  L.bp.hist_entries = k.hist;
+  L.bp.off_entries = k.off;
+  L.bp.test_blocks = (uint32_t)std::max(c->n_cu, 1);
+  L.bp.n_cu = (uint32_t)c->n_cu;
+  if (!L.ws.cap_wg) L.ws.cap_wg = (uint32_t)std::max<uint64_t>(8192, 2 * bucket / L.bp.test_blocks);
+  return ensure_state(c, L, bucket);
+}
+
+// Cap on the device levels' cell refinement (powers of two below the
+// instance's own cell).  PBIG_DEEP_REFINE overrides it (for measurements).
+uint32_t deep_refine() {
+  static const uint32_t v = [] {
+    const char* e = getenv("PBIG_DEEP_REFINE");
+    return e ? (uint32_t)atoi(e) : 2u;
+  }();
+  return v;
+}
+
+// First bounds of the levels, from level 0's plan and bucket; the
+// synchronous driver grows them to what the levels report.
+void init_deep_caps(BinnedState& st) {
+  const uint64_t m0 = std::max<uint64_t>(st.ws.cap_bucket, 4096);
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+    pbig::DeepCaps& k = st.dcaps[l];
+    k.inst = std::min<uint32_t>(2048, std::max<uint32_t>(64, st.bp.n_bins / 4));
+    k.chunks = (uint32_t)std::min<uint64_t>(0x7FFFFFFF, m0 / 4096 + k.inst);
+    k.bins = k.inst * 16;
+    k.nb_max = 1024;
+    k.hist = (uint64_t)k.chunks * pbig::hist_pitch(64);
+    k.off = (uint64_t)k.chunks * pbig::off_pitch(64);
+  }
+}
+
+// Enqueue level 0 (the two-pass launch, direct output) and the levels, then
+// the result words into d_result (optional).
+int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
+                 uint64_t* d_result, hipStream_t s) {
+  hipEvent_t* ev = next_events(c, kPathDeep);
+  if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
+  st.bp.direct = 1u;
+  st.bp.dense_exit = 0u;
+  HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, nullptr, st.bp, d_out, cap, nullptr, s, nullptr));
+  if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
+  const BinnedWorkspace* parent = &st.ws;
+  uint32_t parent_bins = st.bp.n_bins;
+  pbig::DeepMetas dm{};
+  dm.meta[0] = st.ws.meta;
+  const uint32_t target = std::max<uint32_t>(c->cfg.bin_target / 2, 64);
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+    const BinnedState& L = *st.dl[l];
+    HIP_TRY(c, pbig::launch_deep_level(*parent, parent_bins, l > 1, L.ws, st.dcaps[l], target, st.inst[0].inv_f,
+                                       deep_refine(), (uint32_t)c->n_cu, d_out, cap, s));
+    if (ev) HIP_TRY(c, hipEventRecord(ev[l + 1], s));
+    dm.meta[l] = L.ws.meta;
+    parent = &L.ws;
+    parent_bins = st.dcaps[l].bins;
+  }
+  dm.levels = pbig::kDeepLevels;
+  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, d_result, s));
+  return PBIG_OK;
+}
+
+// The launch with device levels, synchronously.  0: done (*m: pair count,
+// level 0's memberships); 1: the levels cannot complete this plan (the host
+// re-binning runs instead); < 0: an error.  Grows every level's workspace
+// to what it reports and runs again (a few times at most).
+int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
+  BinnedState& st = c->plain;
+  if (st.deep_failed || st.bp.n_inst != 1 || st.bp.n_bins == 0) return 1;
+  int rc;
+  if (!st.d_res) HIP_TRY(c, hipMalloc((void**)&st.d_res, 2 * sizeof(uint64_t)));
+  if (!st.dcaps[1].inst) init_deep_caps(st);
+  std::vector<uint64_t> bucket(pbig::kDeepLevels, 0);
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+    bucket[l] = st.dl[l] ? st.dl[l]->ws.cap_bucket : 2 * std::max<uint64_t>(st.ws.cap_bucket, 4096);
+  const uint32_t saved_exit = st.bp.dense_exit;
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+      if ((rc = ensure_deep_level(c, st, l, bucket[l]))) return rc;
+    if ((rc = after_upload(c, st, s))) return rc;
+    if ((rc = enqueue_deep(c, st, d_boxes, d_out, cap, st.d_res, s))) return rc;
+    st.bp.dense_exit = saved_exit;
+    uint64_t res[2];
+    std::vector<Meta> lm(pbig::kDeepLevels);
+    HIP_TRY(c, hipMemcpyAsync(res, st.d_res, sizeof(res), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&lm[0], st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+      HIP_TRY(c, hipMemcpyAsync(&lm[l], st.dl[l]->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (res[1] == 0) {
+      *m = lm[0];
+      m->n_pairs = res[0];
+      m->error = 0;
+      st.deep_ok = true;
+      // The levels' grids are their bounds: tighten them to what this plan
+      // needs (a quarter more), so that asynchronous calls launch no
+      // workgroups that find nothing to do.  The buffers stay as grown.
+      for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+        const Meta& x = lm[l];
+        pbig::DeepCaps& k = st.dcaps[l];
+        k.inst = std::max<uint32_t>(x.dev_inst + x.dev_inst / 4, 16);
+        k.chunks = std::max<uint32_t>(x.dev_chunks + x.dev_chunks / 4, 16);
+        k.bins = std::max<uint32_t>(x.dev_bins + x.dev_bins / 4, 64);
+        k.hist = std::max<uint64_t>(x.dev_hist + x.dev_hist / 4, 4096);
+        k.off = std::max<uint64_t>(x.dev_off + x.dev_off / 4, 4096);
+      }
+      return 0;
+    }
+    if (res[1] & pbig::kErrInvalidBox)
+      return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    bool grown = false;
+    // level 0: bucket and pair regions
+    if (lm[0].error & pbig::kErrWorkspace) {
+      if (lm[0].items > st.ws.cap_wg)
+        st.ws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, lm[0].items + lm[0].items / 4 + 1024);
+      uint64_t b0 = std::max<uint64_t>(st.ws.cap_bucket, lm[0].memberships + lm[0].memberships / 16 + 4096);
+      if (lm[0].memberships <= 0xFFFFFFFFull) b0 = std::min<uint64_t>(b0, 0xFFFFFFFFull);
+      if ((rc = ensure_state(c, st, b0))) return rc;
+      grown = true;
+    }
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+      const Meta& x = lm[l];
+      pbig::DeepCaps& k = st.dcaps[l];
+      if (x.plan_err & 1u) {
+        if (x.dev_inst > pbig::kDeepMaxInst) continue;  // more bins beyond LDS than a level takes
+        k.inst = std::max(k.inst, x.dev_inst + x.dev_inst / 4 + 16);
+        k.chunks = std::max(k.chunks, x.dev_chunks + x.dev_chunks / 4 + 64);
+        k.bins = std::max(k.bins, x.dev_bins + x.dev_bins / 4 + 256);
+        k.hist = std::max<uint64_t>(k.hist, x.dev_hist + x.dev_hist / 4 + 4096);
+        k.off = std::max<uint64_t>(k.off, x.dev_off + x.dev_off / 4 + 4096);
+        grown = true;
+      }
+      if (x.error & pbig::kErrWorkspace) {
+        BinnedState& L = *st.dl[l];
+        if (x.items > L.ws.cap_wg) {
+          L.ws.cap_wg = (uint32_t)std::min<uint64_t>(0x7FFFFFFFull, x.items + x.items / 4 + 1024);
+          grown = true;
+        }
+        if (x.memberships > L.ws.cap_bucket) {
+          if (x.memberships > 0xFFFFFFFFull) continue;
+          bucket[l] = std::min<uint64_t>(x.memberships + x.memberships / 8 + 4096, 0xFFFFFFFFull);
+          grown = true;
+        }
+      }
+    }
+    if (!grown) break;  // bins the levels cannot split or finish
+  }
+  st.deep_failed = true;
+  return 1;
+}
+
 // Pairs of a launch's instances found outside it (by re-binning their
 // heavy bins one level down, or by the general path): per instance the
 // extra count and the device runs holding those pairs, in order; and the
@@ -916,6 +1067,7 @@ int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_
   if (!st.sub) {
     st.sub = std::make_unique<BinnedState>();
     HIP_TRY(c, hipMalloc((void**)&st.sub->ws.meta, sizeof(Meta)));
+    HIP_TRY(c, hipMemset(st.sub->ws.meta, 0, sizeof(Meta)));
     st.sub->own_meta = true;
   }
   BinnedState& V = *st.sub;
@@ -979,7 +1131,9 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
   Meta m;
   int rc = run_onepass_sync(c, d_boxes, d_out, cap, s, &m);
   if (rc < 0) return rc;
-  if (rc == 1) {  // the two-pass launch, with re-binning of bins beyond LDS
+  if (rc == 1) rc = run_deep_sync(c, d_boxes, d_out, cap, s, &m);  // re-binning levels on the device
+  if (rc < 0) return rc;
+  if (rc == 1) {  // the two-pass launch, with re-binning of bins beyond LDS on the host
     Splice sp;
     std::vector<uint8_t> failed;
     if ((rc = resolve(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, 0, &m, &sp, &failed))) return rc;
@@ -1066,6 +1220,11 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
   c->ws.test_blocks = (uint32_t)c->n_cu * 8;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PBIG_E_HIP; }
   if (hipMalloc((void**)&c->ws.meta, sizeof(Meta)) != hipSuccess) { delete c; return PBIG_E_NOMEM; }
+  if (hipMemset(c->ws.meta, 0, sizeof(Meta)) != hipSuccess) {
+    (void)hipFree(c->ws.meta);
+    delete c;
+    return PBIG_E_HIP;
+  }
   if (hipHostMalloc((void**)&c->h_meta, sizeof(Meta), hipHostMallocDefault) != hipSuccess) {
     (void)hipFree(c->ws.meta);
     delete c;
@@ -1210,6 +1369,10 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
       if ((rc = launch_onepass(c, c->plain, (const float4*)d_boxes, (uint2*)d_pairs, cap, s, &dm))) return rc;
       if (d_result) HIP_TRY(c, pbig::launch_result(dm, d_result, s));
       return PBIG_OK;
+    }
+    if (c->plain.deep_ok && !c->plain.deep_failed) {
+      if ((rc = after_upload(c, c->plain, s))) return rc;
+      return enqueue_deep(c, c->plain, (const float4*)d_boxes, (uint2*)d_pairs, cap, d_result, s);
     }
     HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, nullptr, c->plain.bp, (uint2*)d_pairs, cap,
                                    nullptr, s, next_events(c, PBIG_PATH_BINNED)));
@@ -1390,6 +1553,7 @@ int pbig_hip_timing_read(pbig_hip_ctx* c, double* ms_per_kernel, int n_kernels, 
   HIP_TRY(c, hipSetDevice(c->device));
   const int nk = c->timing_path == PBIG_PATH_BINNED ? pbig::kBinnedKernels
                  : c->timing_path == kPathOnePass ? pbig::kOnePassKernels
+                 : c->timing_path == kPathDeep    ? (int)pbig::kDeepLevels
                                                    : pbig::kPlainKernels;
   for (int k = 0; k < n_kernels; ++k) ms_per_kernel[k] = 0.0;
   for (int call = 0; call < c->timing_used; ++call) {
@@ -1540,7 +1704,9 @@ int pbig_hip_batch_create(pbig_hip_ctx* c, const pbig_hip_instance* inst, uint32
   HIP_TRY(c, hipSetDevice(c->device));
   pbig_hip_batch* b = new pbig_hip_batch();
   b->ctx = c;
-  if (hipMalloc((void**)&b->st.ws.meta, sizeof(Meta)) != hipSuccess) {
+  if (hipMalloc((void**)&b->st.ws.meta, sizeof(Meta)) != hipSuccess ||
+      hipMemset(b->st.ws.meta, 0, sizeof(Meta)) != hipSuccess) {
+    if (b->st.ws.meta) (void)hipFree(b->st.ws.meta);
     delete b;
     return fail(c, PBIG_E_NOMEM, "hipMalloc of the batch's result words failed");
   }

@@ -34,8 +34,18 @@ struct Meta {
   unsigned long long items;
   unsigned int max_per_cell;
   unsigned int wide;  // binned path: some box spans more than two fine cells on an axis
-  unsigned int pad[6];
+  // A re-binning level planned on the device: the sizes its plan kernel
+  // wrote (also when they exceed the level's workspace, which plan_err then
+  // flags), read by the level's other kernels.
+  unsigned int dev_inst, dev_chunks, dev_bins, dev_nbmax;
+  unsigned long long dev_hist, dev_off;
+  // Direct output shared by a launch's levels: the pair counter every
+  // level's test kernel reserves its output from (device-planned levels).
+  unsigned long long* out_count;
+  unsigned int plan_err;  // the level's plan did not fit its workspace
+  unsigned int pad[3];
 };
+static_assert(sizeof(Meta) == 96, "Meta layout");
 
 struct ScanPartial {
   unsigned long long memberships;

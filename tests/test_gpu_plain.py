@@ -343,14 +343,14 @@ def test_binned_c3_rebins_heavy_bins(torch_cuda, binned, orc, manifest, cs):
     assert orc.pair_hash(got) == m["sha256"]
 
 
-def test_auto_sends_dense_instance_to_general_path(torch_cuda, orc, manifest):
+def test_auto_dense_instance_on_device_levels(torch_cuda, orc, manifest):
     """The clustered instance under the automatic choice: the first
     asynchronous call tries the single-pass launch, whose dense bins outgrow
-    their fixed share of the bucket, and stops with error bit 5 and no pairs
-    (the two-pass launch would stop after the bin totals with bit 4: heavy
-    bins hold most memberships); the synchronous call settles the path (the
-    general one) and the next asynchronous call follows it, with the exact
-    set."""
+    their fixed share of the bucket, and stops with error bit 5 and no pairs;
+    the synchronous call settles the path: the two-pass launch followed by
+    re-binning levels planned on the device (the bins beyond LDS split into
+    finer ones with no host work between levels), and the next asynchronous
+    calls run that whole sequence on the device, with the exact set."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -366,13 +366,15 @@ def test_auto_sends_dense_instance_to_general_path(torch_cuda, orc, manifest):
         n_async, err = (int(v) for v in res.cpu().tolist())
         assert err in (16, 32) and n_async == 0, (n_async, err)
         n = e.check_device(d, L, 64.0, out)
-        assert e.stats()["path"] == PATHS["general"]
+        assert e.stats()["path"] == PATHS["binned"]
         assert n == m["pairs"] and orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
-        res.zero_()
-        e.check_device_async(d, L, 64.0, out, res)
-        torch_cuda.cuda.synchronize()
-        assert res.cpu().tolist() == [m["pairs"], 0]
-        assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
+        for rep in range(2):
+            res.zero_()
+            out.fill_(-1)
+            e.check_device_async(d, L, 64.0, out, res)
+            torch_cuda.cuda.synchronize()
+            assert res.cpu().tolist() == [m["pairs"], 0], rep
+            assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"], rep
 
 
 def test_binned_deep_rebinning(torch_cuda, binned, orc):
@@ -451,3 +453,30 @@ def test_auto_keeps_large_narrow_bins_binned(torch_cuda, orc, manifest):
         assert orc.pair_hash(got) == m["sha256"]
     finally:
         eng.close()
+
+
+def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc):
+    """6,000 identical boxes share one fine cell at every refinement: the
+    device re-binning levels cannot split their bin and report it, and the
+    synchronous call falls back (host re-binning, then the general path) with
+    the exact set; the asynchronous call after it follows the path that
+    worked."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    same = np.tile(np.array([[100.0, 101.0, 100.0, 101.0]], np.float32), (6000, 1))
+    rest = workloads.uniform(20000, 4096.0, 1.0, 8.0, 77)
+    boxes = np.concatenate([rest[:7000], same, rest[7000:]]).astype(np.float32)
+    want = orc.sort_pairs(orc.sweep_pairs(boxes))
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((len(want) + 4096, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["auto"], 2000)
+        n = e.check_device(d, 4096.0, 8.0, out)
+        assert n == len(want)
+        assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
+        e.check_device_async(d, 4096.0, 8.0, out, res)
+        torch_cuda.cuda.synchronize()
+        assert res.cpu().tolist() == [len(want), 0]
+        assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
