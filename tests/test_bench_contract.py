@@ -21,11 +21,14 @@ def test_alg_bytes_of_the_4m_instance():
 
 
 def test_committed_traffic_profile_is_read():
-    traffic, src = bench.read_traffic("binned", 32.0, 4096)  # bench.py's defaults
-    assert src == os.path.join("profiles", "traffic_plain_uniform_4m_binned_cs32_bt4096.txt")
-    for k in ("bin_hist", "bin_scan", "bin_scatter", "bin_test"):
+    # bench.py's defaults run the single-pass launch (place, test)
+    traffic, src = bench.read_traffic("onepass", 32.0, 4096)
+    assert src == os.path.join("profiles", "traffic_plain_uniform_4m_onepass_cs32_bt4096.txt")
+    for k in ("bin_place", "bin_test"):
         assert traffic[k] > 0, k
-    # the test kernel reads at least its bucket: 20 bytes per coarse membership
+    # the place kernel reads every box and writes a 20-byte bucket entry per
+    # coarse membership; the test kernel reads at least the bucket
+    assert traffic["bin_place"] > 16 * 4194304 + 20 * 4194304
     assert traffic["bin_test"] > 20 * 4194304
 
 
