@@ -138,7 +138,7 @@ def clustered_leg(torch, eng_cls, steps=50, reps=5):
     dev = torch.device("cuda:0")
     d = torch.from_numpy(boxes).to(dev)
     pairs = torch.empty((40 * n, 2), dtype=torch.int32, device=dev)
-    cell = float(autotune.cell_sizes([0, n], [L])[0])
+    cell = float(autotune.cell_sizes([0, n], [L], boxes=boxes)[0])
     ref_cs = pbig_amd.reference_cell_size(boxes, L)
     out = {"workload": "2,097,152 float32 rects, 90% in 64 Gaussian clusters (sigma 327.68) + 10% uniform "
                        "in [0,65536)^2, log-normal sides, seed 3, boxes resident in HBM",
@@ -274,7 +274,7 @@ def batched_leg(torch, steps, warmup, baseline=True):
     k = offsets.shape[0] - 1
     N = int(boxes.shape[0])
     table = autotune.load_table()
-    cells = autotune.cell_sizes(offsets, ranges, table)
+    cells = autotune.cell_sizes(offsets, ranges, table, boxes)
     dev = torch.device("cuda:0")
     eng = pbig_amd.Engine(0)
     eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
@@ -394,7 +394,7 @@ def serving_leg(torch, count, rate, budget_gb):
     for q in spec:
         b = workloads.request_boxes(q)
         r = Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
-                    cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table)[0]))
+                    cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table, b)[0]))
         r.host = torch.from_numpy(b).pin_memory()
         r.device = r.host.to(dev)
         reqs.append(r)

@@ -3,7 +3,12 @@
 * Fine cell side.  Expressed as boxes per cell, q = rho * c^2 (rho = boxes per
   unit area of the instance's domain), so that one value serves instances of
   any size and density.  q is tuned per size class: small instances pay more
-  per-bin fixed cost and prefer other cells than large ones.
+  per-bin fixed cost and prefer other cells than large ones.  When the boxes
+  are given, the cell is never narrower than the widest box of a sample (the
+  first 1,000 boxes, as the reference's cell-size rule samples them,
+  src/pbig/pbig.cu:1296-1308) if their sides are bounded (sample maximum at
+  most 2.5 times the sample median): then every box spans at most two cells
+  per axis and the narrow-box test schedule runs, whatever q is.
 * Work tile: the number of boxes a coarse bin (the unit one test workgroup
   holds in LDS) aims for, the context's ``bin_target``.
 
@@ -64,18 +69,36 @@ def class_of(n: int, table: dict) -> int:
     return len(table["classes"]) - 1
 
 
-def cell_side(n: int, L: float, q: float) -> float:
-    """Cell side holding q box origins on average: sqrt(q L^2 / n), rounded
-    to a float32 value; at least L / 4096 and at most L."""
+SAMPLE = 1000        # boxes sampled for the side floor
+BOUNDED_RATIO = 2.5  # sample max / sample median at most this: the sides are bounded
+HEADROOM = 1.1       # the floor's margin over the sample's widest side
+
+
+def side_floor(boxes) -> float:
+    """HEADROOM times the widest side among the first SAMPLE boxes when the
+    sides look bounded (max <= BOUNDED_RATIO * median), else 0 (no floor)."""
+    b = np.asarray(boxes[:SAMPLE], dtype=np.float32).reshape(-1, 4)
+    if b.shape[0] == 0:
+        return 0.0
+    side = np.maximum(b[:, 1] - b[:, 0], b[:, 3] - b[:, 2]).astype(np.float64)
+    mx, med = float(side.max()), float(np.median(side))
+    return HEADROOM * mx if mx <= BOUNDED_RATIO * med else 0.0
+
+
+def cell_side(n: int, L: float, q: float, floor: float = 0.0) -> float:
+    """Cell side holding q box origins on average, sqrt(q L^2 / n), and at
+    least `floor`, rounded to a float32 value; at least L / 4096 and at most
+    L."""
     if n <= 0:
         return float(np.float32(max(L, 1.0)))
-    c = math.sqrt(q * L * L / n)
+    c = max(math.sqrt(q * L * L / n), floor)
     c = min(max(c, L / 4096.0, 1e-3), max(L, 1e-3))
     return float(np.float32(c))
 
 
-def cell_sizes(offsets, ranges, table: dict | None = None) -> np.ndarray:
-    """Per-instance fine cell sides for a batch (offsets: k+1 box offsets)."""
+def cell_sizes(offsets, ranges, table: dict | None = None, boxes=None) -> np.ndarray:
+    """Per-instance fine cell sides for a batch (offsets: k+1 box offsets;
+    boxes, optional: the concatenated boxes, for the side floor)."""
     t = table or load_table()
     offsets = np.asarray(offsets, dtype=np.int64)
     n = np.diff(offsets)
@@ -83,7 +106,8 @@ def cell_sizes(offsets, ranges, table: dict | None = None) -> np.ndarray:
     out = np.empty(n.shape[0], dtype=np.float32)
     for i in range(n.shape[0]):
         q = t["classes"][class_of(int(n[i]), t)]["boxes_per_cell"]
-        out[i] = cell_side(int(n[i]), float(ranges[i]), q)
+        fl = side_floor(boxes[offsets[i]:offsets[i + 1]]) if boxes is not None else 0.0
+        out[i] = cell_side(int(n[i]), float(ranges[i]), q, fl)
     return out
 
 
@@ -119,7 +143,7 @@ def time_batch(engine, d_boxes, starts, counts, ranges, cells, reps: int = 5) ->
         return s.elapsed_time(e) / reps
 
 
-def tune(engine, boxes, offsets, ranges, qs=(0.5, 1.0, 2.0, 4.0, 8.0), bin_targets=(1000, 2000, 3000, 3500),
+def tune(engine, boxes, offsets, ranges, qs=(0.25, 0.5, 1.0, 2.0, 4.0), bin_targets=(1000, 2000, 3000, 4096),
          reps: int = 5, base: dict | None = None, log=print) -> dict:
     """Pick boxes-per-cell per size class, then the bin target, by timing
     graph replays of the workload's instances on the GPU."""
@@ -131,6 +155,7 @@ def tune(engine, boxes, offsets, ranges, qs=(0.5, 1.0, 2.0, 4.0, 8.0), bin_targe
     n = np.diff(offsets)
     d = torch.from_numpy(np.ascontiguousarray(boxes, dtype=np.float32)).to("cuda:0")
     measured = {"classes": [], "bin_target": {}}
+    floors = np.array([side_floor(boxes[offsets[i]:offsets[i + 1]]) for i in range(n.shape[0])])
     engine.configure(0, t["bin_target"])
     for k, cls in enumerate(t["classes"]):
         idx = np.array([i for i in range(n.shape[0]) if class_of(int(n[i]), t) == k], dtype=np.int64)
@@ -139,12 +164,12 @@ def tune(engine, boxes, offsets, ranges, qs=(0.5, 1.0, 2.0, 4.0, 8.0), bin_targe
             continue
         times = {}
         for q in qs:
-            cells = np.array([cell_side(int(n[i]), float(ranges[i]), q) for i in idx], dtype=np.float32)
+            cells = np.array([cell_side(int(n[i]), float(ranges[i]), q, floors[i]) for i in idx], dtype=np.float32)
             times[q] = time_batch(engine, d, offsets[idx], n[idx], ranges[idx], cells, reps)
             log(f"autotune class {k} ({idx.size} instances): q={q} -> {times[q]:.3f} ms")
         cls["boxes_per_cell"] = float(min(times, key=times.get))
         measured["classes"].append({str(q): v for q, v in times.items()})
-    cells = cell_sizes(offsets, ranges, t)
+    cells = cell_sizes(offsets, ranges, t, boxes)
     for bt in bin_targets:
         engine.configure(0, int(bt))
         measured["bin_target"][str(bt)] = time_batch(engine, d, offsets[:-1], n, ranges, cells, reps)

@@ -210,7 +210,8 @@ class Server:
         if req.n > self.pool.capacity:
             raise ValueError(f"request {req.rid}: {req.n} boxes exceed the pool of {self.pool.capacity}")
         if req.cell_size <= 0:
-            req.cell_size = float(autotune.cell_sizes([0, req.n], [req.L], self.table)[0])
+            req.cell_size = float(autotune.cell_sizes([0, req.n], [req.L], self.table,
+                                                      req.boxes if req.boxes is not None else None)[0])
         req._alone = False
         self.queue.append(req)
 

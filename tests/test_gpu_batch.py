@@ -244,7 +244,7 @@ def test_batch_c4_full(torch_cuda, eng):
 
     boxes, offsets, ranges = workloads.batched(4096, workloads.SEEDS["c4"])
     table = autotune.load_table()
-    cells = autotune.cell_sizes(offsets, ranges, table)
+    cells = autotune.cell_sizes(offsets, ranges, table, boxes)
     eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
     d = to_dev(torch_cuda, boxes)
     cap = int(0.2 * boxes.shape[0])

@@ -131,7 +131,10 @@ def test_serve_batch_with_overfull_bins_delivers_the_rest(torch_cuda):
             b = workloads.uniform(n, 65536.0 * (n / 4194304) ** 0.5, 1.0, 32.0, 11 + i)
         reqs.append(Request(rid=i, boxes=b, L=65536.0 * (n / 2097152) ** 0.5 if i % 3 == 0
                             else 65536.0 * (n / 4194304) ** 0.5))
-    srv = Server(hbm_budget=1 << 30, slots=3)
+    # a fixed table (the scenario needs the batch's bins to outgrow LDS)
+    table = {"version": 1, "bin_target": 3000, "source": "test",
+             "classes": [{"max_n": None, "boxes_per_cell": 1.0}]}
+    srv = Server(hbm_budget=1 << 30, slots=3, table=table)
     try:
         srv.run(reqs, realtime=False)  # sizes the slots' workspaces
         for r in reqs:
@@ -163,7 +166,7 @@ def test_serve_c5_stream_full_sizes(torch_cuda):
     for q in spec:
         b = workloads.request_boxes(q)
         reqs.append(Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
-                            cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table)[0])))
+                            cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table, b)[0])))
     assert sum(r.n > 500_000 for r in reqs) >= 2
     assert {q["kind"] for q in spec} == {"uniform", "clustered"}
     assert max(r.n for r in reqs) > 900_000

@@ -38,3 +38,22 @@ def test_pool_random_never_overlaps():
     for s in list(live):
         p.release(s, live.pop(s))
     assert p.free == [(0, 10_000)]
+
+
+def test_cell_side_floor():
+    """The autotuner's cells are never narrower than the widest box of a
+    bounded-size sample (with headroom), so boxes span at most two cells per
+    axis; unbounded (log-normal) sides set no floor."""
+    import numpy as np
+
+    from pbig_amd import autotune, workloads
+
+    b, L = workloads.c2_large_uniform(n=200000)
+    fl = autotune.side_floor(b)
+    assert 32.0 < fl < 36.0
+    t = {"classes": [{"max_n": None, "boxes_per_cell": 0.25}], "bin_target": 3000}
+    c = autotune.cell_sizes([0, b.shape[0]], [L], t, b)[0]
+    assert c >= fl and c >= np.max(np.maximum(b[:, 1] - b[:, 0], b[:, 3] - b[:, 2]))
+    bc, Lc = workloads.c3_clustered(n=100000)
+    assert autotune.side_floor(bc) == 0.0
+    assert autotune.cell_sizes([0, bc.shape[0]], [Lc], t, bc)[0] == autotune.cell_sizes([0, bc.shape[0]], [Lc], t)[0]
