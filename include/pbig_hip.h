@@ -68,7 +68,13 @@ enum { PBIG_PATH_AUTO = 0, PBIG_PATH_GENERAL = 1, PBIG_PATH_BINNED = 2 };
 typedef struct pbig_hip_config {
   int32_t path;          /* PBIG_PATH_* */
   uint32_t bin_target;   /* binned path: boxes per coarse bin to aim for (0 = 2000) */
-  uint32_t reserved0, reserved1;
+  /* PBIG_PATH_AUTO, synchronous calls: the smallest instance (in boxes)
+   * whose coarse bins beyond LDS, outside mostly dense clusters, are
+   * re-binned by levels on the device; smaller ones run on the general path
+   * (the levels' fixed cost, about 0.3 ms, exceeds the general path's whole
+   * cost below about a million boxes).  0 = 2^20. */
+  uint32_t levels_min_boxes;
+  uint32_t reserved1;
 } pbig_hip_config;
 
 /* Lifecycle.  Replaces pbig_init (src/pbig/pbig.h:9, pbig.cu:2139-2217) and

@@ -211,10 +211,12 @@ struct OnePass {
   uint32_t* fill = nullptr;       // per bin: memberships reserved (this call)
   uint32_t* fill_next = nullptr;  // the next call's counters, zeroed by this one
   uint32_t fill_words = 0;        // entries of each fill array
-  uint32_t* bin_start = nullptr;  // per bin: b * bin_cap
+  uint32_t* bin_start = nullptr;  // per bin: b * bin_cap (written by the place kernel)
   uint32_t bin_cap = 0;
   Meta* meta = nullptr;           // this call's result words
   Meta* meta_next = nullptr;      // the next call's, zeroed by this one
+  uint32_t dense_exit = 0;        // a bin over its share: count to the end, and flag kErrDense when the
+                                  // instance is mostly dense clusters
 };
 
 constexpr int kOnePassKernels = 2;  // place, test

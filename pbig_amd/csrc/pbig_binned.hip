@@ -619,7 +619,9 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
 //   2. reserves the round's share of every bin it touches with one device
 //      atomic on fill[b] (about 1,000 per round instead of one per
 //      membership): first memberships, then the others.  A share past the
-//      bin's capacity flags kErrBinCap and the workgroup stops placing;
+//      bin's capacity flags kErrBinCap, and the workgroup stops placing
+//      (with count_all its later rounds only count, so that the counters
+//      end with every bin's total);
 //   3. writes the other memberships straight to slots handed out by LDS
 //      atomics, from a list of the boxes that have any (kept in the stage's
 //      space during the count: a few percent of narrow boxes; a round with
@@ -657,12 +659,14 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
                                                              uint32_t* __restrict__ fill_next, uint32_t fill_words,
                                                              float4* __restrict__ bbox, uint32_t* __restrict__ bid,
                                                              uint32_t bin_cap, uint32_t stage,
+                                                             uint32_t* __restrict__ bin_start,
                                                              Meta* __restrict__ meta,
-                                                             Meta* __restrict__ meta_next) {
+                                                             Meta* __restrict__ meta_next, uint32_t count_all) {
   extern __shared__ __align__(16) unsigned char place_raw[];
   __shared__ uint32_t flags;  // 1 invalid box, 2 wide box, 4 bin over capacity (or too many boxes listed)
   __shared__ uint32_t wsum[kPlaceThreads / 64];
   __shared__ uint32_t npend;  // boxes of the round with memberships besides a first one
+  __shared__ uint64_t resume;           // first box not counted when a bin outgrew its share
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const InstDesc I = inst[0];
   const uint32_t nb = I.nb, nbx = I.nbx;
@@ -674,6 +678,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
   uint32_t* const base1 = cnt2 + nb;    // per bin: bucket slot of the first memberships' run
   for (uint32_t i = blockIdx.x * kPlaceThreads + tid; i < fill_words; i += gridDim.x * kPlaceThreads)
     fill_next[i] = 0u;
+  for (uint32_t b = blockIdx.x * kPlaceThreads + tid; b < I.nb; b += gridDim.x * kPlaceThreads)
+    bin_start[b] = b * bin_cap;  // every bin's share starts at b * bin_cap (read by the test kernel)
   if (blockIdx.x == 0 && tid < sizeof(Meta) / sizeof(unsigned long long))
     reinterpret_cast<unsigned long long*>(meta_next)[tid] = 0ull;
   if (tid == 0) flags = 0u;
@@ -776,7 +782,10 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
       }
     }
     __syncthreads();
-    if (flags & 4u) break;  // the launch fails over to the two-pass pipeline
+    if (flags & 4u) {  // the launch fails over to another pipeline
+      if (tid == 0) resume = r1;
+      break;
+    }
     // 3. the listed boxes' other memberships (every one of a wide box)
     {
       const uint32_t np = npend;
@@ -833,7 +842,29 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
     }
   }
   __syncthreads();
-  if (tid == 0 && flags) {
+  // A bin over its share, with count_all: the rest of the workgroup's range
+  // is only counted (every membership in LDS, one device atomic per touched
+  // bin and round), so that the counters end with every bin's total (the
+  // test kernel then applies the dense rule to them).
+  if ((flags & 4u) && count_all) {
+    for (uint64_t r0 = resume; r0 < g1; r0 += kPlaceRound) {
+      const uint64_t r1 = r0 + kPlaceRound < g1 ? r0 + kPlaceRound : g1;
+      for (uint32_t b = tid; b < nb; b += kPlaceThreads) cnt1[b] = 0u;
+      __syncthreads();
+      for (uint64_t i = r0 + tid; i < r1; i += kPlaceThreads) {
+        const float4 b = boxes[I.box0 + i];
+        if (!((b.x <= b.y) && (b.z <= b.w))) continue;  // (flagged by no one: the launch has failed anyway)
+        const BinRange rg = bin_range_of(fine_corners(b, I), I);
+        for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
+          for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) atomicAdd(&cnt1[cy * nbx + cx], 1u);
+      }
+      __syncthreads();
+      for (uint32_t b = tid; b < nb; b += kPlaceThreads)
+        if (cnt1[b]) atomicAdd(&fill[b], cnt1[b]);
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
     if (flags & 1u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrInvalidBox);
     if (flags & 2u) atomicOr(&meta->wide, 1u);
     if (flags & 4u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrBinCap);
@@ -892,6 +923,7 @@ struct TestLds {
   uint32_t out_n;                  // pairs appended to this workgroup's region
   uint32_t strip_hi;
   unsigned long long out_base;     // direct output: this workgroup's first output position
+  unsigned long long memb;         // kSingleSum: memberships of this workgroup's bins
 };
 static_assert(sizeof(TestLds) <= 160u * 1024u, "one test workgroup per CU");
 
@@ -966,6 +998,45 @@ __device__ __forceinline__ void scatter_strip(TestLds& L, const float4& q, uint3
   }
 }
 
+// The single-pass launch's bins' totals are the place kernel's counters:
+// each test workgroup adds its bins' totals to the memberships in the result
+// words (kSingleSum).  When a bin outgrew its share and the place kernel
+// counted to the end (kSingleDense), workgroup 0 applies the
+// automatic path choice's dense rule to them: bins beyond what the two-pass
+// test schedule holds (kNarrowMax memberships, kMaxCoarse with wide boxes)
+// carrying over a quarter of the memberships flag kErrDense, as the
+// two-pass launch's bin totals would.
+constexpr uint32_t kSingleSum = 1u, kSingleDense = 2u;
+
+__device__ void single_pass_dense_rule(const uint32_t* __restrict__ total, uint32_t n_bins, uint32_t single,
+                                       unsigned long long err, Meta* __restrict__ meta) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  if (!(err & kErrBinCap) || !(single & kSingleDense) || blockIdx.x != 0 || (err & kErrInvalidBox)) return;
+  const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
+  unsigned long long h = 0, g = 0;
+  for (uint32_t b = tid; b < n_bins; b += blockDim.x) {
+    const uint32_t t = total[b];
+    g += t;
+    h += t > limit ? t : 0u;
+  }
+  for (uint32_t o = 32; o; o >>= 1) {
+    h += __shfl_xor(h, o);
+    g += __shfl_xor(g, o);
+  }
+  __shared__ unsigned long long sh, sg;
+  if (tid == 0) sh = sg = 0ull;
+  __syncthreads();
+  if (lane == 0) {
+    atomicAdd(&sh, h);
+    atomicAdd(&sg, g);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    meta->memberships = sg;
+    if (4ull * sh > sg) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrDense);
+  }
+}
+
 __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float4* __restrict__ bbox,
                                                            const uint32_t* __restrict__ bid,
                                                            const uint32_t* __restrict__ bin_start,
@@ -976,12 +1047,15 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
                                                            uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
                                                            uint2* __restrict__ out, uint64_t cap,
-                                                           unsigned long long* out_count, Meta* __restrict__ meta) {
+                                                           unsigned long long* out_count, Meta* __restrict__ meta,
+                                                           uint32_t single) {
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
+  const unsigned long long err = meta->error;
+  if (err & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
+    if (single) single_pass_dense_rule(total, n_bins, single, err, meta);
     return;
   }
   uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
@@ -994,6 +1068,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
   if (tid == 0) {
     L.out_n = 0u;
     L.nfine = 0u;
+    L.memb = 0ull;
   }
 
   // Registers: unconditional loads with the index clamped into the bucket.
@@ -1013,7 +1088,9 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
         const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
         const uint32_t ex = I.fx0 + I.span_x - gx, ey = I.fy0 + I.span_y - gy;  // cells left in the region
         const uint32_t kx = I.k < ex ? I.k : ex, ky = I.k < ey ? I.k : ey;
-        L.binfo[u] = make_uint2(bin_start[gb], total[gb]);
+        const uint32_t tt = total[gb];
+        L.binfo[u] = make_uint2(bin_start[gb], tt);
+        if ((single & kSingleSum) && tt) atomicAdd(&L.memb, (unsigned long long)tt);
         L.bingrid[u] = make_uint4(gx, gy, kx | (ky << 16), I.nf);
         L.bininv[u] = I.inv_f;
       }
@@ -1274,6 +1351,8 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
     for (uint32_t k = tid; k < n; k += kTestThreads)
       if (base + k < cap) out[base + k] = region[k];
   }
+  // (after the pair counter's atomic, which the copy above waits for)
+  if ((single & kSingleSum) && tid == 0 && L.memb) atomicAdd(&meta->memberships, L.memb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1317,6 +1396,7 @@ struct NarrowLds {
   uint32_t dnext;                  // next chunk of positions to claim, past the first 16
   uint32_t out_n;
   unsigned long long out_base;
+  unsigned long long memb;         // kSingleSum: memberships of this workgroup's bins
 };
 static_assert(sizeof(NarrowLds) <= 160u * 1024u, "one test workgroup per CU");
 static_assert(kMaxExt <= 5u * kTestThreads, "the cell scan takes five cells per thread");
@@ -1333,20 +1413,25 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
                                                                   uint32_t* __restrict__ overfull,
                                                                   uint2* __restrict__ out, uint64_t cap,
                                                                   unsigned long long* out_count,
-                                                                  Meta* __restrict__ meta) {
+                                                                  Meta* __restrict__ meta, uint32_t single) {
   NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
   const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
-  if (meta->error & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
+  const unsigned long long err = meta->error;
+  if (err & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
     for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
+    if (single) single_pass_dense_rule(total, n_bins, single, err, meta);
     return;
   }
   uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
   uint32_t sn = 0;
   uint32_t seg0 = 0;
   for (uint32_t c = tid; c < kMaxExt; c += kTestThreads) L.cnt[c] = 0u;  // zero at the start of every bin
-  if (tid == 0) L.out_n = 0u;
+  if (tid == 0) {
+    L.out_n = 0u;
+    L.memb = 0ull;
+  }
 
   const uint64_t last = cap_bucket - 1;
   float4 rb[kNarrowRegs];
@@ -1362,7 +1447,9 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
         const uint32_t ex = I.fx0 + I.span_x - gx, ey = I.fy0 + I.span_y - gy;
         const uint32_t kx = I.k < ex ? I.k : ex, ky = I.k < ey ? I.k : ey;
-        L.binfo[u] = make_uint2(bin_start[gb], total[gb]);
+        const uint32_t tt = total[gb];
+        L.binfo[u] = make_uint2(bin_start[gb], tt);
+        if ((single & kSingleSum) && tt) atomicAdd(&L.memb, (unsigned long long)tt);
         L.bingrid[u] = make_uint4(gx, gy, kx | (ky << 16), I.nf);
         L.bininv[u] = I.inv_f;
       }
@@ -1552,6 +1639,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     for (uint32_t k = tid; k < n; k += kTestThreads)
       if (base + k < cap) out[base + k] = region[k];
   }
+  // (after the pair counter's atomic, which the copy above waits for)
+  if ((single & kSingleSum) && tid == 0 && L.memb) atomicAdd(&meta->memberships, L.memb);
 }
 
 // One launch for both schedules: the histogram's wide flag, uniform over the
@@ -1570,7 +1659,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                                                            uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
                                                            uint2* __restrict__ out, uint64_t cap, uint32_t direct,
-                                                           Meta* __restrict__ meta) {
+                                                           Meta* __restrict__ meta, uint32_t single = 0u) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   // direct output: the launch's pair counter (a single instance's own, or
   // the one its device-planned levels share)
@@ -1579,10 +1668,10 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (meta->wide)
     test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                 overfull, out, cap, out_count, meta);
+                 overfull, out, cap, out_count, meta, single);
   else
     test_narrow(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                overfull, out, cap, out_count, meta);
+                overfull, out, cap, out_count, meta, single);
 }
 
 // The pair count and error word, copied to the caller's two result words
@@ -1874,11 +1963,13 @@ hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, c
   if (stage < 64u) return hipErrorInvalidValue;  // too many bins for the stage (the planner keeps nb <= kMaxBins)
   k_bin_place<<<grid, kPlaceThreads, 24u * stage + 12u * nbp, s>>>(boxes, ws.inst, op.fill, op.fill_next,
                                                                   op.fill_words, ws.bbox, ws.bid, op.bin_cap, stage,
-                                                                  op.meta, op.meta_next);
+                                                                  op.bin_start, op.meta, op.meta_next,
+                                                                  op.dense_exit);
   PBIG_MARK(1)
   k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(
       ws.bbox, ws.bid, op.bin_start, op.fill, ws.inst, ws.bin_inst, bp.n_bins, (uint64_t)bp.n_bins * op.bin_cap,
-      ws.scratch, ws.cap_wg, ws.bin_seg, ws.overfull, out, cap, 1u, op.meta);
+      ws.scratch, ws.cap_wg, ws.bin_seg, ws.overfull, out, cap, 1u, op.meta,
+      kSingleSum | (op.dense_exit ? kSingleDense : 0u));
   PBIG_MARK(2)
 #undef PBIG_MARK
   return hipGetLastError();

@@ -68,9 +68,8 @@ struct BinnedState {
   uint32_t* fill[2] = {nullptr, nullptr};
   uint64_t cap_fill = 0;
   Meta* meta1 = nullptr;         // two result-word sets
-  uint32_t* start1 = nullptr;    // per bin: b * bin_cap
+  uint32_t* start1 = nullptr;    // per bin: b * bin_cap (written by each place kernel)
   uint64_t cap_start1 = 0;
-  uint32_t start1_nb = 0, start1_cap = 0;
   uint32_t parity = 0;
   // Re-binning levels planned on the device (levels 1.. of the plain path's
   // two-pass launch), each with its own workspace and bounds.  deep_ok: the
@@ -80,6 +79,7 @@ struct BinnedState {
   std::unique_ptr<BinnedState> dl[pbig::kDeepLevels];
   pbig::DeepCaps dcaps[pbig::kDeepLevels];
   bool deep_ok = false, deep_failed = false;
+  bool to_general = false;  // the automatic choice sends this plan's instance to the general path
   uint64_t* d_res = nullptr;  // result words of a synchronous call on them
 };
 
@@ -107,7 +107,7 @@ struct pbig_hip_ctx {
   hipEvent_t ring_ev[16] = {};
   uint32_t ring_next = 0;
   unsigned char* noalloc = nullptr;  // this device's address of the no-allocation scratch
-  pbig_hip_config cfg{PBIG_PATH_AUTO, 2000, 0, 0};
+  pbig_hip_config cfg{PBIG_PATH_AUTO, 2000, 1u << 20, 0};
   int last_path = 0;
   pbig_hip_stats stats{};
   std::string err;
@@ -523,7 +523,6 @@ void free_state(BinnedState& st) {
   st.fill[0] = st.fill[1] = st.start1 = nullptr;
   st.meta1 = nullptr;
   st.cap_fill = st.cap_start1 = 0;
-  st.start1_nb = st.start1_cap = 0;
   st.onepass_dirty = true;
 }
 
@@ -590,6 +589,7 @@ int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_
                      : 0u;
     st.onepass_failed = false;
     st.deep_ok = st.deep_failed = false;
+    st.to_general = false;
   }
   st.ws.meta = c->ws.meta;
   int rc;
@@ -676,15 +676,7 @@ int prepare_onepass(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, bool* ok) {
     HIP_TRY(c, hipMalloc((void**)&st.meta1, 2 * sizeof(Meta)));
     st.onepass_dirty = true;
   }
-  if (st.start1_nb != nb || st.start1_cap != st.bin_cap) {
-    if ((rc = grow(c, &st.start1, &st.cap_start1, nb))) return rc;
-    std::vector<uint32_t> v(nb);
-    for (uint32_t b = 0; b < nb; ++b) v[b] = b * st.bin_cap;
-    HIP_TRY(c, hipMemcpyAsync(st.start1, v.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(c, hipStreamSynchronize(s));  // v is pageable and goes out of scope
-    st.start1_nb = nb;
-    st.start1_cap = st.bin_cap;
-  }
+  if ((rc = grow(c, &st.start1, &st.cap_start1, nb))) return rc;  // filled by the place kernel
   if (st.onepass_dirty) {
     HIP_TRY(c, hipMemsetAsync(st.fill[0], 0, st.cap_fill * sizeof(uint32_t), s));
     HIP_TRY(c, hipMemsetAsync(st.fill[1], 0, st.cap_fill * sizeof(uint32_t), s));
@@ -708,6 +700,7 @@ int launch_onepass(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint
   op.bin_cap = st.bin_cap;
   op.meta = st.meta1 + p;
   op.meta_next = st.meta1 + (p ^ 1u);
+  op.dense_exit = st.bp.dense_exit;
   const hipError_t e = pbig::launch_binned_onepass(st.ws, op, d_boxes, st.bp, d_out, cap, s,
                                                    next_events(c, kPathOnePass));
   if (e != hipSuccess) {
@@ -720,9 +713,10 @@ int launch_onepass(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint
 }
 
 // The single-pass launch, synchronously.  0: done, *m holds the result
-// words (memberships summed from the bins' counters); 1: the plan runs the
-// two-pass launch instead (not eligible, or the launch reported trouble,
-// which also marks the plan); < 0: an error.
+// words; 1: the plan runs the two-pass launch instead (not eligible, or the
+// launch reported trouble, which also marks the plan); 2: a bin outgrew its
+// share and the automatic choice's dense rule holds (the general path runs
+// instead); < 0: an error.
 int run_onepass_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
   BinnedState& st = c->plain;
   int rc;
@@ -742,16 +736,14 @@ int run_onepass_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint6
       if ((rc = ensure_state(c, st, st.ws.cap_bucket))) return rc;
       continue;
     }
+    if (m->error & pbig::kErrDense) {  // (the automatic choice: mostly dense clusters)
+      st.onepass_failed = st.to_general = true;
+      return 2;
+    }
     if (m->error) {
       st.onepass_failed = true;
       return 1;
     }
-    // the counters of this call were left in the set it used
-    const uint32_t nb = st.bp.n_bins;
-    std::vector<uint32_t> f(nb);
-    HIP_TRY(c, hipMemcpy(f.data(), st.fill[(st.parity ^ 1u) & 1u], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    m->memberships = 0;
-    for (uint32_t v : f) m->memberships += v;
     return 0;
   }
   st.onepass_failed = true;
@@ -819,8 +811,7 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
                  uint64_t* d_result, hipStream_t s) {
   hipEvent_t* ev = next_events(c, kPathDeep);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
-  st.bp.direct = 1u;
-  st.bp.dense_exit = 0u;
+  st.bp.direct = 1u;  // (dense_exit as the caller set it: a dense instance stops after level 0's bin totals)
   HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, nullptr, st.bp, d_out, cap, nullptr, s, nullptr));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
   const BinnedWorkspace* parent = &st.ws;
@@ -844,7 +835,8 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
 
 // The launch with device levels, synchronously.  0: done (*m: pair count,
 // level 0's memberships); 1: the levels cannot complete this plan (the host
-// re-binning runs instead); < 0: an error.  Grows every level's workspace
+// re-binning runs instead); 2: the automatic choice's dense rule stopped
+// level 0 (the general path runs instead); < 0: an error.  Grows every level's workspace
 // to what it reports and runs again (a few times at most).
 int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
   BinnedState& st = c->plain;
@@ -855,13 +847,11 @@ int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t
   std::vector<uint64_t> bucket(pbig::kDeepLevels, 0);
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
     bucket[l] = st.dl[l] ? st.dl[l]->ws.cap_bucket : 2 * std::max<uint64_t>(st.ws.cap_bucket, 4096);
-  const uint32_t saved_exit = st.bp.dense_exit;
   for (int attempt = 0; attempt < 6; ++attempt) {
     for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
       if ((rc = ensure_deep_level(c, st, l, bucket[l]))) return rc;
     if ((rc = after_upload(c, st, s))) return rc;
     if ((rc = enqueue_deep(c, st, d_boxes, d_out, cap, st.d_res, s))) return rc;
-    st.bp.dense_exit = saved_exit;
     uint64_t res[2];
     std::vector<Meta> lm(pbig::kDeepLevels);
     HIP_TRY(c, hipMemcpyAsync(res, st.d_res, sizeof(res), hipMemcpyDeviceToHost, s));
@@ -873,23 +863,15 @@ int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t
       *m = lm[0];
       m->n_pairs = res[0];
       m->error = 0;
-      st.deep_ok = true;
-      // The levels' grids are their bounds: tighten them to what this plan
-      // needs (a quarter more), so that asynchronous calls launch no
-      // workgroups that find nothing to do.  The buffers stay as grown.
-      for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
-        const Meta& x = lm[l];
-        pbig::DeepCaps& k = st.dcaps[l];
-        k.inst = std::max<uint32_t>(x.dev_inst + x.dev_inst / 4, 16);
-        k.chunks = std::max<uint32_t>(x.dev_chunks + x.dev_chunks / 4, 16);
-        k.bins = std::max<uint32_t>(x.dev_bins + x.dev_bins / 4, 64);
-        k.hist = std::max<uint64_t>(x.dev_hist + x.dev_hist / 4, 4096);
-        k.off = std::max<uint64_t>(x.dev_off + x.dev_off / 4, 4096);
-      }
+      st.deep_ok = true;  // (the bounds stay: instances of other shapes follow on this context)
       return 0;
     }
     if (res[1] & pbig::kErrInvalidBox)
       return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    if (res[1] & pbig::kErrDense) {
+      st.to_general = true;
+      return 2;
+    }
     bool grown = false;
     // level 0: bucket and pair regions
     if (lm[0].error & pbig::kErrWorkspace) {
@@ -1128,11 +1110,28 @@ int resolve(pbig_hip_ctx* c, BinnedState& st, const float4* boxes, const uint32_
 int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d_out, uint64_t cap,
                     hipStream_t s, uint64_t* n_pairs) {
   BinnedState& st = c->plain;
+  const bool auto_choice = st.bp.dense_exit != 0;
+  if (st.to_general && auto_choice) return 1;  // (known from this plan's last call)
   Meta m;
   int rc = run_onepass_sync(c, d_boxes, d_out, cap, s, &m);
   if (rc < 0) return rc;
+  if (rc == 2) return 1;  // mostly dense clusters: the general path
+  if (rc == 1 && auto_choice && !st.deep_ok) {
+    // The automatic choice runs the two-pass launch alone first: its bin
+    // totals send an instance of mostly dense clusters to the general path
+    // (which stops the launch before it scatters), and an instance with no
+    // bin beyond LDS is done.  Bins beyond LDS go to the device levels in
+    // instances of levels_min_boxes or more, else to the general path.
+    if ((rc = run_state_sync(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, &m))) return rc;
+    if ((m.error & pbig::kErrDense) || (m.error && n < c->cfg.levels_min_boxes)) {
+      st.to_general = true;
+      return 1;
+    }
+    rc = m.error ? 1 : 0;
+  }
   if (rc == 1) rc = run_deep_sync(c, d_boxes, d_out, cap, s, &m);  // re-binning levels on the device
   if (rc < 0) return rc;
+  if (rc == 2) return 1;  // mostly dense clusters: the general path
   if (rc == 1) {  // the two-pass launch, with re-binning of bins beyond LDS on the host
     Splice sp;
     std::vector<uint8_t> failed;
@@ -1577,6 +1576,7 @@ int pbig_hip_configure(pbig_hip_ctx* c, const pbig_hip_config* cfg) {
   if (cfg->path < PBIG_PATH_AUTO || cfg->path > PBIG_PATH_BINNED) return fail(c, PBIG_E_ARG, "unknown path %d", cfg->path);
   c->cfg = *cfg;
   if (c->cfg.bin_target == 0) c->cfg.bin_target = 2000;
+  if (c->cfg.levels_min_boxes == 0) c->cfg.levels_min_boxes = 1u << 20;
   c->last_path = 0;
   return PBIG_OK;
 }

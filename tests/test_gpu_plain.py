@@ -343,14 +343,7 @@ def test_binned_c3_rebins_heavy_bins(torch_cuda, binned, orc, manifest, cs):
     assert orc.pair_hash(got) == m["sha256"]
 
 
-def test_auto_dense_instance_on_device_levels(torch_cuda, orc, manifest):
-    """The clustered instance under the automatic choice: the first
-    asynchronous call tries the single-pass launch, whose dense bins outgrow
-    their fixed share of the bucket, and stops with error bit 5 and no pairs;
-    the synchronous call settles the path: the two-pass launch followed by
-    re-binning levels planned on the device (the bins beyond LDS split into
-    finer ones with no host work between levels), and the next asynchronous
-    calls run that whole sequence on the device, with the exact set."""
+def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors):
     import pbig_amd
     from pbig_amd import workloads
 
@@ -360,13 +353,13 @@ def test_auto_dense_instance_on_device_levels(torch_cuda, orc, manifest):
     out = torch_cuda.empty((m["pairs"] + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
     res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
     with pbig_amd.Engine(0) as e:
-        e.configure(PATHS["auto"], 2000)
+        e.configure(PATHS[path], 2000)
         e.check_device_async(d, L, 64.0, out, res)
         torch_cuda.cuda.synchronize()
         n_async, err = (int(v) for v in res.cpu().tolist())
-        assert err in (16, 32) and n_async == 0, (n_async, err)
+        assert err in first_errors and n_async == 0, (n_async, err)
         n = e.check_device(d, L, 64.0, out)
-        assert e.stats()["path"] == PATHS["binned"]
+        settled = e.stats()["path"]
         assert n == m["pairs"] and orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
         for rep in range(2):
             res.zero_()
@@ -375,6 +368,30 @@ def test_auto_dense_instance_on_device_levels(torch_cuda, orc, manifest):
             torch_cuda.cuda.synchronize()
             assert res.cpu().tolist() == [m["pairs"], 0], rep
             assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"], rep
+    return settled
+
+
+def test_auto_dense_instance_goes_to_the_general_path(torch_cuda, orc, manifest):
+    """The clustered instance under the automatic choice: the first
+    asynchronous call tries the single-pass launch, whose dense bins outgrow
+    their fixed share of the bucket; it stops with error bit 5 and no pairs,
+    and with bit 4 too, since the bins' totals (counted to the end) put over
+    a quarter of the memberships in bins beyond LDS (mostly dense clusters).
+    The synchronous call then sends the instance to the general path, and
+    the next asynchronous calls follow it, with the exact set."""
+    settled = _dense_instance_settles(torch_cuda, orc, manifest, "auto", (16, 32, 48))
+    print("settled path", settled)
+    assert settled == PATHS["general"]
+
+
+def test_binned_dense_instance_on_device_levels(torch_cuda, orc, manifest):
+    """The same instance on the binned path forced: the first asynchronous
+    call reports its dense bins (error bit 5); the synchronous call runs the
+    two-pass launch followed by re-binning levels planned on the device (the
+    bins beyond LDS split into finer ones with no host work between levels),
+    and the next asynchronous calls run that whole sequence on the device,
+    with the exact set."""
+    assert _dense_instance_settles(torch_cuda, orc, manifest, "binned", (8, 32)) == PATHS["binned"]
 
 
 def test_binned_deep_rebinning(torch_cuda, binned, orc):
@@ -455,12 +472,16 @@ def test_auto_keeps_large_narrow_bins_binned(torch_cuda, orc, manifest):
         eng.close()
 
 
-def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc):
-    """6,000 identical boxes share one fine cell at every refinement: the
-    device re-binning levels cannot split their bin and report it, and the
-    synchronous call falls back (host re-binning, then the general path) with
-    the exact set; the asynchronous call after it follows the path that
-    worked."""
+@pytest.mark.parametrize("levels_min", [0, 1])
+def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc, levels_min):
+    """6,000 identical boxes among 20,000 uniform ones share one fine cell at
+    every refinement; their bin is beyond LDS but holds under a quarter of
+    the memberships.  At the default levels_min_boxes (2^20) the automatic
+    choice sends this 26,000-box instance to the general path; at 1 the
+    device re-binning levels cannot split the bin and report it, and the
+    synchronous call falls back (host re-binning, then the general path).
+    Either way the set is exact, and the asynchronous call after it follows
+    the path that worked."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -472,11 +493,46 @@ def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc):
     out = torch_cuda.empty((len(want) + 4096, 2), dtype=torch_cuda.int32, device="cuda:0")
     res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
     with pbig_amd.Engine(0) as e:
-        e.configure(PATHS["auto"], 2000)
+        e.configure(PATHS["auto"], 2000, levels_min)
         n = e.check_device(d, 4096.0, 8.0, out)
         assert n == len(want)
+        assert e.stats()["path"] == PATHS["general"]
         assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
         e.check_device_async(d, 4096.0, 8.0, out, res)
         torch_cuda.cuda.synchronize()
         assert res.cpu().tolist() == [len(want), 0]
         assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
+
+
+@pytest.mark.parametrize("levels_min", [0, 1])
+def test_auto_spot_in_uniform_instance(torch_cuda, orc, levels_min):
+    """100,000 uniform boxes and a spot of 15,000 small ones: the spot's bin
+    is beyond LDS but holds under a quarter of the memberships (not mostly
+    dense clusters).  With levels_min_boxes above the instance's size the
+    automatic choice sends it to the general path; at 1 the device levels
+    split the spot's bin, the call stays on the binned path, and the
+    asynchronous calls after it run the levels too.  The set is exact
+    every time."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    rest = workloads.uniform(100000, 8192.0, 1.0, 8.0, 91)
+    spot = workloads.uniform(15000, 60.0, 0.5, 2.0, 92) + np.float32(4000.0)
+    boxes = np.concatenate([rest[:50000], spot, rest[50000:]]).astype(np.float32)
+    want = orc.sort_pairs(orc.sweep_pairs(boxes))
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((len(want) + 4096, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["auto"], 2000, levels_min)
+        n = e.check_device(d, 8192.0, 8.0, out)
+        assert n == len(want)
+        assert e.stats()["path"] == (PATHS["binned"] if levels_min == 1 else PATHS["general"])
+        assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
+        for rep in range(2):
+            res.zero_()
+            out.fill_(-1)
+            e.check_device_async(d, 8192.0, 8.0, out, res)
+            torch_cuda.cuda.synchronize()
+            assert res.cpu().tolist() == [len(want), 0], rep
+            assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want), rep
