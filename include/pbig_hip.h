@@ -74,7 +74,13 @@ typedef struct pbig_hip_config {
    * (the levels' fixed cost, about 0.3 ms, exceeds the general path's whole
    * cost below about a million boxes).  0 = 2^20. */
   uint32_t levels_min_boxes;
-  uint32_t reserved1;
+  /* Nonzero: coarse bins beyond LDS are always re-binned by the host
+   * between levels (synchronous calls; asynchronous launches report them,
+   * error bit 3), never by levels planned on the device.  0 (default):
+   * synchronous calls settle on device levels where they complete the
+   * instance or batch, and later asynchronous launches and captures of it
+   * run them too. */
+  uint32_t host_levels;
 } pbig_hip_config;
 
 /* Lifecycle.  Replaces pbig_init (src/pbig/pbig.h:9, pbig.cu:2139-2217) and
@@ -214,19 +220,24 @@ int pbig_hip_batch_update(pbig_hip_batch* batch, const pbig_hip_instance* inst, 
 /* Synchronous batch check.  d_boxes holds n_boxes_total boxes (every
  * instance's range must lie inside).  h_inst_pairs (optional, host) receives
  * n_inst x {offset, count}.  Regrows the workspace when needed.  A coarse
- * bin whose boxes exceed LDS is re-binned into smaller bins (recursively);
- * an instance with a single fine cell beyond LDS runs on the general path.
- * Either way its pairs are placed in its run.  *n_pairs = total;
- * PBIG_E_OVERFLOW when it exceeds pair_capacity (offsets and counts are
- * still returned). */
+ * bin whose boxes exceed LDS is re-binned into smaller bins: by up to three
+ * levels planned on the device over every such bin of the batch in one
+ * launch sequence (the batch then keeps them: its later launches and
+ * captures run them, complete on the device), else (bins the levels cannot
+ * split, more of them than a level takes, or host_levels configured) by the
+ * host between levels; an instance with a single fine cell beyond LDS runs
+ * on the general path.  Either way its pairs are placed in its run.
+ * *n_pairs = total; PBIG_E_OVERFLOW when it exceeds pair_capacity (offsets
+ * and counts are still returned). */
 int pbig_hip_batch_check(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
                          uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
                          uint64_t* n_pairs, void* stream);
 
 /* Complete a pbig_hip_batch_launch (same arguments, same stream) whose error
- * word was nonzero: when only coarse bins beyond LDS were reported, they are
- * re-binned and spliced without running the launch again; otherwise this is
- * pbig_hip_batch_check. */
+ * word was nonzero: the launch runs again with levels on the device (which
+ * the batch then keeps); where they do not apply and only coarse bins beyond
+ * LDS were reported, those are re-binned by the host and spliced without
+ * running the launch again; otherwise this is pbig_hip_batch_check. */
 int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t n_boxes_total,
                           uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* h_inst_pairs,
                           uint64_t* n_pairs, void* stream);
@@ -234,7 +245,8 @@ int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t 
 /* After a pbig_hip_batch_launch (or replay, on this stream) whose error word
  * holds only the bins-beyond-LDS bit and whose pair total fits the launch's
  * pair_capacity: h_flags[i] = 1 (n_inst bytes) for each instance that had
- * such a bin, *n_flagged their number.  Every other instance's (offset,
+ * such a bin, *n_flagged their number (after a launch with device levels
+ * that completed every instance, none).  Every other instance's (offset,
  * count) entry and pair run from that launch is then complete; a flagged
  * instance's count is short, and pbig_hip_batch_finish, or a call on that
  * instance alone, completes it.  Fails with PBIG_E_ARG when the error word
@@ -245,7 +257,9 @@ int pbig_hip_batch_finish(pbig_hip_batch* batch, const float* d_boxes, uint64_t 
  * a batch without waiting for the others' host work. */
 int pbig_hip_batch_unresolved(pbig_hip_batch* batch, uint8_t* h_flags, uint32_t* n_flagged, void* stream);
 
-/* Asynchronous batch launch: enqueues the seven kernels and returns; no host
+/* Asynchronous batch launch: enqueues the seven kernels (once a synchronous
+ * call settled the batch on device levels, also the levels and the merge of
+ * their pairs into the instances' runs) and returns; no host
  * synchronisation, no allocation, no memset, so it may be captured in a HIP
  * graph.  d_inst_pairs (optional, device) receives n_inst x {offset, count};
  * d_result (optional, device, 2 x uint64) the total and the error word of

@@ -119,7 +119,7 @@ class Instance(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int32), ("bin_target", ctypes.c_uint32),
-                ("levels_min_boxes", ctypes.c_uint32), ("reserved1", ctypes.c_uint32)]
+                ("levels_min_boxes", ctypes.c_uint32), ("host_levels", ctypes.c_uint32)]
 
 
 _lib = None
@@ -315,11 +315,13 @@ class Engine:
         names = KERNELS.get(path.value, ())
         return {k: float(arr[i]) for i, k in enumerate(names)}, int(calls.value), PATH_NAMES.get(path.value, "")
 
-    def configure(self, path: int = PATH_AUTO, bin_target: int = 0, levels_min_boxes: int = 0):
+    def configure(self, path: int = PATH_AUTO, bin_target: int = 0, levels_min_boxes: int = 0,
+                  host_levels: bool = False):
         """Path choice and tuning (pbig_hip_configure): bin_target 0 = 2000;
         levels_min_boxes 0 = 2^20 (the automatic choice's smallest instance
-        whose bins beyond LDS are re-binned on the device)."""
-        cfg = Config(int(path), int(bin_target), int(levels_min_boxes), 0)
+        whose bins beyond LDS are re-binned on the device); host_levels: bins
+        beyond LDS always re-binned by the host, never by device levels."""
+        cfg = Config(int(path), int(bin_target), int(levels_min_boxes), int(bool(host_levels)))
         rc = self._lib.pbig_hip_configure(self._h, ctypes.byref(cfg))
         if rc != PBIG_OK:
             self._raise(rc)

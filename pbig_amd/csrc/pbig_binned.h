@@ -57,7 +57,7 @@ struct InstDesc {
   uint32_t span_x;   // fine columns of the region (nf for an owned instance)
   uint32_t span_y;   // fine rows of the region
   uint32_t parent;   // owning instance one level up (virtual instances)
-  uint32_t pad;
+  uint32_t root;     // level-0 instance a virtual instance belongs to (levels planned on the device)
 };
 static_assert(sizeof(InstDesc) == 112, "InstDesc layout");
 
@@ -173,9 +173,10 @@ hipError_t binned_init();
 // events (optional): kBinnedKernels + 1.
 // ids_in: null for instances that own their boxes; the ids of the boxes
 // array (a parent launch's bin-ordered ids) for virtual instances.
+// output = false: stop after the test kernel (launch_binned_output later).
 hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
                          const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
-                         hipEvent_t* events = nullptr);
+                         hipEvent_t* events = nullptr, bool output = true);
 
 // One run of pairs found outside a launch, to be placed after instance
 // inst's own pairs at position `within` of its extra area.
@@ -194,8 +195,10 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 
 // Only the last two kernels (pair scan and output), after the instance table
 // changed its pairs0 / pairs_extra fields: the test's regions are reused.
+// merge (optional): the batch merge's words (shifts and counts of pairs
+// found by device-planned levels, added to the runs).
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
-                                uint64_t* inst_out, hipStream_t s);
+                                uint64_t* inst_out, hipStream_t s, const uint64_t* merge = nullptr);
 
 // The pair count and error word of a call (meta's first two words) copied
 // to the caller's two device words by one small kernel.
@@ -244,13 +247,32 @@ struct DeepMetas {
 // One re-binning level: its plan kernel reads the level above's list of bins
 // beyond LDS (parent, parent_bins bins; parent_dev: that level was itself
 // planned on the device), then hist, scan, starts, scatter and test run over
-// its virtual instances with device counts; its test kernel writes pairs
-// through the shared counter in its meta's out_count.
-// root_inv_f, max_refine: the cap on the cells' refinement below the
-// instance's own (plan_virtual_bin).
+// its virtual instances with device counts.  direct: its test kernel writes
+// pairs through the shared counter in its meta's out_count (one instance);
+// else it leaves per-bin runs in its regions, for the batch merge below.
+// max_refine caps the cells' refinement below the root instance's own
+// (plan_virtual_bin): the root's inv_f is root_inst[root].inv_f when
+// root_inst (the level-0 instance table) is given, else root_inv_f.
 hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
                              const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
-                             uint32_t max_refine, uint32_t n_cu, uint2* out, uint64_t cap, hipStream_t s);
+                             const InstDesc* root_inst, uint32_t max_refine, uint32_t n_cu, uint32_t direct,
+                             uint2* out, uint64_t cap, hipStream_t s);
+
+// Batch merge of the levels' pairs into the instances' runs.  `merge` holds
+// 3 * n_inst + 1 words: per level-0 instance the pairs its levels found, the
+// exclusive scan of those (the shift of its run), a cursor, then the total.
+//   launch_merge_clear   zero the counts and cursors (first, every launch)
+//   launch_level_count   add a level's per-bin pair counts to their roots
+//   launch_merge_scan    shifts and total
+//   launch_binned_output (with merge) level 0's runs, shifted, and every
+//                        instance's (offset, count) including its levels'
+//   launch_level_copy    a level's runs after their root's level-0 pairs
+hipError_t launch_merge_clear(uint64_t* merge, uint32_t n_inst, hipStream_t s);
+hipError_t launch_level_count(const BinnedWorkspace& lv, uint32_t max_bins, uint64_t* merge, hipStream_t s);
+hipError_t launch_merge_scan(uint64_t* merge, uint32_t n_inst, hipStream_t s);
+hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint32_t lv_test_blocks,
+                             const BinnedWorkspace& ws0, const uint64_t* bin_out0, uint64_t* merge, uint32_t n_inst,
+                             uint2* out, uint64_t cap, hipStream_t s);
 hipError_t launch_deep_result(const DeepMetas& m, uint64_t* result, hipStream_t s);
 
 // Tables of a launch of one instance written by a kernel from its argument:

@@ -1664,7 +1664,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   // direct output: the launch's pair counter (a single instance's own, or
   // the one its device-planned levels share)
   unsigned long long* const out_count =
-      kDev ? meta->out_count : (direct ? (unsigned long long*)&meta->n_pairs : nullptr);
+      direct ? (kDev ? meta->out_count : (unsigned long long*)&meta->n_pairs) : nullptr;
   if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (meta->wide)
     test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
@@ -1688,13 +1688,15 @@ __global__ __launch_bounds__(64) void k_bin_result(const Meta* __restrict__ meta
 // Pairs are laid out in bin order, so each instance's pairs are one run.
 __global__ __launch_bounds__(1024) void k_bin_pair_scan(const uint2* __restrict__ bin_seg, uint32_t n_bins,
                                                         uint64_t* __restrict__ bin_out, uint64_t extra_total,
+                                                        const uint64_t* __restrict__ dev_extra,
                                                         Meta* __restrict__ meta) {
   __shared__ ScanLds L;
   const uint64_t grand = block_exscan(
       n_bins, L, [&](uint32_t i) { return (uint64_t)bin_seg[i].y; }, [&](uint32_t i, uint64_t ex) { bin_out[i] = ex; });
   if (threadIdx.x == 0) {
     bin_out[n_bins] = grand;
-    meta->n_pairs = grand + extra_total;  // the true count, also past the capacity
+    // the true count, also past the capacity
+    meta->n_pairs = grand + extra_total + (dev_extra ? *dev_extra : 0ull);
   }
 }
 
@@ -1706,19 +1708,21 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
                                                       const InstDesc* __restrict__ inst,
                                                       const uint32_t* __restrict__ bin_inst, uint32_t n_inst,
                                                       uint2* __restrict__ out, uint64_t cap,
-                                                      uint64_t* __restrict__ inst_out) {
+                                                      uint64_t* __restrict__ inst_out,
+                                                      const uint64_t* __restrict__ merge) {
   if (inst_out) {
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_inst; i += gridDim.x * 256) {
       const InstDesc I = inst[i];
       const uint64_t o0 = bin_out[I.bin0];
-      inst_out[2 * (uint64_t)i] = o0 + I.pairs0;
-      inst_out[2 * (uint64_t)i + 1] = bin_out[I.bin0 + I.nb] - o0 + I.pairs_extra;
+      inst_out[2 * (uint64_t)i] = o0 + I.pairs0 + (merge ? merge[n_inst + i] : 0ull);
+      inst_out[2 * (uint64_t)i + 1] = bin_out[I.bin0 + I.nb] - o0 + I.pairs_extra + (merge ? merge[i] : 0ull);
     }
   }
   const uint32_t gb = blockIdx.x;
   if (gb >= n_bins) return;
   const uint2 seg = bin_seg[gb];
-  const uint64_t dst0 = bin_out[gb] + inst[bin_inst[gb]].pairs0;
+  const uint32_t gi = bin_inst[gb];
+  const uint64_t dst0 = bin_out[gb] + inst[gi].pairs0 + (merge ? merge[n_inst + gi] : 0ull);
   const uint2* src = scratch + (size_t)(gb % nwg) * cap_wg;
   // a region that overflowed holds only its first cap_wg pairs (the host
   // regrows and runs again)
@@ -1748,7 +1752,8 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
                                                     const uint32_t* __restrict__ pstart,
                                                     const uint32_t* __restrict__ poverfull, uint32_t p_bins,
                                                     uint32_t p_dev, const Meta* __restrict__ pmeta, uint32_t target,
-                                                    DeepCaps caps, float root_inv_f, uint32_t max_refine,
+                                                    DeepCaps caps, float root_inv_f,
+                                                    const InstDesc* __restrict__ root_inst, uint32_t max_refine,
                                                     InstDesc* __restrict__ inst, uint32_t* __restrict__ chunk_inst,
                                                     uint32_t* __restrict__ bin_inst, Meta* __restrict__ meta) {
   __shared__ ScanLds L;
@@ -1771,11 +1776,13 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
     if (b < p_bins) {
       const uint32_t pi = pbin_inst[b];
       const InstDesc P = pinst[pi];
-      if (!plan_virtual_bin(P, pi, b - P.bin0, ptotal[b], pstart[b], target, caps.nb_max, &v, root_inv_f,
-                            max_refine)) {
+      const uint32_t root = p_dev ? P.root : pi;
+      if (!plan_virtual_bin(P, pi, b - P.bin0, ptotal[b], pstart[b], target, caps.nb_max, &v,
+                            root_inst ? root_inst[root].inv_f : root_inv_f, max_refine)) {
         v = InstDesc{};
         atomicOr(&unsplit, 1u);
       }
+      v.root = root;
     }
     inst[i] = v;
     atomicMax(&nbmax, v.n ? v.nb : 0u);
@@ -1846,10 +1853,11 @@ __global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, uint64_t* __res
 
 hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
                              const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
-                             uint32_t max_refine, uint32_t n_cu, uint2* out, uint64_t cap, hipStream_t s) {
+                             const InstDesc* root_inst, uint32_t max_refine, uint32_t n_cu, uint32_t direct,
+                             uint2* out, uint64_t cap, hipStream_t s) {
   k_deep_plan<<<1, 1024, 0, s>>>(parent.inst, parent.bin_inst, parent.total, parent.bin_start, parent.overfull,
-                                 parent_bins, parent_dev ? 1u : 0u, parent.meta, target, caps, root_inv_f, max_refine,
-                                 ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta);
+                                 parent_bins, parent_dev ? 1u : 0u, parent.meta, target, caps, root_inv_f, root_inst,
+                                 max_refine, ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta);
   const uint32_t nc = caps.chunks ? caps.chunks : 1u, nbins = caps.bins ? caps.bins : 1u;
   k_bin_hist<true><<<nc, kHistThreads, 0, s>>>(parent.bbox, ws.inst, ws.chunk_inst, 0u, ws.hist, ws.meta);
   k_bin_scan<true><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(
@@ -1862,7 +1870,105 @@ hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins
       ws.cap_bucket, caps.nb_max ? caps.nb_max : 1u, ws.total, 0u, 0u, ws.meta);
   k_bin_test<true><<<n_cu ? n_cu : 1u, kTestThreads, kTestLdsBytes, s>>>(
       ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, ws.bin_inst, 0u, ws.cap_bucket, ws.scratch, ws.cap_wg,
-      ws.bin_seg, ws.overfull, out, cap, 1u, ws.meta);
+      ws.bin_seg, ws.overfull, out, cap, direct, ws.meta);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batch merge of the device-planned levels' pairs (see launch_level_count
+// and the others in the header): per level-0 instance, the pairs its levels
+// found (by the root of each level bin), their exclusive scan (the shift of
+// the instance's run: the level-0 compaction leaves room after each
+// instance's own pairs) and a cursor that hands each level bin its place in
+// that room.
+
+__global__ __launch_bounds__(256) void k_merge_clear(uint64_t* __restrict__ merge, uint32_t n_inst) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_inst; i += gridDim.x * 256) {
+    merge[i] = 0ull;
+    merge[2ull * n_inst + i] = 0ull;
+  }
+}
+
+// Pairs of a level bin's run (a region that overflowed holds only its first
+// cap_wg pairs; the result words then ask for regrowth).
+__device__ __forceinline__ uint32_t region_run(uint2 seg, uint32_t cap_wg) {
+  return seg.x < cap_wg ? (seg.y < cap_wg - seg.x ? seg.y : cap_wg - seg.x) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_level_count(const uint2* __restrict__ bin_seg,
+                                                     const InstDesc* __restrict__ inst,
+                                                     const uint32_t* __restrict__ bin_inst, uint32_t cap_wg,
+                                                     const Meta* __restrict__ meta, uint64_t* __restrict__ merge) {
+  const uint32_t n_bins = meta->plan_err ? 0u : meta->dev_bins;
+  for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < n_bins; b += gridDim.x * 256) {
+    const uint32_t c = region_run(bin_seg[b], cap_wg);
+    if (c) atomicAdd((unsigned long long*)&merge[inst[bin_inst[b]].root], (unsigned long long)c);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_merge_scan(uint64_t* __restrict__ merge, uint32_t n_inst) {
+  __shared__ ScanLds L;
+  const uint64_t total = block_exscan(
+      n_inst, L, [&](uint32_t i) { return merge[i]; }, [&](uint32_t i, uint64_t ex) { merge[n_inst + i] = ex; });
+  if (threadIdx.x == 0) merge[3ull * n_inst] = total;
+}
+
+// One block per level bin: its run goes after its root's level-0 pairs (and
+// after the runs of the root's level bins placed before it).
+__global__ __launch_bounds__(256) void k_level_copy(const uint2* __restrict__ scratch, uint32_t cap_wg, uint32_t nwg,
+                                                    const uint2* __restrict__ bin_seg,
+                                                    const InstDesc* __restrict__ inst,
+                                                    const uint32_t* __restrict__ bin_inst,
+                                                    const Meta* __restrict__ meta,
+                                                    const InstDesc* __restrict__ inst0,
+                                                    const uint64_t* __restrict__ bin_out0,
+                                                    uint64_t* __restrict__ merge, uint32_t n_inst,
+                                                    uint2* __restrict__ out, uint64_t cap) {
+  __shared__ uint64_t dst0;
+  const uint32_t gb = blockIdx.x;
+  const uint32_t n_bins = meta->plan_err ? 0u : meta->dev_bins;
+  if (gb >= n_bins) return;
+  const uint2 seg = bin_seg[gb];
+  const uint32_t n = region_run(seg, cap_wg);
+  if (!n) return;
+  if (threadIdx.x == 0) {
+    const uint32_t r = inst[bin_inst[gb]].root;
+    const InstDesc R = inst0[r];
+    dst0 = bin_out0[R.bin0 + R.nb] + R.pairs0 + merge[n_inst + r] +
+           atomicAdd((unsigned long long*)&merge[2ull * n_inst + r], (unsigned long long)n);
+  }
+  __syncthreads();
+  const uint2* src = scratch + (size_t)(gb % nwg) * cap_wg + seg.x;
+  for (uint32_t k = threadIdx.x; k < n; k += 256) {
+    const uint64_t d = dst0 + k;
+    if (d < cap) out[d] = src[k];
+  }
+}
+
+hipError_t launch_merge_clear(uint64_t* merge, uint32_t n_inst, hipStream_t s) {
+  const uint32_t g = (n_inst + 255u) / 256u;
+  k_merge_clear<<<g ? (g < 1024u ? g : 1024u) : 1u, 256, 0, s>>>(merge, n_inst);
+  return hipGetLastError();
+}
+
+hipError_t launch_level_count(const BinnedWorkspace& lv, uint32_t max_bins, uint64_t* merge, hipStream_t s) {
+  const uint32_t g = (max_bins + 255u) / 256u;
+  k_level_count<<<g ? (g < 1024u ? g : 1024u) : 1u, 256, 0, s>>>(lv.bin_seg, lv.inst, lv.bin_inst, lv.cap_wg,
+                                                                  lv.meta, merge);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_scan(uint64_t* merge, uint32_t n_inst, hipStream_t s) {
+  k_merge_scan<<<1, 1024, 0, s>>>(merge, n_inst);
+  return hipGetLastError();
+}
+
+hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint32_t lv_test_blocks,
+                             const BinnedWorkspace& ws0, const uint64_t* bin_out0, uint64_t* merge, uint32_t n_inst,
+                             uint2* out, uint64_t cap, hipStream_t s) {
+  k_level_copy<<<max_bins ? max_bins : 1u, 256, 0, s>>>(lv.scratch, lv.cap_wg, lv_test_blocks ? lv_test_blocks : 1u,
+                                                        lv.bin_seg, lv.inst, lv.bin_inst, lv.meta, ws0.inst, bin_out0,
+                                                        merge, n_inst, out, cap);
   return hipGetLastError();
 }
 
@@ -1939,11 +2045,12 @@ hipError_t launch_single_tables(const BinnedWorkspace& ws, const InstDesc& d, hi
 }
 
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
-                                uint64_t* inst_out, hipStream_t s) {
+                                uint64_t* inst_out, hipStream_t s, const uint64_t* merge) {
   const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;
-  k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
+  k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total,
+                                     merge ? merge + 3ull * bp.n_inst : nullptr, ws.meta);
   k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
-                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, merge);
   return hipGetLastError();
 }
 
@@ -1977,7 +2084,7 @@ hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, c
 
 hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
                          const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
-                         hipEvent_t* ev) {
+                         hipEvent_t* ev, bool output) {
   hipError_t e;
   // Every kernel tolerates an empty batch (no chunks, no bins): grids of at
   // least one block whose blocks find no work.
@@ -2016,12 +2123,12 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
                                                                    ws.overfull, out, cap, bp.direct, ws.meta);
   PBIG_MARK(5)
-  if (!bp.direct)
-    k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, ws.meta);
+  if (!bp.direct && output)
+    k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, nullptr, ws.meta);
   PBIG_MARK(6)
-  if (!bp.direct)
+  if (!bp.direct && output)
     k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
-                                        ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out);
+                                        ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, nullptr);
   PBIG_MARK(7)
 #undef PBIG_MARK
   return hipGetLastError();

@@ -777,7 +777,12 @@ int ensure_deep_level(pbig_hip_ctx* c, BinnedState& st, uint32_t l, uint64_t buc
   L.bp.test_blocks = (uint32_t)std::max(c->n_cu, 1);
   L.bp.n_cu = (uint32_t)c->n_cu;
   if (!L.ws.cap_wg) L.ws.cap_wg = (uint32_t)std::max<uint64_t>(8192, 2 * bucket / L.bp.test_blocks);
-  return ensure_state(c, L, bucket);
+  const uint64_t gen = L.generation;
+  const int rc = ensure_state(c, L, bucket);
+  // A graph captured with the levels holds their arrays' addresses: any
+  // reallocation invalidates it like one of level 0's.
+  if (L.generation != gen) ++st.generation;
+  return rc;
 }
 
 // Cap on the device levels' cell refinement (powers of two below the
@@ -822,7 +827,7 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
     const BinnedState& L = *st.dl[l];
     HIP_TRY(c, pbig::launch_deep_level(*parent, parent_bins, l > 1, L.ws, st.dcaps[l], target, st.inst[0].inv_f,
-                                       deep_refine(), (uint32_t)c->n_cu, d_out, cap, s));
+                                       nullptr, deep_refine(), (uint32_t)c->n_cu, 1u, d_out, cap, s));
     if (ev) HIP_TRY(c, hipEventRecord(ev[l + 1], s));
     dm.meta[l] = L.ws.meta;
     parent = &L.ws;
@@ -833,25 +838,39 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
   return PBIG_OK;
 }
 
-// The launch with device levels, synchronously.  0: done (*m: pair count,
-// level 0's memberships); 1: the levels cannot complete this plan (the host
-// re-binning runs instead); 2: the automatic choice's dense rule stopped
-// level 0 (the general path runs instead); < 0: an error.  Grows every level's workspace
-// to what it reports and runs again (a few times at most).
-int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
-  BinnedState& st = c->plain;
-  if (st.deep_failed || st.bp.n_inst != 1 || st.bp.n_bins == 0) return 1;
+// A launch with device levels of state st (one instance or a batch),
+// synchronously: enqueue(d_res) enqueues it with its result words in d_res.
+// 0: done (*m: pair count, level 0's memberships); 1: the levels cannot
+// complete this plan (the host re-binning runs instead); 2: the automatic
+// choice's dense rule stopped level 0 (the general path runs instead); < 0:
+// an error.  Grows every level's workspace to what it reports and runs
+// again (a few times at most).
+template <class Enqueue>
+int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, Enqueue enqueue) {
   int rc;
   if (!st.d_res) HIP_TRY(c, hipMalloc((void**)&st.d_res, 2 * sizeof(uint64_t)));
   if (!st.dcaps[1].inst) init_deep_caps(st);
   std::vector<uint64_t> bucket(pbig::kDeepLevels, 0);
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
     bucket[l] = st.dl[l] ? st.dl[l]->ws.cap_bucket : 2 * std::max<uint64_t>(st.ws.cap_bucket, 4096);
+  // The levels' arrays always match their bounds (dcaps, each level's
+  // cap_wg and bucket): a later launch trusts the bounds, so every growth
+  // below is allocated before the next attempt or the return.
+  auto allocate = [&]() -> int {
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+      const int r = ensure_deep_level(c, st, l, bucket[l]);
+      if (r) {  // (a level's arrays may be short of its bounds: no launch may take the levels)
+        st.deep_ok = false;
+        st.deep_failed = true;
+        return r;
+      }
+    }
+    return PBIG_OK;
+  };
+  if ((rc = allocate())) return rc;
   for (int attempt = 0; attempt < 6; ++attempt) {
-    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
-      if ((rc = ensure_deep_level(c, st, l, bucket[l]))) return rc;
     if ((rc = after_upload(c, st, s))) return rc;
-    if ((rc = enqueue_deep(c, st, d_boxes, d_out, cap, st.d_res, s))) return rc;
+    if ((rc = enqueue(st.d_res))) return rc;
     uint64_t res[2];
     std::vector<Meta> lm(pbig::kDeepLevels);
     HIP_TRY(c, hipMemcpyAsync(res, st.d_res, sizeof(res), hipMemcpyDeviceToHost, s));
@@ -907,10 +926,20 @@ int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t
         }
       }
     }
+    if ((rc = allocate())) return rc;
     if (!grown) break;  // bins the levels cannot split or finish
   }
   st.deep_failed = true;
   return 1;
+}
+
+// The plain path's two-pass launch followed by levels on the device,
+// synchronously (run_levels_sync's codes).
+int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
+  BinnedState& st = c->plain;
+  if (c->cfg.host_levels || st.deep_failed || st.bp.n_inst != 1 || st.bp.n_bins == 0) return 1;
+  return run_levels_sync(c, st, s, m,
+                         [&](uint64_t* res) { return enqueue_deep(c, st, d_boxes, d_out, cap, res, s); });
 }
 
 // Pairs of a launch's instances found outside it (by re-binning their
@@ -1369,7 +1398,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
       if (d_result) HIP_TRY(c, pbig::launch_result(dm, d_result, s));
       return PBIG_OK;
     }
-    if (c->plain.deep_ok && !c->plain.deep_failed) {
+    if (c->plain.deep_ok && !c->plain.deep_failed && !c->cfg.host_levels) {
       if ((rc = after_upload(c, c->plain, s))) return rc;
       return enqueue_deep(c, c->plain, (const float4*)d_boxes, (uint2*)d_pairs, cap, d_result, s);
     }
@@ -1606,6 +1635,9 @@ struct pbig_hip_batch {
   uint64_t graph_generation = 0;
   hipStream_t capture_stream = nullptr;
   uint64_t last_cap = 0;            // pair capacity of the last launch (or of the captured graph)
+  uint64_t* d_merge = nullptr;      // merge words of launches with device levels (3 n_inst + 1)
+  uint64_t cap_merge = 0;
+  bool last_levels = false;         // the last launch (or the captured graph) ran device levels
 };
 
 namespace {
@@ -1650,6 +1682,68 @@ int run_general_instance(pbig_hip_batch* b, const float4* d_boxes, uint32_t i, u
   return rc == PBIG_E_OVERFLOW ? PBIG_OK : rc;
 }
 
+// Device levels for a batch: every instance planned on the binned path and
+// the levels not found unable to complete it.
+bool batch_levels_apply(const pbig_hip_batch* b) {
+  if (b->ctx->cfg.host_levels || b->st.deep_failed || b->st.bp.n_bins == 0 || b->desc.empty()) return false;
+  for (uint8_t g : b->general)
+    if (g) return false;
+  return true;
+}
+
+// Enqueue a batch launch with levels on the device: level 0 (the two-pass
+// launch up to its test kernel), the levels over every bin beyond LDS of
+// every instance (their pairs kept per bin), then the merge: each
+// instance's run holds its level-0 pairs followed by its levels' pairs;
+// result words into d_result (optional).  The merge words are grown here.
+int enqueue_batch_levels(pbig_hip_batch* b, const float4* boxes, uint2* out, uint64_t cap, uint64_t* d_inst_out,
+                         uint64_t* d_result, hipStream_t s) {
+  pbig_hip_ctx* c = b->ctx;
+  BinnedState& st = b->st;
+  const uint32_t n_inst = st.bp.n_inst;
+  int rc;
+  if ((rc = grow(c, &b->d_merge, &b->cap_merge, 3 * (uint64_t)n_inst + 1))) return rc;
+  HIP_TRY(c, pbig::launch_merge_clear(b->d_merge, n_inst, s));
+  st.bp.direct = 0u;
+  st.bp.dense_exit = 0u;
+  HIP_TRY(c, pbig::launch_binned(st.ws, boxes, nullptr, st.bp, out, cap, nullptr, s, nullptr, false));
+  const BinnedWorkspace* parent = &st.ws;
+  uint32_t parent_bins = st.bp.n_bins;
+  pbig::DeepMetas dm{};
+  dm.meta[0] = st.ws.meta;
+  const uint32_t target = std::max<uint32_t>(c->cfg.bin_target / 2, 64);
+  const uint32_t n_cu = (uint32_t)std::max(c->n_cu, 1);
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
+    const BinnedState& L = *st.dl[l];
+    HIP_TRY(c, pbig::launch_deep_level(*parent, parent_bins, l > 1, L.ws, st.dcaps[l], target, 0.0f, st.ws.inst,
+                                       deep_refine(), n_cu, 0u, out, cap, s));
+    dm.meta[l] = L.ws.meta;
+    parent = &L.ws;
+    parent_bins = st.dcaps[l].bins;
+  }
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+    HIP_TRY(c, pbig::launch_level_count(st.dl[l]->ws, st.dcaps[l].bins, b->d_merge, s));
+  HIP_TRY(c, pbig::launch_merge_scan(b->d_merge, n_inst, s));
+  HIP_TRY(c, pbig::launch_binned_output(st.ws, st.bp, out, cap, d_inst_out, s, b->d_merge));
+  for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+    HIP_TRY(c, pbig::launch_level_copy(st.dl[l]->ws, st.dcaps[l].bins, n_cu, st.ws, st.ws.bin_out, b->d_merge, n_inst,
+                                       out, cap, s));
+  dm.levels = pbig::kDeepLevels;
+  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, d_result, s));
+  return PBIG_OK;
+}
+
+// A batch launch with device levels, synchronously (run_levels_sync's
+// codes; 1 also when the batch cannot take them).  On success the batch's
+// later asynchronous launches and captures take the levels too.
+int run_batch_levels_sync(pbig_hip_batch* b, const float4* boxes, uint2* out, uint64_t cap, hipStream_t s,
+                          Meta* m) {
+  if (!batch_levels_apply(b)) return 1;
+  return run_levels_sync(b->ctx, b->st, s, m, [&](uint64_t* res) {
+    return enqueue_batch_levels(b, boxes, out, cap, b->d_inst_out, res, s);
+  });
+}
+
 // (Re)plan a batch: validate and plan every instance, lay out the tables,
 // grow the workspace (it is never shrunk) and upload the tables on stream s
 // (which it synchronises).  A captured graph no longer matches and is dropped.
@@ -1677,6 +1771,7 @@ int plan_batch(pbig_hip_batch* b, const pbig_hip_instance* inst, uint32_t n_inst
   }
   b->desc.assign(inst, inst + n_inst);
   b->general = general;
+  b->st.deep_failed = false;  // (deep_ok and the levels' bounds stay: batches of other shapes follow)
   b->max_box_end = max_end;
   b->st.inst = plan;
   std::vector<uint64_t> skip(n_inst, ~0ull);
@@ -1691,6 +1786,7 @@ int plan_batch(pbig_hip_batch* b, const pbig_hip_instance* inst, uint32_t n_inst
   int rc;
   if ((rc = size_state(c, b->st))) return rc;
   if ((rc = grow(c, &b->d_inst_out, &b->cap_inst_out, 2 * (uint64_t)std::max<uint32_t>(n_inst, 1)))) return rc;
+  if ((rc = grow(c, &b->d_merge, &b->cap_merge, 3 * (uint64_t)std::max<uint32_t>(n_inst, 1) + 1))) return rc;
   return upload_tables(c, b->st, s);
 }
 
@@ -1736,7 +1832,7 @@ int pbig_hip_batch_destroy(pbig_hip_batch* b) {
   (void)hipDeviceSynchronize();
   destroy_graph(b);
   free_state(b->st);
-  for (void* p : {(void*)b->st.ws.meta, (void*)b->d_tmp, (void*)b->d_inst_out})
+  for (void* p : {(void*)b->st.ws.meta, (void*)b->d_tmp, (void*)b->d_inst_out, (void*)b->d_merge})
     if (p) (void)hipFree(p);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   delete b;
@@ -1763,9 +1859,12 @@ int pbig_hip_batch_launch(pbig_hip_batch* b, const float* d_boxes, uint64_t n_bo
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = pick_stream(c, stream);
   if ((rc = after_upload(c, b->st, s))) return rc;
+  b->last_cap = cap;
+  b->last_levels = b->st.deep_ok && batch_levels_apply(b);
+  if (b->last_levels)  // (settled by a synchronous call: levels on the device)
+    return enqueue_batch_levels(b, (const float4*)d_boxes, (uint2*)d_pairs, cap, d_inst_pairs, d_result, s);
   HIP_TRY(c, pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
                                  next_events(c, PBIG_PATH_BINNED)));
-  b->last_cap = cap;
   if (d_result)
     HIP_TRY(c, pbig::launch_result(b->st.ws.meta, d_result, s));
   return PBIG_OK;
@@ -1782,12 +1881,21 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
   HIP_TRY(c, hipSetDevice(c->device));
   destroy_graph(b);
   if (b->st.stage_ev) HIP_TRY(c, hipEventSynchronize(b->st.stage_ev));  // tables in place before capture
+  const bool levels = b->st.deep_ok && batch_levels_apply(b);
+  if (levels && (rc = grow(c, &b->d_merge, &b->cap_merge, 3 * (uint64_t)b->st.bp.n_inst + 1)))
+    return rc;  // (no allocation under capture)
   hipStream_t s = b->capture_stream;
   HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap,
-                                     d_inst_pairs, s, nullptr);
-  if (e == hipSuccess && d_result)
-    e = pbig::launch_result(b->st.ws.meta, d_result, s);
+  hipError_t e = hipSuccess;
+  if (levels) {
+    rc = enqueue_batch_levels(b, (const float4*)d_boxes, (uint2*)d_pairs, cap, d_inst_pairs, d_result, s);
+    if (rc != PBIG_OK) e = hipErrorUnknown;
+  } else {
+    e = pbig::launch_binned(b->st.ws, (const float4*)d_boxes, nullptr, b->st.bp, (uint2*)d_pairs, cap, d_inst_pairs, s,
+                            nullptr);
+    if (e == hipSuccess && d_result)
+      e = pbig::launch_result(b->st.ws.meta, d_result, s);
+  }
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -1798,6 +1906,7 @@ int pbig_hip_batch_capture(pbig_hip_batch* b, const float* d_boxes, uint64_t n_b
   HIP_TRY(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   b->graph_generation = b->st.generation;
   b->last_cap = cap;
+  b->last_levels = levels;
   return PBIG_OK;
 }
 
@@ -1814,6 +1923,25 @@ int pbig_hip_batch_replay(pbig_hip_batch* b, void* stream) {
 
 int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
                      uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream, bool after_launch);
+
+// The end of a synchronous batch check: every instance's (offset, count)
+// to the host, the statistics, the total and the capacity check.
+int batch_check_done(pbig_hip_batch* b, const Meta& m, uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs) {
+  pbig_hip_ctx* c = b->ctx;
+  const uint32_t n_inst = (uint32_t)b->desc.size();
+  if (h_inst_pairs && n_inst)
+    HIP_TRY(c, hipMemcpy(h_inst_pairs, b->d_inst_out, 2 * (size_t)n_inst * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  c->stats = pbig_hip_stats{};
+  c->stats.n_boxes = b->st.bp.n_boxes;
+  c->stats.memberships = m.memberships;
+  c->stats.n_pairs = m.n_pairs;
+  c->stats.path = PBIG_PATH_BINNED;
+  *n_pairs = m.n_pairs;
+  if (m.n_pairs > cap)
+    return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)m.n_pairs,
+                (unsigned long long)cap);
+  return PBIG_OK;
+}
 
 int pbig_hip_batch_check(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_total, uint32_t* d_pairs,
                          uint64_t cap, uint64_t* h_inst_pairs, uint64_t* n_pairs, void* stream) {
@@ -1846,6 +1974,20 @@ int pbig_hip_batch_unresolved(pbig_hip_batch* b, uint8_t* h_flags, uint32_t* n_f
     return fail(c, PBIG_E_OVERFLOW, "the last launch found %llu pairs, more than its capacity of %llu",
                 (unsigned long long)m.n_pairs, (unsigned long long)b->last_cap);
   std::fill(h_flags, h_flags + n_inst, (uint8_t)0);
+  if (b->last_levels) {
+    // Levels on the device took the bins beyond LDS: every instance is
+    // complete unless a level reported trouble or still listed such bins
+    // (then each instance with a bin beyond LDS at level 0 is flagged).
+    std::vector<Meta> lm(pbig::kDeepLevels);
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+      HIP_TRY(c, hipMemcpyAsync(&lm[l], st.dl[l]->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    bool complete = true;
+    for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
+      complete = complete && lm[l].plan_err == 0 && !(lm[l].error & ~(unsigned long long)pbig::kErrOverfull);
+    complete = complete && !(lm[pbig::kDeepLevels - 1].error & pbig::kErrOverfull);
+    if (complete) return PBIG_OK;
+  }
   uint32_t nf = 0;
   const uint32_t n_bins = st.bp.n_bins;
   const uint32_t n_over = (uint32_t)std::min<uint64_t>(m.max_per_cell, n_bins);
@@ -1879,10 +2021,30 @@ int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_t
   const float4* boxes = (const float4*)d_boxes;
   const uint32_t n_inst = (uint32_t)b->desc.size();
   BinnedState& st = b->st;
+  Meta m{};
+  // Device levels: the two-pass launch alone first (unless this batch
+  // already settled on the levels, or a launch just reported its bins
+  // beyond LDS); bins beyond LDS, and nothing else, then run the launch
+  // again with levels planned on the device over every such bin.
+  bool done = false;
+  if (batch_levels_apply(b)) {
+    bool levels = st.deep_ok || after_launch;
+    if (!levels) {
+      if ((rc = run_state_sync(c, st, boxes, nullptr, (uint2*)d_pairs, cap, b->d_inst_out, s, &m))) return rc;
+      done = m.error == 0;
+      levels = (m.error & ~(unsigned long long)pbig::kErrOverfull) == 0 && (m.error & pbig::kErrOverfull);
+    }
+    if (!done && levels) {
+      rc = run_batch_levels_sync(b, boxes, (uint2*)d_pairs, cap, s, &m);
+      if (rc < 0) return rc;
+      done = rc == 0;
+    }
+    after_launch = false;  // (the host path below starts over)
+  }
+  if (done) return batch_check_done(b, m, cap, h_inst_pairs, n_pairs);
   // The launch, with heavy coarse bins re-binned one level down; instances
   // that stay unresolved (or that the binned path cannot plan) go to the
   // general path, and every instance's pairs are then placed in its run.
-  Meta m{};
   Splice sp;
   std::vector<uint8_t> failed;
   bool pass_done = false;
@@ -1921,18 +2083,7 @@ int batch_check_impl(pbig_hip_batch* b, const float* d_boxes, uint64_t n_boxes_t
     }
   }
   if (!sp.empty() && (rc = apply_splice(c, st, sp, (uint2*)d_pairs, cap, b->d_inst_out, s, &m))) return rc;
-  if (h_inst_pairs && n_inst)
-    HIP_TRY(c, hipMemcpy(h_inst_pairs, b->d_inst_out, 2 * (size_t)n_inst * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  c->stats = pbig_hip_stats{};
-  c->stats.n_boxes = st.bp.n_boxes;
-  c->stats.memberships = m.memberships;
-  c->stats.n_pairs = m.n_pairs;
-  c->stats.path = PBIG_PATH_BINNED;
-  *n_pairs = m.n_pairs;
-  if (m.n_pairs > cap)
-    return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)m.n_pairs,
-                (unsigned long long)cap);
-  return PBIG_OK;
+  return batch_check_done(b, m, cap, h_inst_pairs, n_pairs);
 }
 
 }  // extern "C"

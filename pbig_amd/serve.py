@@ -24,15 +24,19 @@ per launch:
   launch, no batch plan, and a dense request goes to the general path after
   the bin totals.
 * Completion work off the loop.  A launch whose result word asks for host
-  work (re-binning dense bins, a dense plain request, regrowth) is finished
-  by a synchronous call in a worker thread; every slot has its own context,
-  so the loop keeps admitting, launching and delivering for the other slots
-  meanwhile.  Under light load (no more requests waiting than slots), when
-  a batch's only trouble is bins beyond LDS, its requests without such a bin
-  are delivered at once and the others are queued again as a batch of
-  their own (one request alone runs on the plain path, whose dense exit is a
-  short general-path call), so one clustered request does not hold up its
-  batch; under a backlog the batch is finished whole, which costs less
+  work (bins beyond LDS, a dense plain request, regrowth) is finished by a
+  synchronous call in a worker thread; every slot has its own context, so
+  the loop keeps admitting, launching and delivering for the other slots
+  meanwhile.  A batch's bins beyond LDS are re-binned by levels planned on
+  the device; the first such finish settles the slot's batch on them, and
+  its later launches complete on the device (regrowing when a batch needs
+  more than the levels hold).  With ``host_levels`` (re-binning by the host
+  between levels), under light load (no more requests waiting than slots),
+  when a batch's only trouble is bins beyond LDS, its requests without such
+  a bin are delivered at once and the others are queued again as a batch of
+  their own (one request alone runs on the plain path, whose dense exit is
+  a short general-path call), so one clustered request does not hold up
+  its batch; under a backlog the batch is finished whole, which costs less
   device time.
 * Completion.  With ``results="device"`` a request's pairs stay in HBM (a
   view of the slot's pair buffer, valid while the consumer runs); with
@@ -173,7 +177,7 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 8 << 30, slots: int = 2, pool_frac: float = 0.5,
                  table: dict | None = None, bytes_per_box: int = 96, results: str = "host", consumer=None,
-                 max_batch_boxes: int | None = None):
+                 max_batch_boxes: int | None = None, host_levels: bool = False):
         import torch
 
         if results not in ("host", "device"):
@@ -182,10 +186,11 @@ class Server:
         self.dev = torch.device(f"cuda:{device}")
         torch.zeros(1, device=self.dev)  # torch's HIP runtime before the engine's context
         self.table = table or autotune.load_table()
+        self.host_levels = host_levels
         self.results = results
         self.consumer = consumer or keep_copy
         self.eng = pbig_amd.Engine(device)
-        self.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
+        self._configure(self.eng, pbig_amd.PATH_AUTO)
         pool_boxes = int(hbm_budget * pool_frac) // 16
         self.pool = Pool(pool_boxes)
         self.d_pool = torch.empty((pool_boxes, 4), dtype=torch.float32, device=self.dev)
@@ -195,7 +200,7 @@ class Server:
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.dev)) for _ in range(slots)]
         for sl in self.slots:
             sl.eng = pbig_amd.Engine(device)
-            sl.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
+            self._configure(sl.eng, pbig_amd.PATH_AUTO)
         self.pool_exec = ThreadPoolExecutor(max_workers=max(slots, 1))
         self.queue: list[Request] = []       # arrived, not admitted
         self.admitted: list[Request] = []    # upload issued, not yet batched
@@ -204,6 +209,9 @@ class Server:
         self.n_batches = 0
         self.batch_sizes: list[int] = []
         self.n_requeued = 0                  # requests a batch left incomplete, rerun alone
+
+    def _configure(self, eng, path: int):
+        eng.configure(path, int(self.table["bin_target"]), host_levels=self.host_levels)
 
     # -- client side ---------------------------------------------------------
     def submit(self, req: Request):
@@ -293,7 +301,7 @@ class Server:
         self._ensure_buffers(s, len(reqs), total)
         s.mode = "plain" if len(reqs) == 1 and reqs[0].n > 0 else "batch"
         if s.mode == "plain":
-            s.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))  # no memory of earlier requests
+            self._configure(s.eng, pbig_amd.PATH_AUTO)  # no memory of earlier requests
             if self.upload_ev is not None:
                 s.stream.wait_event(self.upload_ev)
             r = reqs[0]
@@ -352,7 +360,7 @@ class Server:
             r = s.reqs[0]
             view = self.d_pool[r._off:r._off + r.n]
             if err & 16:
-                s.eng.configure(pbig_amd.PATH_GENERAL, int(self.table["bin_target"]))
+                self._configure(s.eng, pbig_amd.PATH_GENERAL)
             try:
                 total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream, allow_overflow=True)
                 if total > s.pairs.shape[0]:
@@ -360,7 +368,7 @@ class Server:
                     total = s.eng.check_device(view, r.L, r.cell_size, s.pairs, stream=s.stream)
             finally:
                 # the slot's later batches choose their paths automatically again
-                s.eng.configure(pbig_amd.PATH_AUTO, int(self.table["bin_target"]))
+                self._configure(s.eng, pbig_amd.PATH_AUTO)
             s.h_io[0, 0] = 0
             s.h_io[0, 1] = total
             return total

@@ -280,9 +280,8 @@ def test_batch_c4_full(torch_cuda, eng):
 @pytest.mark.parametrize("path", ["binned", "auto"])
 def test_batch_clustered_instances_rebinned(torch_cuda, eng, path):
     """Clustered instances whose coarse bins exceed LDS, mixed with uniform
-    ones: on the binned path heavy bins are re-binned and spliced into their
-    instance's run; under the automatic choice these few dense instances go
-    to the general path instead.  Exact either way."""
+    ones: heavy bins are re-binned (by levels on the device) and their pairs
+    placed in their instance's run.  Exact, and again on a second check."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -313,9 +312,27 @@ def test_batch_clustered_instances_rebinned(torch_cuda, eng, path):
         check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io2), want)
 
 
+class host_levels:
+    """The engine configured to re-bin bins beyond LDS on the host (no device
+    levels) inside the block; the automatic defaults after it."""
+
+    def __init__(self, eng):
+        self.eng = eng
+
+    def __enter__(self):
+        import pbig_amd
+
+        self.eng.configure(pbig_amd.PATH_AUTO, host_levels=True)
+
+    def __exit__(self, *exc):
+        import pbig_amd
+
+        self.eng.configure(pbig_amd.PATH_AUTO)
+
+
 def test_batch_finish_after_launch(torch_cuda, eng):
-    """An asynchronous launch that reports bins beyond LDS is completed by
-    finish() from its own results (no second launch), exactly."""
+    """Host re-binning: an asynchronous launch that reports bins beyond LDS is
+    completed by finish() from its own results (no second launch), exactly."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -327,7 +344,7 @@ def test_batch_finish_after_launch(torch_cuda, eng):
     d = to_dev(torch_cuda, boxes)
     pairs = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
     cells = [40.0, 13.0]
-    with pbig_amd.Batch(eng, offsets, ranges, cells) as b:
+    with host_levels(eng), pbig_amd.Batch(eng, offsets, ranges, cells) as b:
         res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
         b.check(d, pairs)  # sizes the workspace
         pairs.fill_(-1)
@@ -339,15 +356,99 @@ def test_batch_finish_after_launch(torch_cuda, eng):
         check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
 
 
+def mixed_batch(seed, k=10):
+    """Clustered instances whose coarse bins exceed LDS among uniform ones."""
+    from pbig_amd import workloads
+
+    inst = []
+    for t in range(k):
+        n = [3000, 40000, 120000, 7000, 250000][t % 5]
+        L = 65536.0 * float(np.sqrt(n / 2097152.0))
+        if t % 2:
+            inst.append((workloads.clustered(n, L, seed + t), L))
+        else:
+            inst.append((workloads.uniform(n, L, 1.0, 32.0, seed + t), L))
+    boxes, offsets, ranges = concat(inst)
+    cells = [float(np.float32(L / np.sqrt(offsets[i + 1] - offsets[i]))) for i, (_, L) in enumerate(inst)]
+    return boxes, offsets, ranges, cells
+
+
+def test_batch_levels_on_device(torch_cuda, eng):
+    """Device levels for a batch: the first asynchronous launch reports bins
+    beyond LDS; finish() runs the launch again with levels planned on the
+    device over every such bin of the batch, and each instance's run holds
+    its level-0 pairs followed by its levels' pairs.  The batch keeps the
+    levels: its next launches complete on the device (error word 0), also
+    captured in a graph and replayed (with eager work between replays), and
+    unresolved() flags none.  Re-planned for other instances, its launch
+    completes, or regrows through finish().  Exact every time."""
+    import pbig_amd
+
+    eng.configure(pbig_amd.PATH_AUTO)
+    boxes, offsets, ranges, cells = mixed_batch(400)
+    want = expected(boxes, offsets)
+    total = sum(w.shape[0] for w in want)
+    d = to_dev(torch_cuda, boxes)
+    pairs = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    k = len(want)
+    io = torch_cuda.zeros((k, 2), dtype=torch_cuda.int64, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+
+    def check_launch(tag):
+        torch_cuda.cuda.synchronize()
+        assert res.tolist() == [total, 0], (tag, res.tolist())
+        hio = io.cpu().numpy()
+        check_all(split(pairs[:total].cpu().numpy().view(np.uint32), hio), want)
+
+    with pbig_amd.Batch(eng, offsets, ranges, cells) as b:
+        b.launch(d, pairs, io, res)
+        torch_cuda.cuda.synchronize()
+        assert res[1].item() & 8, res.tolist()  # bins beyond LDS (and perhaps a workspace to grow)
+        n, hio = b.check(d, pairs, finish=True)
+        assert n == total
+        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), hio), want)
+        for rep in range(2):
+            pairs.fill_(-1)
+            io.zero_()
+            b.launch(d, pairs, io, res)
+            check_launch(f"launch {rep}")
+        assert not b.unresolved().any()
+        b.capture(d, pairs, io, res)
+        for rep in range(3):
+            pairs.fill_(-1)
+            io.zero_()
+            res.zero_()
+            torch_cuda.cuda.synchronize()
+            b.replay()
+            check_launch(f"replay {rep}")
+            torch_cuda.ones(1 << 20, device="cuda:0").sum().item()  # eager work between replays
+        # other instances on the same batch object
+        boxes2, offsets2, ranges2, cells2 = mixed_batch(500, k=7)
+        want2 = expected(boxes2, offsets2)
+        total2 = sum(w.shape[0] for w in want2)
+        d2 = to_dev(torch_cuda, boxes2)
+        pairs2 = torch_cuda.empty((total2 + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+        io2 = torch_cuda.zeros((len(want2), 2), dtype=torch_cuda.int64, device="cuda:0")
+        b.update(offsets2[:-1], np.diff(offsets2), ranges2, cells2)
+        b.launch(d2, pairs2, io2, res)
+        torch_cuda.cuda.synchronize()
+        n2, err2 = res.tolist()
+        if err2:
+            n2, hio2 = b.check(d2, pairs2, finish=True)
+        else:
+            hio2 = io2.cpu().numpy()
+        assert n2 == total2
+        check_all(split(pairs2[:n2].cpu().numpy().view(np.uint32), hio2), want2)
+
+
 def test_batch_unresolved_refuses_after_overflow(torch_cuda, eng):
-    """A launch whose only error is bins beyond LDS but whose pairs exceed the
-    launch's capacity: the pairs past the capacity were dropped, so
-    unresolved() must not vouch for the other instances' runs."""
+    """Host re-binning: a launch whose only error is bins beyond LDS but whose
+    pairs exceed the launch's capacity: the pairs past the capacity were
+    dropped, so unresolved() must not vouch for the other instances' runs."""
     import oracle
     import pbig_amd
     from pbig_amd import workloads
 
-    eng.configure(pbig_amd.PATH_AUTO)
     inst = [(workloads.uniform(50000, 3000.0, 1.0, 8.0, 311), 3000.0),
             (workloads.clustered(150000, 17000.0, 310), 17000.0)]
     boxes, offsets, ranges = concat(inst)
@@ -356,7 +457,7 @@ def test_batch_unresolved_refuses_after_overflow(torch_cuda, eng):
     d = to_dev(torch_cuda, boxes)
     big = torch_cuda.empty((total + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
     small = torch_cuda.empty((want[0].shape[0] // 2, 2), dtype=torch_cuda.int32, device="cuda:0")
-    with pbig_amd.Batch(eng, offsets, ranges, [13.0, 40.0]) as b:
+    with host_levels(eng), pbig_amd.Batch(eng, offsets, ranges, [13.0, 40.0]) as b:
         b.check(d, big)  # sizes the workspace
         res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
         b.launch(d, small, None, res)
