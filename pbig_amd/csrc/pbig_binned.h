@@ -246,7 +246,8 @@ struct DeepMetas {
 
 // One re-binning level: its plan kernel reads the level above's list of bins
 // beyond LDS (parent, parent_bins bins; parent_dev: that level was itself
-// planned on the device), then hist, scan, starts, scatter and test run over
+// planned on the device; parent_level: its index, for the bins it cannot
+// split, which go to dense_list), then hist, scan, starts, scatter and test run over
 // its virtual instances with device counts.  direct: its test kernel writes
 // pairs through the shared counter in its meta's out_count (one instance);
 // else it leaves per-bin runs in its regions, for the batch merge below.
@@ -256,7 +257,8 @@ struct DeepMetas {
 hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
                              const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
                              const InstDesc* root_inst, uint32_t max_refine, uint32_t n_cu, uint32_t direct,
-                             uint2* out, uint64_t cap, hipStream_t s);
+                             uint2* out, uint64_t cap, uint32_t parent_level, uint32_t* dense_list,
+                             Meta* dense, hipStream_t s);
 
 // Batch merge of the levels' pairs into the instances' runs.  `merge` holds
 // 3 * n_inst + 1 words: per level-0 instance the pairs its levels found, the
@@ -273,7 +275,50 @@ hipError_t launch_merge_scan(uint64_t* merge, uint32_t n_inst, hipStream_t s);
 hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint32_t lv_test_blocks,
                              const BinnedWorkspace& ws0, const uint64_t* bin_out0, uint64_t* merge, uint32_t n_inst,
                              uint2* out, uint64_t cap, hipStream_t s);
-hipError_t launch_deep_result(const DeepMetas& m, uint64_t* result, hipStream_t s);
+hipError_t launch_deep_result(const DeepMetas& m, const Meta* dense, uint64_t* result, hipStream_t s);
+
+// Bins no level resolves -- bins a level's plan cannot split (a single fine
+// cell over LDS at the finest cells) and bins the last level still lists
+// beyond LDS -- are tested whole on the device: every pair of their
+// memberships, in tiles of kDenseTile x kDenseTile, under the reference-cell
+// rule on the bin's cells.  The plans append (level << 30 | bin) to `list`
+// (count in the dense meta's dev_inst); the last level's list is read from
+// its workspace.  At most kDenseMaxBins bins per launch (else kErrOverfull).
+constexpr uint32_t kDenseTile = 256;
+constexpr uint32_t kDenseMaxBins = 1024;
+struct DenseSrc {
+  const float4* bbox[kDeepLevels];
+  const uint32_t* bid[kDeepLevels];
+  const uint32_t* bin_start[kDeepLevels];
+  const uint32_t* total[kDeepLevels];
+  const InstDesc* inst[kDeepLevels];
+  const uint32_t* bin_inst[kDeepLevels];
+  const uint32_t* last_overfull;  // the last level's bins beyond LDS
+  const Meta* last_meta;
+  const uint32_t* list;           // bins the plans could not split
+};
+// A run of a batch's dense pairs (batch merge): pairs[off, off + n) belong to
+// level-0 instance `root`.
+struct DenseChunk {
+  uint64_t off;
+  uint32_t n;
+  uint32_t root;
+};
+// dense: its meta (zeroed by launch_dense_clear at the start of the launch):
+// dev_inst = bins appended by the plans; n_pairs = pairs found (batch);
+// items = runs (batch); error = kErrWorkspace when `pairs` or `chunks` was
+// short, kErrOverfull when more than kDenseMaxBins bins were listed.
+// direct: pairs through out_count into out (one instance); else into
+// pairs / chunks for the merge (launch_dense_count, launch_dense_copy).
+hipError_t launch_dense_clear(Meta* dense, hipStream_t s);
+hipError_t launch_dense_bins(const DenseSrc& src, uint32_t last_level, Meta* dense, uint32_t n_cu,
+                             unsigned long long* out_count, uint2* out, uint64_t cap, uint2* pairs,
+                             uint64_t cap_pairs, DenseChunk* chunks, uint32_t cap_chunks, hipStream_t s);
+hipError_t launch_dense_count(const DenseChunk* chunks, uint32_t cap_chunks, const Meta* dense, uint64_t* merge,
+                              uint32_t n_inst, hipStream_t s);
+hipError_t launch_dense_copy(const DenseChunk* chunks, uint32_t cap_chunks, const Meta* dense, const uint2* pairs,
+                             const BinnedWorkspace& ws0, uint64_t* merge, uint32_t n_inst, uint2* out, uint64_t cap,
+                             hipStream_t s);
 
 // Tables of a launch of one instance written by a kernel from its argument:
 // the instance record, and zero in the chunk -> instance and bin -> instance

@@ -1755,18 +1755,17 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
                                                     DeepCaps caps, float root_inv_f,
                                                     const InstDesc* __restrict__ root_inst, uint32_t max_refine,
                                                     InstDesc* __restrict__ inst, uint32_t* __restrict__ chunk_inst,
-                                                    uint32_t* __restrict__ bin_inst, Meta* __restrict__ meta) {
+                                                    uint32_t* __restrict__ bin_inst, Meta* __restrict__ meta,
+                                                    uint32_t parent_level, uint32_t* __restrict__ dense_list,
+                                                    Meta* __restrict__ dense) {
   __shared__ ScanLds L;
   __shared__ uint32_t c0[kDeepMaxInst + 1], b0[kDeepMaxInst + 1];
-  __shared__ uint32_t nbmax, unsplit;
+  __shared__ uint32_t nbmax;
   const uint32_t tid = threadIdx.x;
   // (a level planned on the host has no plan error word of its own)
   uint32_t n_over = p_dev && pmeta->plan_err ? 0u : pmeta->max_per_cell;
   n_over = n_over < p_bins ? n_over : p_bins;
-  if (tid == 0) {
-    nbmax = 0u;
-    unsplit = 0u;
-  }
+  if (tid == 0) nbmax = 0u;
   __syncthreads();
   const uint32_t n_inst = n_over <= caps.inst && n_over <= kDeepMaxInst ? n_over : 0u;
   // 1. every listed bin's virtual instance
@@ -1779,8 +1778,10 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
       const uint32_t root = p_dev ? P.root : pi;
       if (!plan_virtual_bin(P, pi, b - P.bin0, ptotal[b], pstart[b], target, caps.nb_max, &v,
                             root_inst ? root_inst[root].inv_f : root_inv_f, max_refine)) {
+        // no finer bins: the parent's bin is tested whole (dense kernel)
         v = InstDesc{};
-        atomicOr(&unsplit, 1u);
+        const uint32_t at = atomicAdd(&dense->dev_inst, 1u);
+        if (at < kDenseMaxBins) dense_list[at] = (parent_level << 30) | b;
       }
       v.root = root;
     }
@@ -1828,15 +1829,17 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
     meta->dev_nbmax = nbmax;
     meta->dev_hist = hist;
     meta->dev_off = off;
-    meta->plan_err = (fits ? 0u : 1u) | (unsplit ? 2u : 0u);
+    meta->plan_err = fits ? 0u : 1u;
   }
 }
 
 // The result words of a launch with device-planned levels: the pair count
-// (the shared counter), and an error word: an invalid box, a workspace that
-// did not fit at any level (a level's plan, bucket or pair regions), or bins
-// beyond LDS that the last level still lists, or that could not be split.
-__global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, uint64_t* __restrict__ result) {
+// (level 0's: the shared counter of one instance, or a batch's merged total),
+// and an error word: an invalid box, a workspace that did not fit at any
+// level (a level's plan, bucket or pair regions, or the dense kernel's
+// buffers), or more bins left to the dense kernel than it takes.
+__global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, const Meta* __restrict__ dense,
+                                                    uint64_t* __restrict__ result) {
   if (threadIdx.x != 0) return;
   unsigned long long e = 0ull;
   for (uint32_t l = 0; l < m.levels; ++l) {
@@ -1844,9 +1847,8 @@ __global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, uint64_t* __res
     e |= x->error & (kErrInvalidBox | kErrWorkspace | kErrTooDense | kErrDense);
     if (l == 0) continue;  // level 0 is planned on the host
     if (x->plan_err & 1u) e |= kErrWorkspace;
-    if (x->plan_err & 2u) e |= kErrOverfull;
   }
-  e |= m.meta[m.levels - 1]->error & kErrOverfull;
+  e |= dense->error & (kErrWorkspace | kErrOverfull);
   result[0] = m.meta[0]->n_pairs;
   result[1] = e;
 }
@@ -1854,10 +1856,12 @@ __global__ __launch_bounds__(64) void k_deep_result(DeepMetas m, uint64_t* __res
 hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins, bool parent_dev,
                              const BinnedWorkspace& ws, const DeepCaps& caps, uint32_t target, float root_inv_f,
                              const InstDesc* root_inst, uint32_t max_refine, uint32_t n_cu, uint32_t direct,
-                             uint2* out, uint64_t cap, hipStream_t s) {
+                             uint2* out, uint64_t cap, uint32_t parent_level, uint32_t* dense_list, Meta* dense,
+                             hipStream_t s) {
   k_deep_plan<<<1, 1024, 0, s>>>(parent.inst, parent.bin_inst, parent.total, parent.bin_start, parent.overfull,
                                  parent_bins, parent_dev ? 1u : 0u, parent.meta, target, caps, root_inv_f, root_inst,
-                                 max_refine, ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta);
+                                 max_refine, ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta, parent_level, dense_list,
+                                 dense);
   const uint32_t nc = caps.chunks ? caps.chunks : 1u, nbins = caps.bins ? caps.bins : 1u;
   k_bin_hist<true><<<nc, kHistThreads, 0, s>>>(parent.bbox, ws.inst, ws.chunk_inst, 0u, ws.hist, ws.meta);
   k_bin_scan<true><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(
@@ -1972,8 +1976,263 @@ hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint3
   return hipGetLastError();
 }
 
-hipError_t launch_deep_result(const DeepMetas& m, uint64_t* result, hipStream_t s) {
-  k_deep_result<<<1, 64, 0, s>>>(m, result);
+// ---------------------------------------------------------------------------
+// Bins no level resolves, tested whole (see DenseSrc in the header).
+//
+// Every workgroup gathers the listed bins (at most kDenseMaxBins) and the
+// inclusive prefix of their tile counts in LDS; tiles are then taken
+// round-robin over the whole list.  A bin of m memberships has
+// t = ceil(m / kDenseTile) row tiles and t (t + 1) / 2 tiles (ti <= tj).  A
+// tile stages its column memberships (box, id, first fine column and row)
+// in LDS; each thread holds one row membership and tests it against every
+// column (only later ones on a diagonal tile).  A pair is the bin's when
+// its reference cell -- (max of the first columns, max of the first rows),
+// the cell of the lower-left corner of the two boxes' intersection -- lies
+// in the bin's cells: the rule the test kernel applies cell by cell.  Hits
+// go through a per-wave LDS stage.
+
+constexpr uint32_t kDenseThreads = kDenseTile;
+constexpr uint32_t kDenseStage = 512;              // staged pairs per wave
+constexpr uint32_t kDenseMaxTiles = 1u << 20;      // per bin (about 370,000 memberships)
+
+__global__ __launch_bounds__(64) void k_dense_clear(Meta* __restrict__ dense) {
+  if (threadIdx.x < sizeof(Meta) / sizeof(unsigned long long))
+    reinterpret_cast<unsigned long long*>(dense)[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(kDenseThreads) void k_dense_bins(DenseSrc src, uint32_t last_level,
+                                                              Meta* __restrict__ dense,
+                                                              unsigned long long* __restrict__ out_count,
+                                                              uint2* __restrict__ out, uint64_t cap,
+                                                              uint2* __restrict__ pairs, uint64_t cap_pairs,
+                                                              DenseChunk* __restrict__ chunks, uint32_t cap_chunks) {
+  __shared__ uint32_t ent[kDenseMaxBins];  // level << 30 | bin
+  __shared__ uint32_t pre[kDenseMaxBins];  // tiles, then their inclusive prefix
+  __shared__ float4 cb[kDenseTile];
+  __shared__ uint32_t cid[kDenseTile], cfx[kDenseTile], cfy[kDenseTile];
+  __shared__ uint2 stage[kDenseThreads / 64][kDenseStage];
+  __shared__ uint32_t wsum[kDenseThreads / 64];
+  __shared__ uint32_t too_big;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const Meta* lm = src.last_meta;
+  const uint32_t n_plan = dense->dev_inst;
+  const uint32_t n_last = lm->plan_err ? 0u : lm->max_per_cell;
+  if ((uint64_t)n_plan + n_last > kDenseMaxBins) {
+    if (blockIdx.x == 0 && tid == 0) atomicOr((unsigned long long*)&dense->error, (unsigned long long)kErrOverfull);
+    return;
+  }
+  const uint32_t n = n_plan + n_last;
+  if (n == 0) return;
+  if (tid == 0) too_big = 0u;
+  __syncthreads();
+  // 1. the list and each bin's tile count; then the inclusive prefix (four
+  //    consecutive entries per thread, wave scans, block carry)
+  constexpr uint32_t kPer = kDenseMaxBins / kDenseThreads;
+  uint32_t tl[kPer], acc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t e = tid * kPer + k;
+    tl[k] = 0u;
+    if (e < n) {
+      const uint32_t x = e < n_plan ? src.list[e] : ((last_level << 30) | (src.last_overfull[e - n_plan] & ~kOverfullCell));
+      ent[e] = x;
+      const uint32_t m = src.total[x >> 30][x & 0x3FFFFFFFu];
+      const uint64_t t = (m + kDenseTile - 1u) / kDenseTile;
+      const uint64_t tiles = t * (t + 1u) / 2u;
+      if (tiles > kDenseMaxTiles) atomicOr(&too_big, 1u);
+      tl[k] = tiles > kDenseMaxTiles ? 0u : (uint32_t)tiles;
+    }
+    acc += tl[k];
+  }
+  const uint32_t inc = wave_prefix_sum(acc);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (too_big) {  // (the same in every workgroup)
+    if (blockIdx.x == 0 && tid == 0) atomicOr((unsigned long long*)&dense->error, (unsigned long long)kErrOverfull);
+    return;
+  }
+  uint32_t run = inc - acc;
+  for (uint32_t k = 0; k < w; ++k) run += wsum[k];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    run += tl[k];
+    if (tid * kPer + k < n) pre[tid * kPer + k] = run;
+  }
+  __syncthreads();
+  const uint32_t total_tiles = pre[n - 1];
+  uint32_t sn = 0;  // this wave's staged pairs
+  uint32_t root = 0;
+  auto flush = [&]() {
+    uint64_t base = 0;
+    bool ok = true;
+    if (out_count) {
+      if (lane == 0) base = atomicAdd(out_count, (unsigned long long)sn);
+      base = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+             __builtin_amdgcn_readfirstlane((uint32_t)base);
+      for (uint32_t k = lane; k < sn; k += 64)
+        if (base + k < cap) out[base + k] = stage[w][k];
+    } else {
+      uint64_t ch = 0;
+      if (lane == 0) {
+        base = atomicAdd((unsigned long long*)&dense->n_pairs, (unsigned long long)sn);
+        ch = atomicAdd((unsigned long long*)&dense->items, 1ull);
+      }
+      base = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+             __builtin_amdgcn_readfirstlane((uint32_t)base);
+      ch = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ch >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)ch);
+      ok = base + sn <= cap_pairs && ch < cap_chunks;
+      if (ok) {
+        for (uint32_t k = lane; k < sn; k += 64) pairs[base + k] = stage[w][k];
+        if (lane == 0) {
+          DenseChunk c;
+          c.off = base;
+          c.n = sn;
+          c.root = root;
+          chunks[ch] = c;
+        }
+      } else if (lane == 0) {
+        atomicOr((unsigned long long*)&dense->error, (unsigned long long)kErrWorkspace);
+      }
+    }
+    sn = 0;
+  };
+  // 2. the tiles
+  for (uint32_t tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+    uint32_t lo = 0, hi = n - 1u;  // first entry whose prefix exceeds the tile
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] > tile) hi = mid; else lo = mid + 1u;
+    }
+    const uint32_t x = ent[lo], lv = x >> 30, b = x & 0x3FFFFFFFu;
+    const uint32_t u = tile - (lo ? pre[lo - 1u] : 0u);
+    const uint32_t m = src.total[lv][b], s0 = src.bin_start[lv][b];
+    const uint32_t t = (m + kDenseTile - 1u) / kDenseTile;
+    // row ti holds t - ti tiles: S(ti) = ti t - ti (ti - 1) / 2 tiles before it
+    auto S = [&](uint32_t r) { return (uint64_t)r * t - (uint64_t)r * (r - 1u) / 2u; };
+    const double tt = 2.0 * t + 1.0;
+    uint32_t ti = (uint32_t)fmax(0.0, floor((tt - sqrt(fmax(0.0, tt * tt - 8.0 * u))) / 2.0));
+    while (ti > 0u && S(ti) > u) --ti;
+    while (ti + 1u < t && S(ti + 1u) <= u) ++ti;
+    const uint32_t tj = ti + (uint32_t)(u - S(ti));
+    const uint32_t ii = src.bin_inst[lv][b];
+    const InstDesc I = src.inst[lv][ii];
+    root = lv ? I.root : ii;
+    const uint32_t lb = b - I.bin0;
+    const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
+    const uint32_t ex = I.fx0 + I.span_x - gx, ey = I.fy0 + I.span_y - gy;
+    const uint32_t kx = I.k < ex ? I.k : ex, ky = I.k < ey ? I.k : ey;
+    const float4* bb = src.bbox[lv] + s0;
+    const uint32_t* bi = src.bid[lv] + s0;
+    const uint32_t j0 = tj * kDenseTile, nc = m - j0 < kDenseTile ? m - j0 : kDenseTile;
+    if (tid < nc) {
+      const float4 q = bb[j0 + tid];
+      cb[tid] = q;
+      cid[tid] = bi[j0 + tid];
+      cfx[tid] = fine_of(q.x, I.inv_f, I.nf);
+      cfy[tid] = fine_of(q.z, I.inv_f, I.nf);
+    }
+    const uint32_t i = ti * kDenseTile + tid;
+    const bool row = i < m;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t ia = 0, fxa = 0, fya = 0;
+    if (row) {
+      a = bb[i];
+      ia = bi[i];
+      fxa = fine_of(a.x, I.inv_f, I.nf);
+      fya = fine_of(a.z, I.inv_f, I.nf);
+    }
+    __syncthreads();
+    const uint32_t jmin = ti == tj ? tid + 1u : 0u;  // a diagonal tile: later columns only
+    for (uint32_t jj = 0; jj < nc; ++jj) {
+      const float4 q = cb[jj];
+      const uint32_t rx = fxa > cfx[jj] ? fxa : cfx[jj], ry = fya > cfy[jj] ? fya : cfy[jj];
+      const bool hit = row && jj >= jmin && a.x <= q.y && q.x <= a.y && a.z <= q.w && q.z <= a.w &&
+                       rx - gx < kx && ry - gy < ky;
+      const uint64_t mask = __ballot(hit);
+      if (hit) {
+        const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        const uint32_t ib = cid[jj];
+        stage[w][pos] = make_uint2(ia < ib ? ia : ib, ia < ib ? ib : ia);
+      }
+      sn += (uint32_t)__popcll(mask);
+      if (sn > kDenseStage - 64u) flush();
+    }
+    if (sn) flush();
+    __syncthreads();  // the columns are reloaded
+  }
+}
+
+// Batch merge of the dense runs (after launch_dense_bins, whose error word
+// voids the runs: the launch then reports it and nothing is merged).
+__global__ __launch_bounds__(256) void k_dense_count(const DenseChunk* __restrict__ chunks, uint32_t cap_chunks,
+                                                     const Meta* __restrict__ dense, uint64_t* __restrict__ merge,
+                                                     uint32_t n_inst) {
+  if (dense->error) return;
+  const uint64_t nch = dense->items < cap_chunks ? dense->items : cap_chunks;
+  for (uint64_t c = blockIdx.x * 256ull + threadIdx.x; c < nch; c += gridDim.x * 256ull) {
+    const DenseChunk ch = chunks[c];
+    if (ch.root < n_inst) atomicAdd((unsigned long long*)&merge[ch.root], (unsigned long long)ch.n);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dense_copy(const DenseChunk* __restrict__ chunks, uint32_t cap_chunks,
+                                                    const Meta* __restrict__ dense, const uint2* __restrict__ pairs,
+                                                    const InstDesc* __restrict__ inst0,
+                                                    const uint64_t* __restrict__ bin_out0,
+                                                    uint64_t* __restrict__ merge, uint32_t n_inst,
+                                                    uint2* __restrict__ out, uint64_t cap) {
+  __shared__ uint64_t dst0;
+  if (dense->error) return;
+  const uint64_t nch = dense->items < cap_chunks ? dense->items : cap_chunks;
+  for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const DenseChunk ch = chunks[c];
+    if (ch.root >= n_inst) continue;  // (uniform over the block)
+    if (threadIdx.x == 0) {
+      const InstDesc R = inst0[ch.root];
+      dst0 = bin_out0[R.bin0 + R.nb] + R.pairs0 + merge[n_inst + ch.root] +
+             atomicAdd((unsigned long long*)&merge[2ull * n_inst + ch.root], (unsigned long long)ch.n);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < ch.n; k += 256) {
+      const uint64_t d = dst0 + k;
+      if (d < cap) out[d] = pairs[ch.off + k];
+    }
+    __syncthreads();  // dst0 is rewritten
+  }
+}
+
+hipError_t launch_dense_clear(Meta* dense, hipStream_t s) {
+  k_dense_clear<<<1, 64, 0, s>>>(dense);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_bins(const DenseSrc& src, uint32_t last_level, Meta* dense, uint32_t n_cu,
+                             unsigned long long* out_count, uint2* out, uint64_t cap, uint2* pairs,
+                             uint64_t cap_pairs, DenseChunk* chunks, uint32_t cap_chunks, hipStream_t s) {
+  const uint32_t grid = 4u * (n_cu ? n_cu : 1u);
+  k_dense_bins<<<grid, kDenseThreads, 0, s>>>(src, last_level, dense, out_count, out, cap, pairs, cap_pairs, chunks,
+                                             cap_chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_count(const DenseChunk* chunks, uint32_t cap_chunks, const Meta* dense, uint64_t* merge,
+                              uint32_t n_inst, hipStream_t s) {
+  const uint32_t g = (cap_chunks + 255u) / 256u;
+  k_dense_count<<<g ? (g < 1024u ? g : 1024u) : 1u, 256, 0, s>>>(chunks, cap_chunks, dense, merge, n_inst);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_copy(const DenseChunk* chunks, uint32_t cap_chunks, const Meta* dense, const uint2* pairs,
+                             const BinnedWorkspace& ws0, uint64_t* merge, uint32_t n_inst, uint2* out, uint64_t cap,
+                             hipStream_t s) {
+  k_dense_copy<<<1024, 256, 0, s>>>(chunks, cap_chunks, dense, pairs, ws0.inst, ws0.bin_out, merge, n_inst, out, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_deep_result(const DeepMetas& m, const Meta* dense, uint64_t* result, hipStream_t s) {
+  k_deep_result<<<1, 64, 0, s>>>(m, dense, result);
   return hipGetLastError();
 }
 

@@ -81,6 +81,14 @@ struct BinnedState {
   bool deep_ok = false, deep_failed = false;
   bool to_general = false;  // the automatic choice sends this plan's instance to the general path
   uint64_t* d_res = nullptr;  // result words of a synchronous call on them
+  // Bins no level resolves, tested whole: their list, status words, and (a
+  // batch) the pairs and runs they produce for the merge.
+  Meta* dense_meta = nullptr;
+  uint32_t* dense_list = nullptr;
+  uint2* dense_pairs = nullptr;
+  uint64_t cap_dense_pairs = 0;
+  pbig::DenseChunk* dense_chunks = nullptr;
+  uint64_t cap_dense_chunks = 0;
 };
 
 struct pbig_hip_ctx {
@@ -510,9 +518,15 @@ void free_state(BinnedState& st) {
     if (p) (void)hipFree(p);
   if (st.own_meta && b.meta) (void)hipFree(b.meta);
   st.ws = BinnedWorkspace{};
-  for (void* p : {(void*)st.fill[0], (void*)st.fill[1], (void*)st.meta1, (void*)st.start1, (void*)st.d_res})
+  for (void* p : {(void*)st.fill[0], (void*)st.fill[1], (void*)st.meta1, (void*)st.start1, (void*)st.d_res,
+                  (void*)st.dense_meta, (void*)st.dense_list, (void*)st.dense_pairs, (void*)st.dense_chunks})
     if (p) (void)hipFree(p);
   st.d_res = nullptr;
+  st.dense_meta = nullptr;
+  st.dense_list = nullptr;
+  st.dense_pairs = nullptr;
+  st.dense_chunks = nullptr;
+  st.cap_dense_pairs = st.cap_dense_chunks = 0;
   for (auto& lv : st.dl)
     if (lv) {
       free_state(*lv);
@@ -810,8 +824,47 @@ void init_deep_caps(BinnedState& st) {
   }
 }
 
+// The dense kernel's view of a state's levels (level 0 and the device levels).
+pbig::DenseSrc dense_src(const BinnedState& st) {
+  pbig::DenseSrc d{};
+  for (uint32_t l = 0; l < pbig::kDeepLevels; ++l) {
+    const BinnedWorkspace& w = l ? st.dl[l]->ws : st.ws;
+    d.bbox[l] = w.bbox;
+    d.bid[l] = w.bid;
+    d.bin_start[l] = w.bin_start;
+    d.total[l] = w.total;
+    d.inst[l] = w.inst;
+    d.bin_inst[l] = w.bin_inst;
+  }
+  const BinnedWorkspace& last = st.dl[pbig::kDeepLevels - 1]->ws;
+  d.last_overfull = last.overfull;
+  d.last_meta = last.meta;
+  d.list = st.dense_list;
+  return d;
+}
+
+// The dense kernel's status words and list (once), and for a batch its pair
+// and run buffers at least `pairs` and `chunks` entries.
+int ensure_dense(pbig_hip_ctx* c, BinnedState& st, uint64_t pairs, uint64_t chunks) {
+  if (!st.dense_meta) {
+    HIP_TRY(c, hipMalloc((void**)&st.dense_meta, sizeof(Meta)));
+    HIP_TRY(c, hipMemset(st.dense_meta, 0, sizeof(Meta)));
+  }
+  uint64_t dl = 0;
+  if (!st.dense_list) {
+    int rc;
+    if ((rc = grow(c, &st.dense_list, &dl, pbig::kDenseMaxBins))) return rc;
+  }
+  const uint64_t before_p = st.cap_dense_pairs, before_c = st.cap_dense_chunks;
+  int rc;
+  if (pairs && (rc = grow(c, &st.dense_pairs, &st.cap_dense_pairs, pairs))) return rc;
+  if (chunks && (rc = grow(c, &st.dense_chunks, &st.cap_dense_chunks, chunks))) return rc;
+  if (st.cap_dense_pairs != before_p || st.cap_dense_chunks != before_c) ++st.generation;  // (captured graphs)
+  return PBIG_OK;
+}
+
 // Enqueue level 0 (the two-pass launch, direct output) and the levels, then
-// the result words into d_result (optional).
+// the bins no level resolves, then the result words into d_result (optional).
 int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2* d_out, uint64_t cap,
                  uint64_t* d_result, hipStream_t s) {
   hipEvent_t* ev = next_events(c, kPathDeep);
@@ -819,6 +872,7 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
   st.bp.direct = 1u;  // (dense_exit as the caller set it: a dense instance stops after level 0's bin totals)
   HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, nullptr, st.bp, d_out, cap, nullptr, s, nullptr));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
+  HIP_TRY(c, pbig::launch_dense_clear(st.dense_meta, s));
   const BinnedWorkspace* parent = &st.ws;
   uint32_t parent_bins = st.bp.n_bins;
   pbig::DeepMetas dm{};
@@ -827,14 +881,18 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
     const BinnedState& L = *st.dl[l];
     HIP_TRY(c, pbig::launch_deep_level(*parent, parent_bins, l > 1, L.ws, st.dcaps[l], target, st.inst[0].inv_f,
-                                       nullptr, deep_refine(), (uint32_t)c->n_cu, 1u, d_out, cap, s));
+                                       nullptr, deep_refine(), (uint32_t)c->n_cu, 1u, d_out, cap, l - 1,
+                                       st.dense_list, st.dense_meta, s));
     if (ev) HIP_TRY(c, hipEventRecord(ev[l + 1], s));
     dm.meta[l] = L.ws.meta;
     parent = &L.ws;
     parent_bins = st.dcaps[l].bins;
   }
+  HIP_TRY(c, pbig::launch_dense_bins(dense_src(st), pbig::kDeepLevels - 1, st.dense_meta, (uint32_t)c->n_cu,
+                                     (unsigned long long*)&st.ws.meta->n_pairs, d_out, cap, nullptr, 0, nullptr, 0,
+                                     s));
   dm.levels = pbig::kDeepLevels;
-  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, d_result, s));
+  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, st.dense_meta, d_result, s));
   return PBIG_OK;
 }
 
@@ -846,7 +904,7 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
 // an error.  Grows every level's workspace to what it reports and runs
 // again (a few times at most).
 template <class Enqueue>
-int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, Enqueue enqueue) {
+int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, bool batch, Enqueue enqueue) {
   int rc;
   if (!st.d_res) HIP_TRY(c, hipMalloc((void**)&st.d_res, 2 * sizeof(uint64_t)));
   if (!st.dcaps[1].inst) init_deep_caps(st);
@@ -856,7 +914,15 @@ int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, En
   // The levels' arrays always match their bounds (dcaps, each level's
   // cap_wg and bucket): a later launch trusts the bounds, so every growth
   // below is allocated before the next attempt or the return.
+  uint64_t dense_pairs = std::max<uint64_t>(st.cap_dense_pairs, batch ? 1u << 20 : 0u);
+  uint64_t dense_chunks = std::max<uint64_t>(st.cap_dense_chunks, batch ? 1u << 14 : 0u);
   auto allocate = [&]() -> int {
+    int r0 = ensure_dense(c, st, dense_pairs, dense_chunks);
+    if (r0) {
+      st.deep_ok = false;
+      st.deep_failed = true;
+      return r0;
+    }
     for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
       const int r = ensure_deep_level(c, st, l, bucket[l]);
       if (r) {  // (a level's arrays may be short of its bounds: no launch may take the levels)
@@ -877,6 +943,8 @@ int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, En
     HIP_TRY(c, hipMemcpyAsync(&lm[0], st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
       HIP_TRY(c, hipMemcpyAsync(&lm[l], st.dl[l]->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    Meta dn{};
+    HIP_TRY(c, hipMemcpyAsync(&dn, st.dense_meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     if (res[1] == 0) {
       *m = lm[0];
@@ -892,6 +960,11 @@ int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, En
       return 2;
     }
     bool grown = false;
+    if (dn.error & pbig::kErrWorkspace) {  // the dense kernel's pairs or runs (a batch)
+      dense_pairs = std::max<uint64_t>(dense_pairs, dn.n_pairs + dn.n_pairs / 4 + 4096);
+      dense_chunks = std::max<uint64_t>(dense_chunks, dn.items + dn.items / 4 + 1024);
+      grown = true;
+    }
     // level 0: bucket and pair regions
     if (lm[0].error & pbig::kErrWorkspace) {
       if (lm[0].items > st.ws.cap_wg)
@@ -938,7 +1011,7 @@ int run_levels_sync(pbig_hip_ctx* c, BinnedState& st, hipStream_t s, Meta* m, En
 int run_deep_sync(pbig_hip_ctx* c, const float4* d_boxes, uint2* d_out, uint64_t cap, hipStream_t s, Meta* m) {
   BinnedState& st = c->plain;
   if (c->cfg.host_levels || st.deep_failed || st.bp.n_inst != 1 || st.bp.n_bins == 0) return 1;
-  return run_levels_sync(c, st, s, m,
+  return run_levels_sync(c, st, s, m, false,
                          [&](uint64_t* res) { return enqueue_deep(c, st, d_boxes, d_out, cap, res, s); });
 }
 
@@ -1704,6 +1777,7 @@ int enqueue_batch_levels(pbig_hip_batch* b, const float4* boxes, uint2* out, uin
   int rc;
   if ((rc = grow(c, &b->d_merge, &b->cap_merge, 3 * (uint64_t)n_inst + 1))) return rc;
   HIP_TRY(c, pbig::launch_merge_clear(b->d_merge, n_inst, s));
+  HIP_TRY(c, pbig::launch_dense_clear(st.dense_meta, s));
   st.bp.direct = 0u;
   st.bp.dense_exit = 0u;
   HIP_TRY(c, pbig::launch_binned(st.ws, boxes, nullptr, st.bp, out, cap, nullptr, s, nullptr, false));
@@ -1716,20 +1790,27 @@ int enqueue_batch_levels(pbig_hip_batch* b, const float4* boxes, uint2* out, uin
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l) {
     const BinnedState& L = *st.dl[l];
     HIP_TRY(c, pbig::launch_deep_level(*parent, parent_bins, l > 1, L.ws, st.dcaps[l], target, 0.0f, st.ws.inst,
-                                       deep_refine(), n_cu, 0u, out, cap, s));
+                                       deep_refine(), n_cu, 0u, out, cap, l - 1, st.dense_list, st.dense_meta, s));
     dm.meta[l] = L.ws.meta;
     parent = &L.ws;
     parent_bins = st.dcaps[l].bins;
   }
+  HIP_TRY(c, pbig::launch_dense_bins(dense_src(st), pbig::kDeepLevels - 1, st.dense_meta, n_cu, nullptr, out, cap,
+                                     st.dense_pairs, st.cap_dense_pairs, st.dense_chunks,
+                                     (uint32_t)std::min<uint64_t>(st.cap_dense_chunks, 0xFFFFFFFFull), s));
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
     HIP_TRY(c, pbig::launch_level_count(st.dl[l]->ws, st.dcaps[l].bins, b->d_merge, s));
+  const uint32_t cap_chunks = (uint32_t)std::min<uint64_t>(st.cap_dense_chunks, 0xFFFFFFFFull);
+  HIP_TRY(c, pbig::launch_dense_count(st.dense_chunks, cap_chunks, st.dense_meta, b->d_merge, n_inst, s));
   HIP_TRY(c, pbig::launch_merge_scan(b->d_merge, n_inst, s));
   HIP_TRY(c, pbig::launch_binned_output(st.ws, st.bp, out, cap, d_inst_out, s, b->d_merge));
   for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
     HIP_TRY(c, pbig::launch_level_copy(st.dl[l]->ws, st.dcaps[l].bins, n_cu, st.ws, st.ws.bin_out, b->d_merge, n_inst,
                                        out, cap, s));
+  HIP_TRY(c, pbig::launch_dense_copy(st.dense_chunks, cap_chunks, st.dense_meta, st.dense_pairs, st.ws, b->d_merge,
+                                     n_inst, out, cap, s));
   dm.levels = pbig::kDeepLevels;
-  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, d_result, s));
+  if (d_result) HIP_TRY(c, pbig::launch_deep_result(dm, st.dense_meta, d_result, s));
   return PBIG_OK;
 }
 
@@ -1739,7 +1820,7 @@ int enqueue_batch_levels(pbig_hip_batch* b, const float4* boxes, uint2* out, uin
 int run_batch_levels_sync(pbig_hip_batch* b, const float4* boxes, uint2* out, uint64_t cap, hipStream_t s,
                           Meta* m) {
   if (!batch_levels_apply(b)) return 1;
-  return run_levels_sync(b->ctx, b->st, s, m, [&](uint64_t* res) {
+  return run_levels_sync(b->ctx, b->st, s, m, true, [&](uint64_t* res) {
     return enqueue_batch_levels(b, boxes, out, cap, b->d_inst_out, res, s);
   });
 }
@@ -1981,11 +2062,12 @@ int pbig_hip_batch_unresolved(pbig_hip_batch* b, uint8_t* h_flags, uint32_t* n_f
     std::vector<Meta> lm(pbig::kDeepLevels);
     for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
       HIP_TRY(c, hipMemcpyAsync(&lm[l], st.dl[l]->ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    Meta dn{};
+    HIP_TRY(c, hipMemcpyAsync(&dn, st.dense_meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    bool complete = true;
+    bool complete = dn.error == 0;  // (the dense kernel took the bins the levels left)
     for (uint32_t l = 1; l < pbig::kDeepLevels; ++l)
       complete = complete && lm[l].plan_err == 0 && !(lm[l].error & ~(unsigned long long)pbig::kErrOverfull);
-    complete = complete && !(lm[pbig::kDeepLevels - 1].error & pbig::kErrOverfull);
     if (complete) return PBIG_OK;
   }
   uint32_t nf = 0;

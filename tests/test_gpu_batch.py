@@ -169,9 +169,13 @@ def test_batch_edge_instances(torch_cuda, eng):
         check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
 
 
-def test_batch_crowded_instance_falls_back(torch_cuda, eng):
-    """A 6000-box stack in one fine cell overflows LDS: that instance runs on
-    the general path and its pairs land in its slot between the others."""
+@pytest.mark.parametrize("host", [True, False])
+def test_batch_crowded_instance(torch_cuda, eng, host):
+    """A 6000-box stack in one fine cell overflows LDS at every refinement.
+    With host re-binning that instance runs on the general path; with device
+    levels its bin is tested whole on the device, and the batch's later
+    launches complete there.  Its pairs land in its slot between the
+    others."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -183,16 +187,25 @@ def test_batch_crowded_instance_falls_back(torch_cuda, eng):
     total = sum(w.shape[0] for w in want)
     d = to_dev(torch_cuda, boxes)
     pairs = torch_cuda.empty((total, 2), dtype=torch_cuda.int32, device="cuda:0")
-    with pbig_amd.Batch(eng, offsets, ranges, 16.0) as b:
-        n, io = b.check(d, pairs)
-        assert n == total
-        assert [int(c) for _, c in io] == [w.shape[0] for w in want]
-        check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
-        # the asynchronous launch reports the overflow instead
-        res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
-        b.launch(d, pairs, None, res)
-        torch_cuda.cuda.synchronize()
-        assert res[1].item() & 8
+    eng.configure(pbig_amd.PATH_AUTO, host_levels=host)
+    try:
+        with pbig_amd.Batch(eng, offsets, ranges, 16.0) as b:
+            n, io = b.check(d, pairs)
+            assert n == total
+            assert [int(c) for _, c in io] == [w.shape[0] for w in want]
+            check_all(split(pairs[:n].cpu().numpy().view(np.uint32), io), want)
+            res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+            dio = torch_cuda.zeros((3, 2), dtype=torch_cuda.int64, device="cuda:0")
+            pairs.fill_(-1)
+            b.launch(d, pairs, dio, res)
+            torch_cuda.cuda.synchronize()
+            if host:  # the asynchronous launch reports the overflow instead
+                assert res[1].item() & 8
+            else:
+                assert res.tolist() == [total, 0]
+                check_all(split(pairs[:n].cpu().numpy().view(np.uint32), dio.cpu().numpy()), want)
+    finally:
+        eng.configure(pbig_amd.PATH_AUTO)
 
 
 def test_batch_overflow_reports_counts(torch_cuda, eng):

@@ -473,15 +473,15 @@ def test_auto_keeps_large_narrow_bins_binned(torch_cuda, orc, manifest):
 
 
 @pytest.mark.parametrize("levels_min", [0, 1])
-def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc, levels_min):
+def test_unsplittable_bin(torch_cuda, orc, levels_min):
     """6,000 identical boxes among 20,000 uniform ones share one fine cell at
     every refinement; their bin is beyond LDS but holds under a quarter of
     the memberships.  At the default levels_min_boxes (2^20) the automatic
     choice sends this 26,000-box instance to the general path; at 1 the
-    device re-binning levels cannot split the bin and report it, and the
-    synchronous call falls back (host re-binning, then the general path).
-    Either way the set is exact, and the asynchronous call after it follows
-    the path that worked."""
+    device levels cannot split the bin, so it is tested whole on the device
+    (every pair of its memberships under the reference-cell rule) and the
+    call stays on the binned path.  Either way the set is exact, and the
+    asynchronous call after it follows the path that worked."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -496,7 +496,7 @@ def test_device_levels_fall_back_on_unsplittable_bin(torch_cuda, orc, levels_min
         e.configure(PATHS["auto"], 2000, levels_min)
         n = e.check_device(d, 4096.0, 8.0, out)
         assert n == len(want)
-        assert e.stats()["path"] == PATHS["general"]
+        assert e.stats()["path"] == (PATHS["binned"] if levels_min == 1 else PATHS["general"])
         assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
         e.check_device_async(d, 4096.0, 8.0, out, res)
         torch_cuda.cuda.synchronize()
