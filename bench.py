@@ -378,9 +378,11 @@ def serving_leg(torch, count, rate, budget_gb):
     its pairs stay in HBM for a consumer on the GPU) and "host" (boxes
     uploaded from pinned host memory, pairs copied back to pinned host
     memory).  In both, the consumer reads the pairs in place and keeps their
-    count.  Capacity: every request arrives at once (median of three runs by
-    makespan).  Latency: arrivals at the stated rate (after one warm-up run,
-    the median of three runs by p99)."""
+    count.  One server per setting serves every run, as a long-running
+    server would (its slots' workspaces and device levels settle in the
+    warm-up run).  Capacity: every request arrives at once (median of three
+    runs by makespan).  Latency: arrivals at the stated rate (after one
+    warm-up run, the median of three runs by p99)."""
     import numpy as np
 
     import pbig_amd
@@ -411,14 +413,12 @@ def serving_leg(torch, count, rate, budget_gb):
             r.arrival = q["t"]
             r.pairs = None
 
+    srv = None  # one server per setting serves every run, as a long-running one would
+
     def server(mode, realtime, consumer=count_only):
         reset()
-        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, table=table, results=mode,
-                     consumer=consumer)
-        try:
-            return srv.run(reqs, realtime=realtime)
-        finally:
-            srv.close()
+        srv.consumer = consumer
+        return srv.run(reqs, realtime=realtime)
 
     def one(mode, realtime, consumer=count_only):
         reset()
@@ -429,7 +429,8 @@ def serving_leg(torch, count, rate, budget_gb):
         if mode == "host":
             for r in reqs:
                 r.device = None
-        server(mode, False)  # warm-up: plans, workspaces, pinned buffers
+        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, table=table, results=mode)
+        server(mode, False)  # warm-up: plans, workspaces, pinned buffers, the slots' device levels
         one(mode, False)
 
         def median_run(fn):  # a makespan of tens of ms: the median of three runs
@@ -473,7 +474,9 @@ def serving_leg(torch, count, rate, budget_gb):
             for r in reqs:
                 r.pairs = None
             mix = None
-            torch.cuda.empty_cache()
+        srv.close()
+        srv = None
+        torch.cuda.empty_cache()
     eng.close()
     return res
 

@@ -21,11 +21,12 @@ per launch:
   and go into the next free slot, so uploads, the kernels of several batches
   and result copies overlap on the device.  A batch of one request (light
   load) runs on the plain path instead, on the slot's own context: one
-  launch, no batch plan, and a dense request goes to the general path after
-  the bin totals.
+  launch, no batch plan, reading the request's own device array when it
+  handed one over; a request whose bins show mostly dense clusters is
+  launched again, asynchronously, on the general path.
 * Completion work off the loop.  A launch whose result word asks for host
-  work (bins beyond LDS, a dense plain request, regrowth) is finished by a
-  synchronous call in a worker thread; every slot has its own context, so
+  work (bins beyond LDS, regrowth) is finished by a synchronous call in a
+  worker thread; every slot has its own context, so
   the loop keeps admitting, launching and delivering for the other slots
   meanwhile.  A batch's bins beyond LDS are re-binned by levels planned on
   the device; the first such finish settles the slot's batch on them, and
@@ -60,6 +61,8 @@ import pbig_amd
 from pbig_amd import autotune
 
 _ERR_OVERFULL = 8  # error word: a coarse bin did not fit in LDS (re-binned on the host)
+_ERR_DENSE = 16    # error word: the instance is mostly dense clusters (the general path takes it)
+_ERR_BINCAP = 32   # error word: a bin outgrew its share of the single-pass launch
 
 
 @dataclass
@@ -164,6 +167,8 @@ class _Slot:
     reqs: list = field(default_factory=list)
     bid: int = -1
     total: int = 0
+    src: object = None               # a plain launch's boxes (the request's own device array, or its pool run)
+    general: bool = False            # a plain launch relaunched on the general path
 
 
 class Server:
@@ -302,11 +307,15 @@ class Server:
         s.mode = "plain" if len(reqs) == 1 and reqs[0].n > 0 else "batch"
         if s.mode == "plain":
             self._configure(s.eng, pbig_amd.PATH_AUTO)  # no memory of earlier requests
-            if self.upload_ev is not None:
-                s.stream.wait_event(self.upload_ev)
             r = reqs[0]
-            s.eng.check_device_async(self.d_pool[r._off:r._off + r.n], r.L, r.cell_size, s.pairs, s.res,
-                                     stream=s.stream)
+            s.general = False
+            if r.device is not None:  # the request's own device array: no wait for its pool copy
+                s.src = r.device
+            else:
+                if self.upload_ev is not None:
+                    s.stream.wait_event(self.upload_ev)
+                s.src = self.d_pool[r._off:r._off + r.n]
+            s.eng.check_device_async(s.src, r.L, r.cell_size, s.pairs, s.res, stream=s.stream)
         else:
             if s.batch is None:
                 s.batch = pbig_amd.Batch.from_instances(s.eng, starts, counts, Ls, cells)
@@ -358,7 +367,7 @@ class Server:
         torch = self.torch
         if s.mode == "plain":
             r = s.reqs[0]
-            view = self.d_pool[r._off:r._off + r.n]
+            view = s.src
             if err & 16:
                 self._configure(s.eng, pbig_amd.PATH_GENERAL)
             try:
@@ -378,6 +387,17 @@ class Server:
             total, io = s.batch.check(self.d_pool, s.pairs, stream=s.stream)
         s.h_io[:len(s.reqs)].copy_(torch.from_numpy(io.astype(np.int64)))
         return total
+
+    def _relaunch_general(self, s: _Slot):
+        torch = self.torch
+        r = s.reqs[0]
+        s.general = True
+        self._configure(s.eng, pbig_amd.PATH_GENERAL)
+        s.eng.check_device_async(s.src, r.L, r.cell_size, s.pairs, s.res, stream=s.stream)
+        with torch.cuda.stream(s.stream):
+            s.h_res.copy_(s.res, non_blocking=True)
+        s.ev = torch.cuda.Event()
+        s.ev.record(s.stream)
 
     def _light(self) -> bool:
         """Light load: no more requests waiting than there are slots."""
@@ -427,6 +447,10 @@ class Server:
             if (err == _ERR_OVERFULL and s.mode == "batch" and len(s.reqs) > 1 and
                     total <= s.pairs.shape[0] and self._light() and self._split(s)):
                 return self._results_ready(s, total, now)
+            if (s.mode == "plain" and not s.general and err & _ERR_DENSE and not err & ~(_ERR_DENSE | _ERR_BINCAP)):
+                # mostly dense clusters: the general path, asynchronously again
+                self._relaunch_general(s)
+                return True
             if err != 0 or total > s.pairs.shape[0]:
                 s.future = self.pool_exec.submit(self._finish, s, err)
                 s.state = "finishing"
@@ -467,6 +491,7 @@ class Server:
         the start; with realtime=False all arrive at once); returns
         throughput and latency figures."""
         order = sorted(requests, key=lambda r: r.arrival)
+        self.done, self.batch_sizes, self.n_batches, self.n_requeued = [], [], 0, 0  # (this run's figures)
         t0 = time.perf_counter()
         k = 0
         while k < len(order) or self.busy():
