@@ -154,12 +154,28 @@ int pbig_hip_reserve(pbig_hip_ctx* ctx, uint64_t n_boxes, uint64_t n_cells, uint
 
 /* Plain path, host boxes in the reference's record layout: n x {xmin, xmax,
  * ymin, ymax, rid} (20 bytes, RectInfo of src/pbig/rect.h:15-18 with
- * CType = float).  rid must equal the index.  Copies in, runs, copies the
- * pairs out to h_pairs (pair_capacity x {lo, hi}).  This is the host half of
+ * CType = float).  rid must equal the index.  Packs the records into pinned
+ * memory over the context's host threads (at most 16), copies in, runs, and
+ * copies the pairs back in chunks of 8 Mi pairs through two pinned buffers,
+ * each chunk spread into h_pairs (pair_capacity x {lo, hi}) by the host
+ * threads while the next one is in flight.  This is the host half of
  * pbig_check (pbig.cu:1401 copy-in, 1850-1909 copy-out). */
 int pbig_hip_check_host(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, double range,
                         float cell_size, uint32_t* h_pairs, uint64_t pair_capacity,
                         uint64_t* n_pairs);
+
+/* The pairs of a host check handed over chunk by chunk: pairs points at n
+ * {lo, hi} uint32 in the library's pinned memory, valid during the call. */
+typedef void (*pbig_hip_pairs_fn)(void* user, const uint32_t* pairs, uint64_t n);
+
+/* pbig_hip_check_host with the pairs handed to fn(user, chunk, n) chunk by
+ * chunk, on the calling thread, while the next chunk is copied: no buffer
+ * for all pairs.  The reference's match phase prefetches results on one
+ * stream while the host decodes the previous ones into the report
+ * (pbig.cu:1848-1866, decode.cc:26-138); pbig_check's report callbacks run
+ * here.  *n_pairs = the total. */
+int pbig_hip_check_host_each(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, double range,
+                             float cell_size, pbig_hip_pairs_fn fn, void* user, uint64_t* n_pairs);
 
 int pbig_hip_last_stats(const pbig_hip_ctx* ctx, pbig_hip_stats* out);
 
@@ -297,9 +313,13 @@ int pbig_hip_check_u32(pbig_hip_ctx* ctx, const uint32_t* d_boxes, uint32_t n, i
                        uint32_t cell_size, uint32_t* d_pairs, uint64_t pair_capacity, uint64_t* n_pairs,
                        void* stream);
 /* Host records: n x RectInfo {CoordInfo (2*dims uint32), rid} with rid ==
- * index, as pbig_check receives them in the reference's build. */
+ * index, as pbig_check receives them in the reference's build (packing and
+ * the copy back as for pbig_hip_check_host). */
 int pbig_hip_check_host_u32(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, int dims, uint32_t range,
                             uint32_t cell_size, uint32_t* h_pairs, uint64_t pair_capacity, uint64_t* n_pairs);
+/* The same, the pairs handed to fn chunk by chunk (pbig_hip_check_host_each). */
+int pbig_hip_check_host_u32_each(pbig_hip_ctx* ctx, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                                 uint32_t cell_size, pbig_hip_pairs_fn fn, void* user, uint64_t* n_pairs);
 
 /* ------------------------------------------------------------------------
  * Compressed pair stream in the reference's record layout (its default

@@ -38,6 +38,7 @@ EXPORTS = (
     "pbig_hip_check_device_noalloc",
     "pbig_hip_reserve",
     "pbig_hip_check_host",
+    "pbig_hip_check_host_each",
     "pbig_hip_last_stats",
     "pbig_hip_timing_start",
     "pbig_hip_timing_read",
@@ -55,9 +56,13 @@ EXPORTS = (
     "pbig_hip_reference_cell_size_u32",
     "pbig_hip_check_u32",
     "pbig_hip_check_host_u32",
+    "pbig_hip_check_host_u32_each",
     "pbig_hip_rice_encode",
     "pbig_hip_rice_decode",
 )
+
+# pbig_hip_pairs_fn: (user, pairs {lo, hi} uint32 x n, n), pairs valid during the call.
+PAIRS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64)
 
 PATH_AUTO, PATH_GENERAL, PATH_BINNED = 0, 1, 2
 # pbig_hip_timing_read also reports the binned path's single-pass launch of
@@ -150,6 +155,7 @@ def load_library(path: str | None = None):
         "pbig_hip_check_device_noalloc": (i32, [P, P, u32, f64, f32, P, u64, P, P]),
         "pbig_hip_reserve": (i32, [P, u64, u64, u64]),
         "pbig_hip_check_host": (i32, [P, P, u32, f64, f32, P, u64, ctypes.POINTER(u64)]),
+        "pbig_hip_check_host_each": (i32, [P, P, u32, f64, f32, PAIRS_FN, P, ctypes.POINTER(u64)]),
         "pbig_hip_last_stats": (i32, [P, ctypes.POINTER(Stats)]),
         "pbig_hip_timing_start": (i32, [P, i32]),
         "pbig_hip_timing_read": (i32, [P, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i32),
@@ -168,6 +174,7 @@ def load_library(path: str | None = None):
         "pbig_hip_reference_cell_size_u32": (u32, [P, u32, i32, u32]),
         "pbig_hip_check_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64), P]),
         "pbig_hip_check_host_u32": (i32, [P, P, u32, i32, u32, u32, P, u64, ctypes.POINTER(u64)]),
+        "pbig_hip_check_host_u32_each": (i32, [P, P, u32, i32, u32, u32, PAIRS_FN, P, ctypes.POINTER(u64)]),
         "pbig_hip_rice_encode": (i32, [P, u64, u32, P, u64, ctypes.POINTER(u64), P, u64, ctypes.POINTER(u64), P,
                                        u64, ctypes.POINTER(u64)]),
         "pbig_hip_rice_decode": (i32, [P, u64, P, u64, P, u64, P, u64, ctypes.POINTER(u64)]),
@@ -421,6 +428,43 @@ class Engine:
             if rc != PBIG_OK:
                 self._raise(rc)
             return out[: int(cnt.value)]
+
+    def _each(self, call, fn) -> int:
+        errors = []
+
+        def chunk(user, ptr, m):
+            try:
+                fn(np.ctypeslib.as_array(ptr, shape=(int(m), 2)) if m else np.zeros((0, 2), np.uint32))
+            except BaseException as e:  # (ctypes drops exceptions raised in callbacks)
+                errors.append(e)
+
+        cb = PAIRS_FN(chunk)
+        cnt = ctypes.c_uint64(0)
+        rc = call(cb, ctypes.byref(cnt))
+        if errors:
+            raise errors[0]
+        if rc != PBIG_OK:
+            self._raise(rc)
+        return int(cnt.value)
+
+    def check_host_each(self, rects, range_: float, cell_size: float, fn) -> int:
+        """pbig_hip_check_host_each: fn(pairs) on every chunk of the pairs, an
+        (m, 2) uint32 view of the library's pinned memory valid during the
+        call, while the next chunk is copied back.  Returns the total."""
+        rects = as_rects(rects)
+        n = rects.shape[0]
+        return self._each(lambda cb, cnt: self._lib.pbig_hip_check_host_each(
+            self._h, rects.ctypes.data if n else None, n, float(range_), float(cell_size), cb, None, cnt), fn)
+
+    def check_host_u32_each(self, boxes: np.ndarray, dims: int, range_: int, cell_size: int, fn) -> int:
+        """pbig_hip_check_host_u32_each on (n, 2*dims) uint32 boxes (rid = index)."""
+        b = np.ascontiguousarray(boxes, dtype=np.uint32).reshape(-1, 2 * dims)
+        n = b.shape[0]
+        rec = np.empty((n, 2 * dims + 1), dtype=np.uint32)
+        rec[:, :2 * dims] = b
+        rec[:, 2 * dims] = np.arange(n, dtype=np.uint32)
+        return self._each(lambda cb, cnt: self._lib.pbig_hip_check_host_u32_each(
+            self._h, rec.ctypes.data if n else None, n, int(dims), int(range_), int(cell_size), cb, None, cnt), fn)
 
 
 def as_rects(boxes) -> np.ndarray:

@@ -21,8 +21,7 @@ pbig_hip_ctx* g_ctx = nullptr;
 unsigned int g_n = 0;
 unsigned int g_range = 0;
 unsigned int g_cs = 0;
-std::vector<unsigned int> g_pairs;
-unsigned long long g_last_pairs = 0;  // pairs of the last pbig_check (in g_pairs)
+unsigned long long g_last_pairs = 0;  // pairs of the last pbig_check
 
 [[noreturn]] void die(const char* what, int rc) {
   printf("ERROR:%s failed (status %d): %s\n", what, rc, g_ctx ? pbig_hip_last_error(g_ctx) : "no context");
@@ -62,30 +61,34 @@ void pbig_init(unsigned int num, unsigned int range, unsigned int cs, unsigned i
   if (rc) die("pbig_init", rc);
 }
 
+// One host entry point per build of rect.h (the native float 2-D records, or
+// the reference's unsigned records, DIMS 2 or 3): the pairs come back chunk by
+// chunk to fn while the next chunk is copied.
+int check_each(RectInfo* input, pbig_hip_pairs_fn fn, void* user, unsigned long long* np) {
+#if defined(PBIG_COMPAT_U32)
+  return pbig_hip_check_host_u32_each(g_ctx, input, g_n, DIMS, g_range, g_cs, fn, user, (uint64_t*)np);
+#else
+  return pbig_hip_check_host_each(g_ctx, input, g_n, (double)g_range, (float)g_cs, fn, user, (uint64_t*)np);
+#endif
+}
+
 void pbig_check(RectInfo* input, report* r) {
   if (!g_ctx) {
     printf("ERROR:pbig_check called before pbig_init\n");
     exit(-1);
   }
-  unsigned long long cap = g_pairs.size() / 2, np = 0;
-  // One entry point per build of rect.h: the native float 2-D records, or the
-  // reference's unsigned records (DIMS 2 or 3).
-  auto run = [&](unsigned int* out, unsigned long long c) {
-#if defined(PBIG_COMPAT_U32)
-    return pbig_hip_check_host_u32(g_ctx, input, g_n, DIMS, g_range, g_cs, out, c, (uint64_t*)&np);
-#else
-    return pbig_hip_check_host(g_ctx, input, g_n, (double)g_range, (float)g_cs, out, c, (uint64_t*)&np);
-#endif
-  };
-  int rc = run(g_pairs.empty() ? nullptr : g_pairs.data(), cap);
-  if (rc == PBIG_E_OVERFLOW) {
-    g_pairs.resize(2 * (np + np / 8 + 1024));
-    cap = g_pairs.size() / 2;
-    rc = run(g_pairs.data(), cap);
-  }
+  // the report's callbacks run on each chunk of pairs while the next one is
+  // copied back (the reference decodes into the report during its prefetch)
+  unsigned long long np = 0;
+  const int rc = check_each(
+      input,
+      [](void* user, const uint32_t* p, uint64_t n) {
+        report& rep = *static_cast<report*>(user);
+        for (uint64_t k = 0; k < n; ++k) rep(p[2 * k], p[2 * k + 1]);
+      },
+      r, &np);
   if (rc) die("pbig_check", rc);
   g_last_pairs = np;
-  for (unsigned long long k = 0; k < np; ++k) (*r)(g_pairs[2 * k], g_pairs[2 * k + 1]);
 }
 
 void pbig_check_stream(RectInfo* input, std::vector<unsigned long long>* words, std::vector<unsigned int>* cell_start,
@@ -93,9 +96,16 @@ void pbig_check_stream(RectInfo* input, std::vector<unsigned long long>* words, 
   // the pairs of the check, then the encoder (sizes first)
   std::vector<unsigned int> flat;
   {
-    report r;
-    pbig_check(input, &r);
-    flat.assign(g_pairs.begin(), g_pairs.begin() + 2 * g_last_pairs);
+    unsigned long long np = 0;
+    const int rc = check_each(
+        input,
+        [](void* user, const uint32_t* p, uint64_t n) {
+          auto& v = *static_cast<std::vector<unsigned int>*>(user);
+          v.insert(v.end(), p, p + 2 * n);
+        },
+        &flat, &np);
+    if (rc) die("pbig_check_stream", rc);
+    g_last_pairs = np;
   }
   uint64_t nw = 0, nc = 0, nr = 0;
   const uint64_t np = flat.size() / 2;
@@ -126,5 +136,4 @@ void pbig_decode_stream(const std::vector<unsigned long long>& words, const std:
 void pbig_finalize() {
   if (g_ctx) pbig_hip_destroy(g_ctx);
   g_ctx = nullptr;
-  std::vector<unsigned int>().swap(g_pairs);
 }

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pbig_binned.h"
+#include "pbig_host.h"
 #include "pbig_device.h"
 #include "pbig_hip.h"
 #include "pbig_u32.h"
@@ -109,6 +110,14 @@ struct pbig_hip_ctx {
   pbig::U32Workspace uw;
   uint32_t* d_in32 = nullptr;
   uint64_t cap_in32 = 0;
+  uint32_t* h_in32 = nullptr;      // pinned
+  uint64_t cap_h_in32 = 0;
+  // Host entry points: threads for packing and copies, and two pinned
+  // buffers the pairs come back through, chunk by chunk.
+  std::unique_ptr<pbig::HostPool> pool;
+  uint2* h_out[2] = {nullptr, nullptr};
+  uint64_t cap_h_out = 0;
+  hipEvent_t out_ev[2] = {nullptr, nullptr};
   // Pinned ring for the plain path's instance record: uploads stay ordered
   // on the stream without a host synchronisation.
   InstDesc* h_ring = nullptr;
@@ -1300,6 +1309,110 @@ void set_cell_stats(pbig_hip_ctx* c, float cell) {
   c->stats.cell_size_den = den;
 }
 
+// ---------------------------------------------------------------------------
+// Host entry points: packing and the copy of the pairs back.
+
+// The context's host threads (created on first use, at most 16).
+pbig::HostPool& host_pool(pbig_hip_ctx* c) {
+  if (!c->pool) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    c->pool = std::make_unique<pbig::HostPool>(hw ? std::min(hw, 16u) : 4u);
+  }
+  return *c->pool;
+}
+
+// Runs work(t) over the pool's threads, or on this thread alone for small jobs.
+unsigned host_parts(pbig_hip_ctx* c, uint64_t n) { return n >= (1u << 16) ? host_pool(c).size() : 1u; }
+void host_run(pbig_hip_ctx* c, unsigned parts, const std::function<void(unsigned)>& work) {
+  if (parts == 1) work(0);
+  else host_pool(c).run(work);
+}
+
+constexpr uint64_t kOutChunk = 8ull << 20;  // pairs per chunk of the copy back (64 MB)
+
+// np pairs of d_pairs back to the host in chunks through two pinned buffers:
+// consume(chunk, count, first) runs on each chunk while the next one is in
+// flight (the reference's match phase overlaps its prefetch of results with
+// the decode of the previous ones the same way, src/pbig/pbig.cu:1848-1866).
+// consume returns a status; anything but PBIG_OK stops the copy.
+template <class Consume>
+int stream_pairs(pbig_hip_ctx* c, const uint2* d_pairs, uint64_t np, hipStream_t s, Consume consume) {
+  if (np == 0) return PBIG_OK;
+  const uint64_t want = std::min<uint64_t>(np, kOutChunk);
+  if (c->cap_h_out < want) {
+    for (auto& q : c->h_out) {
+      if (q) HIP_TRY(c, hipHostFree(q));
+      q = nullptr;
+    }
+    c->cap_h_out = 0;
+    for (auto& q : c->h_out) HIP_TRY(c, hipHostMalloc((void**)&q, want * sizeof(uint2), hipHostMallocDefault));
+    c->cap_h_out = want;
+  }
+  for (auto& e : c->out_ev)
+    if (!e) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const uint64_t chunk = c->cap_h_out, nch = (np + chunk - 1) / chunk;
+  auto issue = [&](uint64_t i) -> int {
+    const uint64_t first = i * chunk, len = std::min(chunk, np - first);
+    HIP_TRY(c, hipMemcpyAsync(c->h_out[i & 1], d_pairs + first, len * sizeof(uint2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipEventRecord(c->out_ev[i & 1], s));
+    return PBIG_OK;
+  };
+  int rc;
+  if ((rc = issue(0))) return rc;
+  for (uint64_t i = 0; i < nch; ++i) {
+    // the next chunk goes to the other buffer, consumed one iteration ago
+    if (i + 1 < nch && (rc = issue(i + 1))) return rc;
+    HIP_TRY(c, hipEventSynchronize(c->out_ev[i & 1]));
+    const uint64_t first = i * chunk, len = std::min(chunk, np - first);
+    if ((rc = consume(c->h_out[i & 1], len, first))) {
+      HIP_TRY(c, hipStreamSynchronize(s));  // (the copy in flight lands before its buffer is reused)
+      return rc;
+    }
+  }
+  return PBIG_OK;
+}
+
+// A chunk of pairs into the caller's memory, split over the host threads.
+int copy_out(pbig_hip_ctx* c, const uint2* src, uint64_t len, uint32_t* dst) {
+  const unsigned parts = host_parts(c, len);
+  host_run(c, parts, [&](unsigned t) {
+    uint64_t lo, hi;
+    pbig::HostPool::part(len, parts, t, &lo, &hi);
+    memcpy(dst + 2 * lo, src + lo, (hi - lo) * sizeof(uint2));
+  });
+  return PBIG_OK;
+}
+
+// RectInfo records -> the pinned staging of the boxes, over the host threads.
+// check(i, record) returns 0 or a status for record i; each part stops at its
+// first bad record, so the lowest part with one holds the first bad record
+// of the whole array, as a loop in order would report it.
+template <class Pack>
+int pack_records(pbig_hip_ctx* c, uint32_t n, Pack pack, uint64_t* bad_index, int* bad_code) {
+  const unsigned parts = host_parts(c, n);
+  std::vector<uint64_t> bad(parts, ~0ull);
+  std::vector<int> code(parts, 0);
+  host_run(c, parts, [&](unsigned t) {
+    uint64_t lo, hi;
+    pbig::HostPool::part(n, parts, t, &lo, &hi);
+    for (uint64_t i = lo; i < hi; ++i) {
+      const int e = pack((uint32_t)i);
+      if (e) {
+        bad[t] = i;
+        code[t] = e;
+        return;
+      }
+    }
+  });
+  for (unsigned t = 0; t < parts; ++t)
+    if (code[t]) {
+      *bad_index = bad[t];
+      *bad_code = code[t];
+      return code[t];
+    }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1355,6 +1468,11 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
   free_state(c->plain);
   for (void* p : {(void*)c->uw.mxy, (void*)c->uw.mz, (void*)c->uw.mid, (void*)c->uw.mcell, (void*)c->d_in32})
     if (p) (void)hipFree(p);
+  for (void* p : {(void*)c->h_in32, (void*)c->h_out[0], (void*)c->h_out[1]})
+    if (p) (void)hipHostFree(p);
+  for (hipEvent_t e : c->out_ev)
+    if (e) (void)hipEventDestroy(e);
+  c->pool.reset();
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ring_ev)
     if (e) (void)hipEventDestroy(e);
@@ -1579,17 +1697,18 @@ int pbig_hip_check_device_noalloc(pbig_hip_ctx* c, const float* d_boxes, uint32_
   return PBIG_OK;
 }
 
-int pbig_hip_check_host(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double range, float cell_size,
-                        uint32_t* h_pairs, uint64_t cap, uint64_t* n_pairs) {
+// The host entry points' device part (float records): pack RectInfo
+// {xmin,xmax,ymin,ymax,rid} into 16-byte boxes in pinned memory over the host
+// threads, checking the reference's contract rid == index and the boxes'
+// orientation, upload, and check; the pairs are then in c->d_pairs.
+static int host_check_float(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double range, float cell_size,
+                            uint64_t* n_pairs) {
   if (!c) return PBIG_E_ARG;
   if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
   if (n > pbig::kMaxBoxes) return fail(c, PBIG_E_ARG, "n = %u exceeds the limit of 2^30-1 boxes", n);
   if (n && !h_rects) return fail(c, PBIG_E_ARG, "h_rects is null");
-  if (cap && !h_pairs) return fail(c, PBIG_E_ARG, "h_pairs is null with a nonzero capacity");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  // Pack RectInfo {xmin,xmax,ymin,ymax,rid} into 16-byte boxes, checking the
-  // reference's contract rid == index and the box orientation.
   if (c->cap_h_in < n || !c->h_in) {
     if (c->h_in) HIP_TRY(c, hipHostFree(c->h_in));
     c->h_in = nullptr;
@@ -1598,15 +1717,30 @@ int pbig_hip_check_host(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double
     c->cap_h_in = std::max<uint64_t>(n, 1);
   }
   const unsigned char* src = (const unsigned char*)h_rects;
-  for (uint32_t i = 0; i < n; ++i) {
-    float f[4];
+  float4* h_in = c->h_in;
+  uint64_t bad = 0;
+  int code = 0;
+  if (pack_records(
+          c, n,
+          [&](uint32_t i) {
+            float f[4];
+            uint32_t rid;
+            memcpy(f, src + (size_t)i * 20, 16);
+            memcpy(&rid, src + (size_t)i * 20 + 16, 4);
+            if (rid != i) return 1;
+            if (!(f[0] <= f[1]) || !(f[2] <= f[3])) return 2;
+            h_in[i] = make_float4(f[0], f[1], f[2], f[3]);
+            return 0;
+          },
+          &bad, &code)) {
     uint32_t rid;
-    memcpy(f, src + (size_t)i * 20, 16);
-    memcpy(&rid, src + (size_t)i * 20 + 16, 4);
-    if (rid != i) return fail(c, PBIG_E_ARG, "rect %u has rid %u; rid must equal the index", i, rid);
-    if (!(f[0] <= f[1]) || !(f[2] <= f[3]))
-      return fail(c, PBIG_E_INVALID_BOX, "rect %u is inverted or NaN: [%g,%g]x[%g,%g]", i, f[0], f[1], f[2], f[3]);
-    c->h_in[i] = make_float4(f[0], f[1], f[2], f[3]);
+    float f[4];
+    memcpy(f, src + bad * 20, 16);
+    memcpy(&rid, src + bad * 20 + 16, 4);
+    if (code == 1)
+      return fail(c, PBIG_E_ARG, "rect %llu has rid %u; rid must equal the index", (unsigned long long)bad, rid);
+    return fail(c, PBIG_E_INVALID_BOX, "rect %llu is inverted or NaN: [%g,%g]x[%g,%g]", (unsigned long long)bad, f[0],
+                f[1], f[2], f[3]);
   }
   int rc;
   if ((rc = grow(c, &c->d_in, &c->cap_in, n))) return rc;
@@ -1626,15 +1760,35 @@ int pbig_hip_check_host(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double
   set_cell_stats(c, cell_size);
   if (rc) return rc;
   *n_pairs = np;
-  uint64_t ncopy = std::min<uint64_t>(np, cap);
-  if (ncopy) {
-    HIP_TRY(c, hipMemcpyAsync(h_pairs, c->d_pairs, ncopy * sizeof(uint2), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-  }
+  return PBIG_OK;
+}
+
+int pbig_hip_check_host(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double range, float cell_size,
+                        uint32_t* h_pairs, uint64_t cap, uint64_t* n_pairs) {
+  if (c && cap && !h_pairs) return fail(c, PBIG_E_ARG, "h_pairs is null with a nonzero capacity");
+  int rc;
+  if ((rc = host_check_float(c, h_rects, n, range, cell_size, n_pairs))) return rc;
+  const uint64_t np = *n_pairs;
+  if ((rc = stream_pairs(c, c->d_pairs, std::min<uint64_t>(np, cap), c->stream,
+                         [&](const uint2* chunk, uint64_t len, uint64_t first) {
+                           return copy_out(c, chunk, len, h_pairs + 2 * first);
+                         })))
+    return rc;
   if (np > cap)
     return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)np,
                 (unsigned long long)cap);
   return PBIG_OK;
+}
+
+int pbig_hip_check_host_each(pbig_hip_ctx* c, const void* h_rects, uint32_t n, double range, float cell_size,
+                             pbig_hip_pairs_fn fn, void* user, uint64_t* n_pairs) {
+  if (c && !fn) return fail(c, PBIG_E_ARG, "fn is null");
+  int rc;
+  if ((rc = host_check_float(c, h_rects, n, range, cell_size, n_pairs))) return rc;
+  return stream_pairs(c, c->d_pairs, *n_pairs, c->stream, [&](const uint2* chunk, uint64_t len, uint64_t) {
+    fn(user, (const uint32_t*)chunk, len);
+    return PBIG_OK;
+  });
 }
 
 int pbig_hip_timing_start(pbig_hip_ctx* c, int max_calls) {
@@ -2305,29 +2459,45 @@ int pbig_hip_check_u32(pbig_hip_ctx* c, const uint32_t* d_boxes, uint32_t n, int
   return check_u32(c, d_boxes, n, dims, range, cell_size, (uint2*)d_pairs, cap, n_pairs, s);
 }
 
-int pbig_hip_check_host_u32(pbig_hip_ctx* c, const void* h_rects, uint32_t n, int dims, uint32_t range,
-                            uint32_t cell_size, uint32_t* h_pairs, uint64_t cap, uint64_t* n_pairs) {
+// The unsigned host entry points' device part: RectInfo {CoordInfo (2*dims
+// unsigned), rid} packed into pinned memory over the host threads (rid ==
+// index checked), uploaded, checked; the pairs are then in c->d_pairs.
+static int host_check_u32(pbig_hip_ctx* c, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                          uint32_t cell_size, uint64_t* n_pairs) {
   if (!c) return PBIG_E_ARG;
   if (!n_pairs) return fail(c, PBIG_E_ARG, "n_pairs is null");
   if (dims != 2 && dims != 3) return fail(c, PBIG_E_ARG, "dims must be 2 or 3");
   if (n && !h_rects) return fail(c, PBIG_E_ARG, "h_rects is null");
-  if (cap && !h_pairs) return fail(c, PBIG_E_ARG, "h_pairs is null with a nonzero capacity");
   HIP_TRY(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  // RectInfo {CoordInfo (2*dims unsigned), rid}: check rid == index, copy the
-  // coordinates out.
   const size_t w = 2 * (size_t)dims;
-  std::vector<uint32_t> coords((size_t)n * w);
-  const uint32_t* src = (const uint32_t*)h_rects;
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t* r = src + (size_t)i * (w + 1);
-    if (r[w] != i) return fail(c, PBIG_E_ARG, "rect %u has rid %u; rid must equal the index", i, r[w]);
-    memcpy(coords.data() + (size_t)i * w, r, w * 4);
+  const uint64_t words = std::max<uint64_t>((uint64_t)n * w, 1);
+  if (c->cap_h_in32 < words || !c->h_in32) {
+    if (c->h_in32) HIP_TRY(c, hipHostFree(c->h_in32));
+    c->h_in32 = nullptr;
+    c->cap_h_in32 = 0;
+    HIP_TRY(c, hipHostMalloc((void**)&c->h_in32, words * sizeof(uint32_t), hipHostMallocDefault));
+    c->cap_h_in32 = words;
   }
+  const uint32_t* src = (const uint32_t*)h_rects;
+  uint32_t* coords = c->h_in32;
+  uint64_t bad = 0;
+  int code = 0;
+  if (pack_records(
+          c, n,
+          [&](uint32_t i) {
+            const uint32_t* r = src + (size_t)i * (w + 1);
+            if (r[w] != i) return 1;
+            memcpy(coords + (size_t)i * w, r, w * 4);
+            return 0;
+          },
+          &bad, &code))
+    return fail(c, PBIG_E_ARG, "rect %llu has rid %u; rid must equal the index", (unsigned long long)bad,
+                src[bad * (w + 1) + w]);
   int rc;
-  if ((rc = grow(c, &c->d_in32, &c->cap_in32, std::max<uint64_t>((uint64_t)n * w, 1)))) return rc;
-  if (n) HIP_TRY(c, hipMemcpyAsync(c->d_in32, coords.data(), coords.size() * 4, hipMemcpyHostToDevice, s));
-  if (cell_size == 0) cell_size = pbig_hip_reference_cell_size_u32(coords.data(), n, dims, range);
+  if ((rc = grow(c, &c->d_in32, &c->cap_in32, words))) return rc;
+  if (n) HIP_TRY(c, hipMemcpyAsync(c->d_in32, coords, (size_t)n * w * 4, hipMemcpyHostToDevice, s));
+  if (cell_size == 0) cell_size = pbig_hip_reference_cell_size_u32(coords, n, dims, range);
   uint64_t want = std::max<uint64_t>(c->cap_pairs, std::max<uint64_t>(1024, (uint64_t)n));
   if ((rc = grow(c, &c->d_pairs, &c->cap_pairs, want))) return rc;
   uint64_t np = 0;
@@ -2338,15 +2508,35 @@ int pbig_hip_check_host_u32(pbig_hip_ctx* c, const void* h_rects, uint32_t n, in
   }
   if (rc) return rc;
   *n_pairs = np;
-  const uint64_t ncopy = std::min<uint64_t>(np, cap);
-  if (ncopy) {
-    HIP_TRY(c, hipMemcpyAsync(h_pairs, c->d_pairs, ncopy * sizeof(uint2), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-  }
+  return PBIG_OK;
+}
+
+int pbig_hip_check_host_u32(pbig_hip_ctx* c, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                            uint32_t cell_size, uint32_t* h_pairs, uint64_t cap, uint64_t* n_pairs) {
+  if (c && cap && !h_pairs) return fail(c, PBIG_E_ARG, "h_pairs is null with a nonzero capacity");
+  int rc;
+  if ((rc = host_check_u32(c, h_rects, n, dims, range, cell_size, n_pairs))) return rc;
+  const uint64_t np = *n_pairs;
+  if ((rc = stream_pairs(c, c->d_pairs, std::min<uint64_t>(np, cap), c->stream,
+                         [&](const uint2* chunk, uint64_t len, uint64_t first) {
+                           return copy_out(c, chunk, len, h_pairs + 2 * first);
+                         })))
+    return rc;
   if (np > cap)
     return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu", (unsigned long long)np,
                 (unsigned long long)cap);
   return PBIG_OK;
+}
+
+int pbig_hip_check_host_u32_each(pbig_hip_ctx* c, const void* h_rects, uint32_t n, int dims, uint32_t range,
+                                 uint32_t cell_size, pbig_hip_pairs_fn fn, void* user, uint64_t* n_pairs) {
+  if (c && !fn) return fail(c, PBIG_E_ARG, "fn is null");
+  int rc;
+  if ((rc = host_check_u32(c, h_rects, n, dims, range, cell_size, n_pairs))) return rc;
+  return stream_pairs(c, c->d_pairs, *n_pairs, c->stream, [&](const uint2* chunk, uint64_t len, uint64_t) {
+    fn(user, (const uint32_t*)chunk, len);
+    return PBIG_OK;
+  });
 }
 
 }  // extern "C"

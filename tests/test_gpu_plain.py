@@ -237,6 +237,47 @@ def test_host_entry_point_rectinfo(torch_cuda, eng, orc):
     assert e.value.status == pbig_amd.PBIG_E_ARG
 
 
+def test_host_entry_points_in_chunks(torch_cuda, eng, orc):
+    """The host entry points copy the pairs back in chunks of 8 Mi pairs
+    through two pinned buffers: 4,600 identical boxes (10,577,700 pairs, two
+    chunks) beside 20,000 uniform ones.  The pairs spread into the caller's
+    buffer, and the pairs handed chunk by chunk to a callback, both equal the
+    oracle's set."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    stack = np.tile(np.array([[3000.0, 3001.0, 3000.0, 3001.0]], np.float32), (4600, 1))
+    rest = workloads.uniform(20000, 2048.0, 1.0, 8.0, 55)
+    boxes = np.concatenate([rest[:9000], stack, rest[9000:]]).astype(np.float32)
+    want = orc.sort_pairs(orc.sweep_pairs(boxes))
+    assert want.shape[0] > (8 << 20)
+    got = orc.sort_pairs(eng.check_host(pbig_amd.as_rects(boxes), 4096.0, 16.0, capacity=want.shape[0]))
+    assert np.array_equal(got, want)
+    chunks = []
+    n = eng.check_host_each(pbig_amd.as_rects(boxes), 4096.0, 16.0, lambda p: chunks.append(p.copy()))
+    assert n == want.shape[0] and len(chunks) == 2
+    assert np.array_equal(orc.sort_pairs(np.concatenate(chunks)), want)
+
+
+def test_host_entry_point_reports_the_first_bad_record(torch_cuda, eng):
+    """Records are packed over several host threads; the error names the
+    first bad record of the array, as a loop in order would."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    boxes = workloads.uniform(300000, 8192.0, 1.0, 8.0, 56)
+    r = pbig_amd.as_rects(boxes)
+    r["xmin"][250000], r["xmax"][250000] = 5.0, 4.0  # inverted, later
+    r["rid"][100000] = 7
+    with pytest.raises(pbig_amd.PbigError) as e:
+        eng.check_host(r, 8192.0, 16.0)
+    assert e.value.status == pbig_amd.PBIG_E_ARG and "rect 100000 " in str(e.value)
+    r["rid"][100000] = 100000
+    with pytest.raises(pbig_amd.PbigError) as e:
+        eng.check_host(r, 8192.0, 16.0)
+    assert e.value.status == pbig_amd.PBIG_E_INVALID_BOX and "rect 250000 " in str(e.value)
+
+
 def test_repeated_calls_shrinking_and_growing(torch_cuda, eng, orc):
     from pbig_amd import workloads
 

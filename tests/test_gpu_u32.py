@@ -71,8 +71,14 @@ def test_pairs_match_brute_force(torch_cuda, eng, orc, dims, sim, cs):
 
 def test_host_records_entry_point(torch_cuda, eng, orc):
     b = orc.ref_generate(5000, 2000, 100, "UN", 3)
+    want = orc.sort_pairs(orc.brute_u32(b, 3))
     got = orc.sort_pairs(eng.check_host_u32(b, 3, 2000, 0))
-    assert np.array_equal(got, orc.sort_pairs(orc.brute_u32(b, 3)))
+    assert np.array_equal(got, want)
+    # the pairs handed to a callback chunk by chunk
+    chunks = []
+    n = eng.check_host_u32_each(b, 3, 2000, 0, lambda p: chunks.append(p.copy()))
+    assert n == want.shape[0]
+    assert np.array_equal(orc.sort_pairs(np.concatenate(chunks)), want)
 
 
 def test_out_of_range_coordinates_follow_the_reference(torch_cuda, eng, orc):
