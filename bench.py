@@ -507,6 +507,17 @@ def _cpu_model():
     return "unknown"
 
 
+def calibrate_copy(torch):
+    """A 256 MiB device-to-device copy, three times: a kernel of known traffic
+    (256 MiB read, 256 MiB written) for scaling rocprofv3's counters."""
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -536,11 +547,23 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (torch) three times, a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--only", choices=("clustered",),
+                    help="run only this leg (steps = its asynchronous calls) and print its figures, not the "
+                         "bench line: a profiler then sees that leg's kernels alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only:
+        import torch
+
+        import pbig_amd
+
+        if args.calibrate_copy:
+            calibrate_copy(torch)
+        print(json.dumps({args.only: clustered_leg(torch, pbig_amd.Engine, steps=args.steps)}))
+        return
 
     import numpy as np
     import torch
@@ -573,12 +596,7 @@ def main():
     B = alg_bytes(N, C_ref, M_ref, n_pairs)
 
     if args.calibrate_copy:
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(torch)
 
     stream = torch.cuda.Stream(device=dev)
     result = torch.zeros(2, dtype=torch.int64, device=dev)
