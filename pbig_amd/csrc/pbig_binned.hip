@@ -159,6 +159,22 @@ __device__ __forceinline__ void chunk_range(const InstDesc& I, uint32_t w, uint6
   *end = *start + I.chunk < inst_end ? *start + I.chunk : inst_end;
 }
 
+// Test schedule of a launch, the same in every kernel that depends on it:
+// the narrow one (positions keyed by their min-corner cell; the few wide
+// boxes tested against the cells under them) unless more than one box in
+// kWideShare spans more than two fine cells on an axis, and then the general
+// one (fine replication).  The narrow schedule tests each box against about
+// four and a half cells' boxes and the general one each pair of a cell once,
+// so the narrow one wins on sparse cells only; wide boxes come with varied
+// sizes and dense cells (measured on the clustered 2M instance: the general
+// schedule ahead at every wide share above one in a thousand), so a stray
+// wide box keeps the narrow schedule and more send the launch to the general
+// one.
+constexpr uint32_t kWideShare = 1024;
+__device__ __forceinline__ bool general_schedule(const Meta* m) {
+  return (uint64_t)m->wide * kWideShare > (uint64_t)m->boxes;
+}
+
 // kDev (all kernels below): a level planned on the device; its chunk and bin
 // counts are read from the level's result words (meta->dev_*), and the grid
 // is an upper bound.
@@ -168,7 +184,7 @@ __global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restr
                                                     const uint32_t* __restrict__ chunk_inst, uint32_t n_chunks,
                                                     uint32_t* __restrict__ hist, const Meta* __restrict__ meta) {
   __shared__ uint32_t h[kMaxBins];
-  __shared__ uint32_t err;
+  __shared__ uint32_t err;  // bit 0: an invalid box; bits 2..: the chunk's wide boxes
   if (kDev) n_chunks = meta->plan_err ? 0u : meta->dev_chunks;
   if (blockIdx.x >= n_chunks) return;
   const InstDesc I = inst[chunk_inst[blockIdx.x]];
@@ -197,7 +213,7 @@ __global__ __launch_bounds__(kHistThreads) void k_bin_hist(const float4* __restr
       const uint4 f = fine_corners(b, I);
       const BinRange r = bin_range_of(f, I);
       if (f.y - f.x > 1u || f.w - f.z > 1u) {  // spans more than two fine cells
-        atomicOr(&err, 2u);
+        atomicAdd(&err, 4u);
         for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
           for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&h[cy * I.nbx + cx], 1u);
       } else {  // at most two bins per axis: the first, then east, north, north-east
@@ -242,26 +258,48 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t* __restrict__ 
     n_chunks = meta->plan_err ? 0u : meta->dev_chunks;
   }
   if (blockIdx.x == gridDim.x - 1) {
-    // The last block: the result words of the call, the pair counter and the
-    // boxes' validity and width flags, from every chunk's histogram row.
+    // The last block: the result words of the call, the pair counter, the
+    // boxes' validity, and the launch's box and wide-box counts, from every
+    // chunk's histogram row.
     uint32_t e = 0;
+    uint64_t nw = 0, nbox = 0;
     for (uint32_t w = threadIdx.x; w < n_chunks; w += blockDim.x) {
       const InstDesc I = inst[chunk_inst[w]];
-      e |= hist[I.hist0 + (uint64_t)(w - I.chunk0) * hist_pitch(I.nb) + I.nb];
+      const uint32_t x = hist[I.hist0 + (uint64_t)(w - I.chunk0) * hist_pitch(I.nb) + I.nb];
+      e |= x & 1u;
+      nw += x >> 2;
+      uint64_t c0, c1;
+      chunk_range(I, w, &c0, &c1);
+      nbox += c1 - c0;
     }
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
+    for (int d = 32; d > 0; d >>= 1) {
+      e |= __shfl_xor(e, d, 64);
+      nw += __shfl_xor(nw, d, 64);
+      nbox += __shfl_xor(nbox, d, 64);
+    }
     __shared__ uint32_t es[16];
-    if ((threadIdx.x & 63) == 0) es[threadIdx.x >> 6] = e;
+    __shared__ uint64_t ws[16], bs[16];
+    if ((threadIdx.x & 63) == 0) {
+      es[threadIdx.x >> 6] = e;
+      ws[threadIdx.x >> 6] = nw;
+      bs[threadIdx.x >> 6] = nbox;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       uint32_t any = 0;
-      for (int k = 0; k < 16; ++k) any |= es[k];
+      uint64_t tw = 0, tb = 0;
+      for (int k = 0; k < 16; ++k) {
+        any |= es[k];
+        tw += ws[k];
+        tb += bs[k];
+      }
       meta->n_pairs = 0ull;
       meta->error = (any & 1u) ? kErrInvalidBox : 0ull;
       meta->items = 0ull;
       meta->max_per_cell = 0u;
-      meta->wide = (any & 2u) ? 1u : 0u;
+      meta->wide = tw < 0xFFFFFFFFull ? (uint32_t)tw : 0xFFFFFFFFu;
+      meta->boxes = tb < 0xFFFFFFFFull ? (uint32_t)tb : 0xFFFFFFFFu;
     }
     return;
   }
@@ -373,7 +411,7 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
       n_bins, L, [&](uint32_t i) { return (uint64_t)total[i]; },
       [&](uint32_t i, uint64_t ex) { bin_start[i] = (uint32_t)(ex < cap ? ex : cap); });
   if (dense_exit) {
-    const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
+    const uint32_t limit = general_schedule(meta) ? kMaxCoarse : kNarrowMax;
     uint64_t h = 0;
     for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
       const uint32_t t = total[i];
@@ -487,7 +525,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
       rs[k] = in && !fold ? js[b0 + k] : 0u;
     }
   };
-  const bool narrow = !meta->wide;  // the histogram's flag, uniform over the launch
+  const bool narrow = !meta->wide;  // no box of the launch spans more than two fine cells
   if (wc < wend) load_chunk(wc);
   if (fold) {
     // One instance: every workgroup scans its bins' totals into LDS (the
@@ -495,7 +533,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_bin_scatter(const float4* _
     // chunk loads; workgroup 0 also publishes the starts, the membership
     // count and the error bits.  Every workgroup reaches the same verdict.
     const InstDesc I0 = inst[0];
-    const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
+    const uint32_t limit = general_schedule(meta) ? kMaxCoarse : kNarrowMax;
     if (tid == 0) L.heavy = 0ull;
     uint64_t hv = 0;
     const uint64_t grand = block_exscan(
@@ -663,10 +701,11 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
                                                              Meta* __restrict__ meta,
                                                              Meta* __restrict__ meta_next, uint32_t count_all) {
   extern __shared__ __align__(16) unsigned char place_raw[];
-  __shared__ uint32_t flags;  // 1 invalid box, 2 wide box, 4 bin over capacity (or too many boxes listed)
+  __shared__ uint32_t flags;  // 1 invalid box, 4 bin over capacity (or too many boxes listed)
   __shared__ uint32_t wsum[kPlaceThreads / 64];
   __shared__ uint32_t npend;  // boxes of the round with memberships besides a first one
   __shared__ uint64_t resume;           // first box not counted when a bin outgrew its share
+  __shared__ uint32_t nwide;            // wide boxes of the workgroup's range
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const InstDesc I = inst[0];
   const uint32_t nb = I.nb, nbx = I.nbx;
@@ -682,7 +721,10 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
     bin_start[b] = b * bin_cap;  // every bin's share starts at b * bin_cap (read by the test kernel)
   if (blockIdx.x == 0 && tid < sizeof(Meta) / sizeof(unsigned long long))
     reinterpret_cast<unsigned long long*>(meta_next)[tid] = 0ull;
-  if (tid == 0) flags = 0u;
+  if (tid == 0) {
+    flags = 0u;
+    nwide = 0u;
+  }
   const uint64_t n = I.n;
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t g0 = (uint64_t)blockIdx.x * per;
@@ -720,7 +762,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
           const uint32_t bb = rg.y0 * nbx + rg.x0;
           uint32_t sw;
           if (f.y - f.x > 1u || f.w - f.z > 1u) {
-            fl |= 2u;
+            atomicAdd(&nwide, 1u);
             sw = bb | kSlotWide;
             for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
               for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) atomicAdd(&cnt2[cy * nbx + cx], 1u);
@@ -854,7 +896,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
       for (uint64_t i = r0 + tid; i < r1; i += kPlaceThreads) {
         const float4 b = boxes[I.box0 + i];
         if (!((b.x <= b.y) && (b.z <= b.w))) continue;  // (flagged by no one: the launch has failed anyway)
-        const BinRange rg = bin_range_of(fine_corners(b, I), I);
+        const uint4 f = fine_corners(b, I);
+        if (f.y - f.x > 1u || f.w - f.z > 1u) atomicAdd(&nwide, 1u);
+        const BinRange rg = bin_range_of(f, I);
         for (uint32_t cy = rg.y0; cy <= rg.y1; ++cy)
           for (uint32_t cx = rg.x0; cx <= rg.x1; ++cx) atomicAdd(&cnt1[cy * nbx + cx], 1u);
       }
@@ -866,8 +910,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
   }
   if (tid == 0) {
     if (flags & 1u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrInvalidBox);
-    if (flags & 2u) atomicOr(&meta->wide, 1u);
+    if (nwide) atomicAdd(&meta->wide, nwide);
     if (flags & 4u) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrBinCap);
+    if (blockIdx.x == 0) meta->boxes = n < 0xFFFFFFFFull ? (uint32_t)n : 0xFFFFFFFFu;
   }
 }
 
@@ -879,7 +924,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_bin_place(const float4* __res
 // thread); its fine cells (k x k) are counted and scanned in LDS and then
 // processed in strips: row-major ranges of fine cells whose memberships fit
 // the LDS buffer.  For each strip the memberships are scattered into
-// fine-cell order in LDS and every cell's pairs are tested.  Pairs go to a
+// fine-cell order in LDS (cells by size class) and every cell's pairs are
+// tested.  A box covering more than kHugeCells cells of the bin is listed
+// instead and tested against every membership of the bin.  Pairs go to a
 // per-wave LDS stage and then to this workgroup's own region of the pair
 // scratch through an LDS counter: no global atomics.  k_bin_compact later
 // concatenates the regions.  The next bin's registers are loaded after the
@@ -892,16 +939,21 @@ constexpr uint32_t kRegs = kMaxCoarse / kTestThreads;  // coarse memberships per
 
 // Position word: coarse-membership index (13 bits) | local fine cell (12 bits)
 // | "starts in this row" (bit 30) | "starts in this column" (bit 31).
+// A box covering more than kHugeCells fine cells of a bin is not replicated
+// into them (one lane would place it cell by cell, the workgroup waiting):
+// up to kHugeMax such boxes per bin are listed and tested against every
+// coarse membership of the bin, with the reference-cell rule on the bin.
+constexpr uint32_t kHugeCells = 16, kHugeMax = 256;
+constexpr uint32_t kFrHuge = 1u << 31;  // packed fine range: the box is listed (index in the low bits)
 constexpr uint32_t kWordCellShift = 13;
 constexpr uint32_t kWordIdxMask = (1u << kWordCellShift) - 1u;
 constexpr uint32_t kWordCellMask = (1u << 12) - 1u;
 static_assert(kMaxCoarse <= (1u << kWordCellShift), "membership index field");
 static_assert(kMaxCellsPerBin <= (1u << 12), "cell field");
 
-// Cells of one bin are laid out by size class (min(count, 63), largest
-// first) when the whole bin fits one strip, so that the 64 positions of a
-// wave belong to cells of about the same size and the wave's partner loop
-// (its longest lane) wastes few slots.
+// Cells of a strip are laid out by size class (min(count, 63), largest
+// first), so that the 64 positions of a wave belong to cells of about the
+// same size and the wave's partner loop (its longest lane) wastes few slots.
 constexpr uint32_t kSizeClasses = 64;
 
 struct TestLds {
@@ -915,6 +967,7 @@ struct TestLds {
   uint32_t cstart[kSizeClasses];   // first position of each size class, then its cursor
   uint32_t nfine;                  // fine memberships of the bin
   uint32_t npair;                  // positions in cells of two or more memberships (sorted layout)
+  uint32_t nhuge;                  // the bin's large boxes (listed, not replicated)
   uint2 stage[kTestWaves][kBinStage];   // per-wave staging of hits (coarse membership indices)
   uint2 binfo[kMaxBinsPerGroup];   // (start, count) of this workgroup's bins
   uint4 bingrid[kMaxBinsPerGroup]; // (first fine column, first fine row, kx | ky << 16, nf) of each bin
@@ -1012,7 +1065,7 @@ __device__ void single_pass_dense_rule(const uint32_t* __restrict__ total, uint3
                                        unsigned long long err, Meta* __restrict__ meta) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   if (!(err & kErrBinCap) || !(single & kSingleDense) || blockIdx.x != 0 || (err & kErrInvalidBox)) return;
-  const uint32_t limit = meta->wide ? kMaxCoarse : kNarrowMax;
+  const uint32_t limit = general_schedule(meta) ? kMaxCoarse : kNarrowMax;
   unsigned long long h = 0, g = 0;
   for (uint32_t b = tid; b < n_bins; b += blockDim.x) {
     const uint32_t t = total[b];
@@ -1068,6 +1121,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
   if (tid == 0) {
     L.out_n = 0u;
     L.nfine = 0u;
+    L.nhuge = 0u;
     L.memb = 0ull;
   }
 
@@ -1120,8 +1174,14 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
     bool prefetched = false;
     if (m >= 2 && m <= kMaxCoarse) {
       // A. fine-cell counts of the whole bin; ids to LDS
-      uint32_t fr[kRegs];  // packed fine ranges, reused by every strip's scatter
+      uint32_t fr[kRegs];  // packed fine ranges, reused by every strip's scatter (or kFrHuge | list index)
       uint32_t mine = 0;   // this thread's fine memberships
+      // the large boxes' list: the tail of the strip buffer, free until the
+      // first strip's scatter
+      float4* const hbox = L.box + (kMaxFine - kHugeMax);
+      uint32_t* const hidx = L.word + (kMaxFine - kHugeMax);
+      uint32_t* const hcx = L.word + (kMaxFine - 2u * kHugeMax);
+      uint32_t* const hcy = L.word + (kMaxFine - 3u * kHugeMax);
 #pragma unroll
       for (uint32_t k = 0; k < kRegs; ++k) {
         const uint32_t j = tid + k * kTestThreads;
@@ -1129,10 +1189,20 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
         if (j < m) {
           L.id[j] = ri[k];
           const FineRange r = fine_range(rb[k], p, gx, gy);
-          fr[k] = pack_fine(r, gx, gy);
-          mine += (r.x1 - r.x0 + 1u) * (r.y1 - r.y0 + 1u);
-          for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
-            for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * Kx + cx], 1u);
+          const uint32_t cells = (r.x1 - r.x0 + 1u) * (r.y1 - r.y0 + 1u);
+          const uint32_t at = cells > kHugeCells ? atomicAdd(&L.nhuge, 1u) : kHugeMax;
+          if (at < kHugeMax) {
+            hbox[at] = rb[k];
+            hidx[at] = j;
+            hcx[at] = r.fx0;
+            hcy[at] = r.fy0;
+            fr[k] = kFrHuge | at;
+          } else {
+            fr[k] = pack_fine(r, gx, gy);
+            mine += cells;
+            for (uint32_t cy = r.y0; cy <= r.y1; ++cy)
+              for (uint32_t cx = r.x0; cx <= r.x1; ++cx) atomicAdd(&L.cnt[cy * Kx + cx], 1u);
+          }
         }
       }
 #pragma unroll
@@ -1140,45 +1210,11 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
       if (lane == 0) atomicAdd(&L.nfine, mine);
       __syncthreads();
       const uint32_t nfine = L.nfine;
-      const bool sorted = nfine <= kMaxFine;
-      // B. cell positions (kk <= 4096: 4 cells per thread)
-      uint32_t npair = nfine;
-      if (sorted) {
-        // by size class: class totals, their exclusive scan (largest class
-        // first), then each cell takes its run from its class's cursor
-        uint32_t v[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t c = tid * 4 + k;
-          v[k] = c < kk ? L.cnt[c] : 0u;
-          if (c < kk) L.ccount[c] = (uint16_t)v[k];
-          if (v[k]) atomicAdd(&L.csum[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
-        }
-        __syncthreads();
-        if (w == 0) {
-          const uint32_t cls = kSizeClasses - 1u - lane;  // lane 0: the largest class
-          const uint32_t own = L.csum[cls];
-          const uint32_t inc = wave_prefix_sum(own);
-          L.cstart[cls] = inc - own;
-          L.csum[cls] = 0u;  // cleared for the next bin
-          if (cls == 1u) L.npair = inc - own;
-          if (lane == 0) L.nfine = 0u;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t c = tid * 4 + k;
-          if (v[k]) {
-            const uint32_t st = atomicAdd(&L.cstart[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
-            L.start[c] = st;
-            L.cnt[c] = st;  // scatter cursor
-          }
-        }
-        if (tid == 0) L.start[kk] = nfine;
-        __syncthreads();
-        npair = L.npair;
-      } else {
-        // row-major: exclusive scan of the counts
+      const bool multi = nfine > kMaxFine;  // more than one strip
+      // B. each cell's count kept (ccount); with more than one strip, the
+      //    row-major exclusive scan of the counts (start[]) places the
+      //    strips' boundaries (kk <= 4096: 4 cells per thread)
+      {
         uint32_t v[4], acc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -1187,30 +1223,73 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
           if (c < kk) L.ccount[c] = (uint16_t)v[k];
           acc += v[k];
         }
-        const uint32_t inc = wave_prefix_sum(acc);
-        if (lane == 63) L.wsum[w] = inc;
-        __syncthreads();
-        uint32_t ex = inc - acc;
-        ex += __builtin_amdgcn_readlane(row_prefix_sum(lane < w ? L.wsum[lane] : 0u), 15);  // waves before mine
+        if (multi) {
+          const uint32_t inc = wave_prefix_sum(acc);
+          if (lane == 63) L.wsum[w] = inc;
+          __syncthreads();
+          uint32_t ex = inc - acc;
+          ex += __builtin_amdgcn_readlane(row_prefix_sum(lane < w ? L.wsum[lane] : 0u), 15);  // waves before mine
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t c = tid * 4 + k;
-          if (c < kk) {
-            L.start[c] = ex;
-            L.cnt[c] = ex;  // scatter cursor
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t c = tid * 4 + k;
+            if (c < kk) L.start[c] = ex;
+            ex += v[k];
           }
-          ex += v[k];
+          if (tid == kTestThreads - 1) L.start[kk] = ex;
         }
-        if (tid == kTestThreads - 1) L.start[kk] = ex;
-        if (tid == 0) L.nfine = 0u;  // every thread read it before the barrier above
         __syncthreads();
+        if (tid == 0) L.nfine = 0u;  // (every thread has read it)
+      }
+      // L. the listed large boxes against every coarse membership of the bin
+      //    (each lane its registers' boxes; a listed box against the entries
+      //    after its own): a pair is the bin's when its reference cell R =
+      //    (max of the min-corner cells) is in the bin, that is, not west
+      //    or south of it (both boxes reach the bin, so R is never east or
+      //    north of it).
+      const uint32_t nh = L.nhuge < kHugeMax ? L.nhuge : kHugeMax;
+      if (nh) {
+        uint32_t afx[kRegs], afy[kRegs], hfrom[kRegs];
+#pragma unroll
+        for (uint32_t k = 0; k < kRegs; ++k) {
+          const uint32_t j = tid + k * kTestThreads;
+          afx[k] = fine_of(rb[k].x, p.inv_f, p.nf);
+          afy[k] = fine_of(rb[k].z, p.inv_f, p.nf);
+          hfrom[k] = j >= m ? nh : ((fr[k] & kFrHuge) ? (fr[k] & ~kFrHuge) + 1u : 0u);
+        }
+        float4 xn = hbox[0];
+        uint32_t hxn = hcx[0], hyn = hcy[0], hjn = hidx[0];
+        for (uint32_t h = 0; h < nh; ++h) {
+          const float4 x = xn;
+          const uint32_t hx = hxn, hy = hyn, hj = hjn;
+          if (h + 1u < nh) {  // the next listed box, in flight during these tests
+            xn = hbox[h + 1u];
+            hxn = hcx[h + 1u];
+            hyn = hcy[h + 1u];
+            hjn = hidx[h + 1u];
+          }
+#pragma unroll
+          for (uint32_t k = 0; k < kRegs; ++k) {
+            const float4 a = rb[k];
+            const uint64_t mask = __ballot(h >= hfrom[k] && max(afx[k], hx) >= gx && max(afy[k], hy) >= gy &&
+                                           a.x <= x.y && x.x <= a.y && a.z <= x.w && x.z <= a.w);
+            if (mask) {
+              const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+              if (__builtin_amdgcn_inverse_ballot_w64(mask)) L.stage[w][pos] = make_uint2(tid + k * kTestThreads, hj);
+              sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
+              if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
+            }
+          }
+        }
+        __syncthreads();  // the list is the strip buffer's tail
+        if (tid == 0) L.nhuge = 0u;
       }
       // C/D per strip
       // start[] is read-only from here on
       uint32_t c_lo = 0;
       while (c_lo < kk) {
         uint32_t c_hi = kk;  // usually the rest of the bin fits in one strip
-        if (!sorted && L.start[kk] - L.start[c_lo] > kMaxFine) {
+        if (multi && L.start[kk] - L.start[c_lo] > kMaxFine) {
           if (tid == 0) {
             // largest c_hi with start[c_hi] - start[c_lo] <= kMaxFine
             const uint32_t lim = L.start[c_lo] + kMaxFine;
@@ -1235,12 +1314,45 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
           __syncthreads();
           break;
         }
-        const uint32_t base = sorted ? 0u : L.start[c_lo];
-        const uint32_t len = sorted ? npair : L.start[c_hi] - base;
+        // the strip's cells laid out by size class: class totals, their
+        // exclusive scan (largest class first), then each cell takes its run
+        // from its class's cursor (start[] of the strip's cells becomes the
+        // strip position; the next strip's boundary reads only later cells)
+        {
+          uint32_t v[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t c = tid * 4 + k;
+            v[k] = c >= c_lo && c < c_hi ? (uint32_t)L.ccount[c] : 0u;
+            if (v[k]) atomicAdd(&L.csum[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
+          }
+          __syncthreads();
+          if (w == 0) {
+            const uint32_t cls = kSizeClasses - 1u - lane;  // lane 0: the largest class
+            const uint32_t own = L.csum[cls];
+            const uint32_t inc = wave_prefix_sum(own);
+            L.cstart[cls] = inc - own;
+            L.csum[cls] = 0u;  // cleared for the next strip
+            if (cls == 1u) L.npair = inc - own;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t c = tid * 4 + k;
+            if (v[k]) {
+              const uint32_t st = atomicAdd(&L.cstart[v[k] < kSizeClasses ? v[k] : kSizeClasses - 1u], v[k]);
+              L.start[c] = st;
+              L.cnt[c] = st;  // scatter cursor
+            }
+          }
+          __syncthreads();
+        }
+        const uint32_t base = 0u;
+        const uint32_t len = L.npair;  // positions in cells of two or more memberships
 #pragma unroll
         for (uint32_t k = 0; k < kRegs; ++k) {
           const uint32_t j = tid + k * kTestThreads;
-          if (j < m) scatter_strip(L, rb[k], j, fr[k], Kx, c_lo, c_hi, base);
+          if (j < m && !(fr[k] & kFrHuge)) scatter_strip(L, rb[k], j, fr[k], Kx, c_lo, c_hi, base);
         }
         __syncthreads();
         const bool last_strip = c_hi == kk;
@@ -1386,20 +1498,22 @@ struct NarrowLds {
   float4 box[kNarrowMax];          // the bin's coarse memberships in cell order
   uint16_t cell[kNarrowMax];       // extended cell of each position
   uint32_t id[kNarrowMax];         // box id of each position
-  uint32_t cnt[kMaxExt];           // counts (each membership's rank in its cell is the count before it)
-  uint16_t start[kMaxExt + 1];     // first position of each extended cell
+  uint32_t cnt[kMaxExt + 1];       // counts (each membership's rank in its cell is the count before it);
+                                   // the wide boxes count as one more cell after the extended ones
+  uint16_t start[kMaxExt + 2];     // first position of each extended cell (and of the wide boxes)
   uint2 stage[kTestWaves][kBinStage];
   uint2 binfo[kMaxBinsPerGroup];
   uint4 bingrid[kMaxBinsPerGroup];
   float bininv[kMaxBinsPerGroup];
   uint32_t wsum[kTestWaves];
   uint32_t dnext;                  // next chunk of positions to claim, past the first 16
+  uint32_t wnext;                  // next wide box to claim
   uint32_t out_n;
   unsigned long long out_base;
   unsigned long long memb;         // kSingleSum: memberships of this workgroup's bins
 };
 static_assert(sizeof(NarrowLds) <= 160u * 1024u, "one test workgroup per CU");
-static_assert(kMaxExt <= 5u * kTestThreads, "the cell scan takes five cells per thread");
+static_assert(kMaxExt + 1u <= 5u * kTestThreads, "the cell scan takes five cells per thread");
 
 __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4* __restrict__ bbox,
                                                                   const uint32_t* __restrict__ bid,
@@ -1427,7 +1541,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
   uint2* region = scratch + (size_t)blockIdx.x * cap_wg;
   uint32_t sn = 0;
   uint32_t seg0 = 0;
-  for (uint32_t c = tid; c < kMaxExt; c += kTestThreads) L.cnt[c] = 0u;  // zero at the start of every bin
+  for (uint32_t c = tid; c <= kMaxExt; c += kTestThreads) L.cnt[c] = 0u;  // zero at the start of every bin
+  const bool any_wide = meta->wide != 0u;  // (the launch's count: the classification below is skipped without)
   if (tid == 0) {
     L.out_n = 0u;
     L.memb = 0ull;
@@ -1478,8 +1593,9 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     const uint32_t nf = bg.w;
     bool prefetched = false;
     if (m >= 2 && m <= kNarrowMax) {
-      // A. each membership's extended cell, and its rank among the cell's
-      //    memberships (the count before it)
+      // A. each membership's extended cell (a wide box: cell ne, after the
+      //    extended cells), and its rank among the cell's memberships (the
+      //    count before it)
       uint32_t ek[kNarrowRegs], rk[kNarrowRegs];
 #pragma unroll
       for (uint32_t k = 0; k < kNarrowRegs; ++k) {
@@ -1487,21 +1603,26 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         ek[k] = 0u;
         rk[k] = 0u;
         if (j < m) {
-          int rx = (int)fine_of(rb[k].x, inv_f, nf) - gx, ry = (int)fine_of(rb[k].z, inv_f, nf) - gy;
-          rx = rx < -1 ? -1 : (rx > (int)Kx - 1 ? (int)Kx - 1 : rx);
-          ry = ry < -1 ? -1 : (ry > (int)Ky - 1 ? (int)Ky - 1 : ry);
-          ek[k] = (uint32_t)(rx + 1) + (uint32_t)(ry + 1) * ex;
+          const uint32_t fx = fine_of(rb[k].x, inv_f, nf), fy = fine_of(rb[k].z, inv_f, nf);
+          if (any_wide && (fine_of(rb[k].y, inv_f, nf) - fx > 1u || fine_of(rb[k].w, inv_f, nf) - fy > 1u)) {
+            ek[k] = ne;
+          } else {
+            int rx = (int)fx - gx, ry = (int)fy - gy;
+            rx = rx < -1 ? -1 : (rx > (int)Kx - 1 ? (int)Kx - 1 : rx);
+            ry = ry < -1 ? -1 : (ry > (int)Ky - 1 ? (int)Ky - 1 : ry);
+            ek[k] = (uint32_t)(rx + 1) + (uint32_t)(ry + 1) * ex;
+          }
           rk[k] = atomicAdd(&L.cnt[ek[k]], 1u);
         }
       }
       __syncthreads();
-      // B. exclusive scan of the counts (ne <= kMaxExt: 5 per thread)
+      // B. exclusive scan of the counts (ne + 1 <= kMaxExt + 1: 5 per thread)
       {
         uint32_t v[5], acc = 0;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           const uint32_t c = tid * 5 + k;
-          v[k] = c < ne ? L.cnt[c] : 0u;
+          v[k] = c <= ne ? L.cnt[c] : 0u;
           acc += v[k];
         }
         const uint32_t inc = wave_prefix_sum(acc);
@@ -1512,11 +1633,14 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           const uint32_t c = tid * 5 + k;
-          if (c < ne) L.start[c] = (uint16_t)exs;
+          if (c <= ne) L.start[c] = (uint16_t)exs;
           exs += v[k];
         }
-        if (tid == kTestThreads - 1) L.start[ne] = (uint16_t)exs;
-        if (tid == 0) L.dnext = 0u;
+        if (tid == kTestThreads - 1) L.start[ne + 1u] = (uint16_t)exs;
+        if (tid == 0) {
+          L.dnext = 0u;
+          L.wnext = 0u;
+        }
       }
       __syncthreads();
       // C. one position per membership, in cell order: box, cell and id
@@ -1531,7 +1655,8 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         }
       }
       __syncthreads();
-      for (uint32_t c = tid; c < ne; c += kTestThreads) L.cnt[c] = 0u;  // for the next bin
+      const uint32_t mn = L.start[ne];  // narrow positions; the wide boxes follow
+      for (uint32_t c = tid; c <= ne; c += kTestThreads) L.cnt[c] = 0u;  // for the next bin
       if (t + 1 < my_bins) {
 #pragma unroll
         for (uint32_t k = 0; k < kNarrowRegs; ++k) {
@@ -1549,11 +1674,11 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
       //    Chunks of 64 positions: a wave's first by its index, the rest from
       //    a shared counter, so that waves whose chunks had fewer partners
       //    take more of them.
-      for (uint32_t p0 = w * 64; p0 < m;) {
+      for (uint32_t p0 = w * 64; p0 < mn;) {
         const uint32_t p = p0 + lane;
         uint32_t b1 = 0, n1 = 0, b3 = 0, n3 = 0;
         float4 a = make_float4(0.f, -1.f, 0.f, -1.f);
-        if (p < m) {
+        if (p < mn) {
           a = L.box[p];
           const uint32_t e = L.cell[p];
           const uint32_t X = e % ex, Y = e / ex;
@@ -1601,6 +1726,59 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         uint32_t claim = 0;
         if (lane == 0) claim = atomicAdd(&L.dnext, 64u);
         p0 = kTestThreads + __builtin_amdgcn_readfirstlane(claim);
+      }
+      // E. the wide boxes (positions mn..m-1), claimed one at a time by a
+      //    wave.  A narrow box B overlapping a wide box A has its min-corner
+      //    cell in A's cells or one column west or one row south of them, and
+      //    the pair's R = (max of the min-corner cells) is inside the bin
+      //    exactly when B's column is >= the bin's first column unless A's
+      //    is, and likewise for rows: in each of those rows of the extended
+      //    grid the partners are one contiguous run of positions, a lane per
+      //    partner.  Then the later wide boxes, R checked.
+      if (m > mn) {
+        const uint32_t nwb = m - mn;
+        for (;;) {
+          uint32_t claim = 0;
+          if (lane == 0) claim = atomicAdd(&L.wnext, 1u);
+          const uint32_t i = __builtin_amdgcn_readfirstlane(claim);
+          if (i >= nwb) break;
+          const uint32_t pw = mn + i;
+          const float4 a = L.box[pw];
+          const int ax0 = (int)fine_of(a.x, inv_f, nf), ax1 = (int)fine_of(a.y, inv_f, nf);
+          const int ay0 = (int)fine_of(a.z, inv_f, nf), ay1 = (int)fine_of(a.w, inv_f, nf);
+          auto stage_pairs = [&](uint64_t mask, uint32_t qq) {
+            const uint32_t pos = sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            if (__builtin_amdgcn_inverse_ballot_w64(mask)) L.stage[w][pos] = make_uint2(pw, qq);
+            sn = __builtin_amdgcn_readfirstlane(sn + (uint32_t)__popcll(mask));
+            if (sn > kBinStage - 64) stage_flush(L, w, sn, region, cap_wg);
+          };
+          // extended columns X0..X1 and rows Y0..Y1 (bin column c is extended column c + 1)
+          const int X0 = (ax0 >= gx ? ax0 - 1 : gx) - gx + 1, X1 = min(ax1, gx + (int)Kx - 1) - gx + 1;
+          const int Y0 = (ay0 >= gy ? ay0 - 1 : gy) - gy + 1, Y1 = min(ay1, gy + (int)Ky - 1) - gy + 1;
+          if (X0 <= X1 && Y0 <= Y1) {
+            for (int Y = Y0; Y <= Y1; ++Y) {
+              const uint32_t e0 = (uint32_t)Y * ex;
+              const uint32_t s0 = L.start[e0 + (uint32_t)X0], s1 = L.start[e0 + (uint32_t)X1 + 1u];
+              for (uint32_t q0 = s0; q0 < s1; q0 += 64) {
+                const uint32_t q = q0 + lane < s1 ? q0 + lane : s1 - 1u;
+                const float4 x = L.box[q];
+                const uint64_t hit =
+                    __ballot(q0 + lane < s1 && a.x <= x.y && x.x <= a.y && a.z <= x.w && x.z <= a.w);
+                if (hit) stage_pairs(hit, q);
+              }
+            }
+          }
+          for (uint32_t j0 = i + 1u; j0 < nwb; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const uint32_t q = mn + (j < nwb ? j : nwb - 1u);
+            const float4 x = L.box[q];
+            const bool r_in = max(ax0, (int)fine_of(x.x, inv_f, nf)) >= gx && max(ay0, (int)fine_of(x.z, inv_f, nf)) >= gy;
+            const uint64_t hit =
+                __ballot(j < nwb && r_in && a.x <= x.y && x.x <= a.y && a.z <= x.w && x.z <= a.w);
+            if (hit) stage_pairs(hit, q);
+          }
+        }
       }
       if (sn) stage_flush(L, w, sn, region, cap_wg);
       __syncthreads();  // the positions are reused by the next bin
@@ -1666,7 +1844,7 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   unsigned long long* const out_count =
       direct ? (kDev ? meta->out_count : (unsigned long long*)&meta->n_pairs) : nullptr;
   if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
-  if (meta->wide)
+  if (general_schedule(meta))
     test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
                  overfull, out, cap, out_count, meta, single);
   else
@@ -1760,6 +1938,7 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
                                                     Meta* __restrict__ dense) {
   __shared__ ScanLds L;
   __shared__ uint32_t c0[kDeepMaxInst + 1], b0[kDeepMaxInst + 1];
+  __shared__ uint32_t ord[kDeepMaxInst];
   __shared__ uint32_t nbmax;
   const uint32_t tid = threadIdx.x;
   // (a level planned on the host has no plan error word of its own)
@@ -1768,9 +1947,25 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
   if (tid == 0) nbmax = 0u;
   __syncthreads();
   const uint32_t n_inst = n_over <= caps.inst && n_over <= kDeepMaxInst ? n_over : 0u;
+  // 0. the listed bins in increasing order: the test kernel appends them in
+  //    the order its workgroups finish, and the plan (so every bin's
+  //    workgroup and every workgroup's pair count) must be the same in each
+  //    launch of the same input.  Rank of each entry by a broadcast walk.
+  for (uint32_t i = tid; i < n_inst; i += 1024) c0[i] = poverfull[i];
+  __syncthreads();
+  for (uint32_t i = tid; i < n_inst; i += 1024) {
+    const uint32_t v = c0[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n_inst; ++j) {
+      const uint32_t u = c0[j];
+      r += (u < v || (u == v && j < i)) ? 1u : 0u;
+    }
+    ord[r] = v;
+  }
+  __syncthreads();
   // 1. every listed bin's virtual instance
   for (uint32_t i = tid; i < n_inst; i += 1024) {
-    const uint32_t b = poverfull[i] & ~kOverfullCell;
+    const uint32_t b = ord[i] & ~kOverfullCell;
     InstDesc v{};
     if (b < p_bins) {
       const uint32_t pi = pbin_inst[b];

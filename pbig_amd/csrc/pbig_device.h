@@ -33,7 +33,11 @@ struct Meta {
   unsigned long long memberships;
   unsigned long long items;
   unsigned int max_per_cell;
-  unsigned int wide;  // binned path: some box spans more than two fine cells on an axis
+  // Binned path: the launch's boxes that span more than two fine cells on an
+  // axis (any: the scatter places every membership by loops), and its box
+  // count (saturated at 2^32 - 1), from which every kernel of the launch
+  // picks the same test schedule.
+  unsigned int wide;
   // A re-binning level planned on the device: the sizes its plan kernel
   // wrote (also when they exceed the level's workspace, which plan_err then
   // flags), read by the level's other kernels.
@@ -43,7 +47,8 @@ struct Meta {
   // level's test kernel reserves its output from (device-planned levels).
   unsigned long long* out_count;
   unsigned int plan_err;  // the level's plan did not fit its workspace
-  unsigned int pad[3];
+  unsigned int boxes;
+  unsigned int pad[2];
 };
 static_assert(sizeof(Meta) == 96, "Meta layout");
 

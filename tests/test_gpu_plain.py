@@ -448,28 +448,90 @@ def test_binned_deep_rebinning(torch_cuda, binned, orc):
     assert np.array_equal(got, orc.sort_pairs(orc.sweep_pairs(boxes)))
 
 
+def _wide_share(boxes: np.ndarray, cs: float) -> float:
+    """Share of boxes spanning more than two cells of side cs on an axis
+    (the narrow test schedule keeps a launch up to one in 1,024)."""
+    inv = np.float32(1.0) / np.float32(cs)
+    f = np.floor(boxes * inv)
+    return float(np.mean((f[:, 1] - f[:, 0] > 1) | (f[:, 3] - f[:, 2] > 1)))
+
+
+def _lattice_wide_boxes(n: int, L: float, cs: float, seed: int) -> np.ndarray:
+    """n boxes 2 to 40 cells wide on a quarter-cell lattice (edges on cell
+    lines, touching the lattice boxes' edges), some of them flat."""
+    rng = np.random.default_rng(seed)
+    q = cs / 4
+    x0 = rng.integers(-8, int(L / q), n) * q
+    y0 = rng.integers(-8, int(L / q), n) * q
+    w = rng.integers(8, 160, n) * q
+    h = np.where(rng.random(n) < 0.2, 0.0, rng.integers(1, 160, n) * q)
+    sw = rng.random(n) < 0.5  # which side is the wide one
+    ww, hh = np.where(sw, w, h), np.where(sw, h, w)
+    return np.stack([x0, x0 + ww, y0, y0 + hh], 1).astype(np.float32)
+
+
 @pytest.mark.parametrize("bin_target", [200, 3500])
 def test_narrow_kernel_on_a_touching_lattice(torch_cuda, binned, orc, bin_target):
     """Boxes exactly one cell wide on a half-cell lattice: every box spans two
     cells per axis, neighbours share edges and corners (closed intervals), and
     many pairs have their reference cell on a bin edge or in the halo.  The
-    narrow-box test kernel runs; then one box two cells wide is added, which
-    sends the same launch to the general test kernel.  Both must give the
-    brute-force set."""
+    narrow-box test schedule runs; then one box two cells wide is added; then,
+    on a lattice four times larger, 50 wide boxes (under one in 1,024: the
+    narrow schedule's wide-box pass, with wide boxes crossing many bins,
+    starting west of and south of the domain, and meeting each other with
+    their reference cell in another bin); then, on the first lattice, enough
+    wide boxes to send the launch to the general schedule.  Every launch must
+    give the oracle's set."""
     cs = 8.0
-    g = np.arange(0, 120, dtype=np.float32) * (cs / 2)
-    x, y = np.meshgrid(g, g, indexing="ij")
-    x, y = x.ravel(), y.ravel()
-    boxes = np.stack([x, x + cs, y, y + cs], 1).astype(np.float32)
-    L = float(g[-1] + cs)
+
+    def lattice(side):
+        g = np.arange(0, side, dtype=np.float32) * (cs / 2)
+        x, y = np.meshgrid(g, g, indexing="ij")
+        x, y = x.ravel(), y.ravel()
+        return np.stack([x, x + cs, y, y + cs], 1).astype(np.float32), float(g[-1] + cs)
+
+    boxes, L = lattice(120)
+    big, L_big = lattice(240)
     binned.configure(PATHS["binned"], bin_target)
     got = run_device(torch_cuda, binned, boxes, L, cs)
     want = orc.sort_pairs(orc.brute_pairs(boxes))
     assert np.array_equal(got, want), (len(got), len(want))
-    wide = np.concatenate([boxes, np.array([[100.0, 100.0 + 2 * cs, 100.0, 101.0]], np.float32)])
-    got = run_device(torch_cuda, binned, wide, L, cs)
-    want = orc.sort_pairs(orc.brute_pairs(wide))
-    assert np.array_equal(got, want), (len(got), len(want))
+    one = np.concatenate([boxes, np.array([[100.0, 100.0 + 2 * cs, 100.0, 101.0]], np.float32)])
+    few = np.concatenate([big, _lattice_wide_boxes(50, L_big, cs, 5)])
+    many = np.concatenate([boxes, _lattice_wide_boxes(3000, L, cs, 6)])
+    assert _wide_share(one, cs) < 1 / 1024 and _wide_share(few, cs) < 1 / 1024 < _wide_share(many, cs)
+    for case, side in ((one, L), (few, L_big), (many, L)):
+        got = run_device(torch_cuda, binned, case, side, cs)
+        want = orc.sort_pairs(orc.sweep_pairs(case))
+        assert np.array_equal(got, want), (len(case), len(got), len(want))
+
+
+@pytest.mark.parametrize("bin_target", [200, 3500])
+def test_general_schedule_large_boxes_and_strips(torch_cuda, binned, orc, bin_target):
+    """Under the general test schedule a box covering more than 16 fine cells
+    of a bin is listed instead of replicated and tested against every
+    membership of the bin (up to 256 listed per bin; the rest replicated),
+    and bins beyond one strip lay out each strip by cell size.  30,000 small
+    boxes in a 1,000-unit domain at cell 4 (dense cells, several strips per
+    bin), 600 boxes 20 to 400 units wide (whole bins covered, listed boxes
+    meeting each other with their reference cell in another bin, more than
+    256 in some bins) and boxes west of and south of the domain: the pair
+    set must be the oracle's."""
+    from pbig_amd import workloads
+
+    rng = np.random.default_rng(41)
+    small = workloads.uniform(30000, 1000.0, 1.0, 6.0, 43)
+    n = 1500
+    x0 = rng.uniform(-50.0, 1000.0, n)
+    y0 = rng.uniform(-50.0, 1000.0, n)
+    w = rng.uniform(20.0, 400.0, n)
+    h = rng.uniform(20.0, 400.0, n)
+    large = np.stack([x0, x0 + w, y0, y0 + h], 1).astype(np.float32)
+    boxes = np.concatenate([small, large]).astype(np.float32)
+    binned.configure(PATHS["binned"], bin_target)
+    got = run_device(torch_cuda, binned, boxes, 1000.0, 4.0)
+    assert binned.stats()["path"] == PATHS["binned"]
+    assert np.array_equal(got, orc.sort_pairs(orc.sweep_pairs(boxes)))
 
 
 @pytest.mark.parametrize("bin_target", [16, 200, 3500, 5000])
