@@ -150,6 +150,8 @@ struct BinnedWorkspace {
   uint32_t* total = nullptr;      // per bin: memberships
   uint32_t* bin_start = nullptr;  // per bin: exclusive scan of total
   uint32_t* overfull = nullptr;   // bins that did not fit in LDS
+  uint32_t* order = nullptr;      // the test kernel's bin order (k_bin_starts; two-pass launches)
+  uint32_t* rank = nullptr;       // each bin's entry in that order
   uint2* bin_seg = nullptr;       // per bin: (first pair in its workgroup's region, pairs)
   uint64_t* bin_out = nullptr;    // per bin + 1: output position of its pairs
   uint2* scratch = nullptr;       // test_blocks x cap_wg pair regions

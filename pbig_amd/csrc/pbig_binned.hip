@@ -399,12 +399,23 @@ __device__ __forceinline__ uint64_t block_exscan(uint32_t n, ScanLds& L, Get get
 // schedule holds: kNarrowMax memberships when no box is wide, else
 // kMaxCoarse) hold more than a quarter of its memberships is flagged
 // kErrDense, and the scatter and test kernels then do nothing.
+// With `order`, also the test kernel's bin order: the bins in four tiers by
+// memberships (the schedule's overfull bins, which the test only lists, in
+// the last), heaviest first, in bin order inside a tier, so that the test
+// workgroups, which take the order in snake rounds, each get a share of the
+// heavy bins, and every launch of the same input the same share.
+__device__ __forceinline__ uint32_t test_tier(uint32_t m, uint32_t limit) {
+  return m > limit ? 3u : (m >= 2048u ? 0u : (m >= 512u ? 1u : (m >= 64u ? 2u : 3u)));
+}
+
 template <bool kDev>
 __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict__ total, uint32_t n_bins,
                                                      uint32_t* __restrict__ bin_start, uint64_t cap,
-                                                     uint32_t dense_exit, Meta* __restrict__ meta) {
+                                                     uint32_t dense_exit, Meta* __restrict__ meta,
+                                                     uint32_t* __restrict__ order, uint32_t* __restrict__ rank) {
   __shared__ ScanLds L;
   __shared__ unsigned long long heavy;
+  __shared__ uint32_t tier_w[16][4];
   if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (threadIdx.x == 0) heavy = 0ull;
   const uint64_t grand = block_exscan(
@@ -426,6 +437,39 @@ __global__ __launch_bounds__(1024) void k_bin_starts(const uint32_t* __restrict_
     if (grand > cap) atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrWorkspace);
     if (dense_exit && 4ull * heavy > grand)
       atomicOr((unsigned long long*)&meta->error, (unsigned long long)kErrDense);
+  }
+  if (order) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t limit = general_schedule(meta) ? kMaxCoarse : kNarrowMax;
+    const uint32_t per = (n_bins + 1023u) / 1024u;
+    const uint32_t i0 = tid * per < n_bins ? tid * per : n_bins;
+    const uint32_t i1 = i0 + per < n_bins ? i0 + per : n_bins;
+    uint32_t c[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t i = i0; i < i1; ++i) ++c[test_tier(total[i], limit)];
+    uint32_t at[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t inc = wave_prefix_sum(c[k]);
+      if (lane == 63) tier_w[w][k] = inc;
+      at[k] = inc - c[k];
+    }
+    __syncthreads();
+    uint32_t base = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t before = 0, all = 0;
+      for (uint32_t v = 0; v < 16; ++v) {
+        before += v < w ? tier_w[v][k] : 0u;
+        all += tier_w[v][k];
+      }
+      at[k] += base + before;
+      base += all;
+    }
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t r = at[test_tier(total[i], limit)]++;
+      order[r] = i;
+      rank[i] = r;
+    }
   }
 }
 
@@ -1090,6 +1134,26 @@ __device__ void single_pass_dense_rule(const uint32_t* __restrict__ total, uint3
   }
 }
 
+// The bins of this test workgroup: with an order (k_bin_starts), round t
+// takes entry t * grid + (t even: blockIdx.x, odd: grid - 1 - blockIdx.x);
+// without, bin blockIdx.x + t * grid.  test_wg: the workgroup (so the pair
+// region) of bin gb, from its rank in the order.
+__device__ __forceinline__ uint32_t test_bin_count(uint32_t n_bins, const uint32_t* order) {
+  const uint32_t G = gridDim.x, w = blockIdx.x;
+  if (!order) return w < n_bins ? (n_bins - w + G - 1u) / G : 0u;
+  const uint32_t full = n_bins / G, rem = n_bins % G;
+  return full + (((full & 1u) ? G - 1u - w : w) < rem ? 1u : 0u);
+}
+__device__ __forceinline__ uint32_t test_bin(uint32_t t, const uint32_t* order) {
+  const uint32_t G = gridDim.x, w = blockIdx.x;
+  return order ? order[t * G + ((t & 1u) ? G - 1u - w : w)] : w + t * G;
+}
+__device__ __forceinline__ uint32_t test_wg(uint32_t gb, uint32_t nwg, const uint32_t* rank) {
+  if (!rank) return gb % nwg;
+  const uint32_t idx = rank[gb], t = idx / nwg, pos = idx % nwg;
+  return (t & 1u) ? nwg - 1u - pos : pos;
+}
+
 __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float4* __restrict__ bbox,
                                                            const uint32_t* __restrict__ bid,
                                                            const uint32_t* __restrict__ bin_start,
@@ -1101,13 +1165,13 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
                                                            uint32_t* __restrict__ overfull,
                                                            uint2* __restrict__ out, uint64_t cap,
                                                            unsigned long long* out_count, Meta* __restrict__ meta,
-                                                           uint32_t single) {
+                                                           uint32_t single, const uint32_t* __restrict__ order) {
   TestLds& L = *reinterpret_cast<TestLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  const uint32_t my_bins = test_bin_count(n_bins, order);
   const unsigned long long err = meta->error;
   if (err & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
-    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
+    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[test_bin(t, order)] = make_uint2(0u, 0u);
     if (single) single_pass_dense_rule(total, n_bins, single, err, meta);
     return;
   }
@@ -1136,7 +1200,7 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
       // their instance's grid.
       __syncthreads();
       for (uint32_t u = tid; u < kMaxBinsPerGroup && t + u < my_bins; u += kTestThreads) {
-        const uint32_t gb = blockIdx.x + (t + u) * gridDim.x;
+        const uint32_t gb = test_bin(t + u, order);
         const InstDesc I = inst[bin_inst[gb]];
         const uint32_t lb = gb - I.bin0;
         const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
@@ -1161,12 +1225,13 @@ __device__ __forceinline__ void test_general(unsigned char* lds_raw, const float
         }
       }
     }
-    const uint32_t b = blockIdx.x + t * gridDim.x;
+    const uint32_t b = test_bin(t, order);
+    const uint32_t bN = t + 1 < my_bins ? test_bin(t + 1, order) : 0u;
     const uint32_t m = L.binfo[slot].y;
     // Next bin's registers: entries past its end repeat its last one (same
     // cache line, no extra HBM traffic) so that every load is unconditional.
-    const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
-    const uint32_t mN = t + 1 < my_bins ? total[b + gridDim.x] : 0u;
+    const uint64_t sN = t + 1 < my_bins ? bin_start[bN] : 0ull;
+    const uint32_t mN = t + 1 < my_bins ? total[bN] : 0u;
     const uint32_t jN = mN ? mN - 1u : 0u;
     const uint4 bg = L.bingrid[slot];
     const uint32_t gx = bg.x, gy = bg.y, Kx = bg.z & 0xFFFFu, Ky = bg.z >> 16, kk = Kx * Ky;
@@ -1527,14 +1592,15 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
                                                                   uint32_t* __restrict__ overfull,
                                                                   uint2* __restrict__ out, uint64_t cap,
                                                                   unsigned long long* out_count,
-                                                                  Meta* __restrict__ meta, uint32_t single) {
+                                                                  Meta* __restrict__ meta, uint32_t single,
+                                                                  const uint32_t* __restrict__ order) {
   NarrowLds& L = *reinterpret_cast<NarrowLds*>(lds_raw);
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
-  const uint32_t my_bins = blockIdx.x < n_bins ? (n_bins - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  const uint32_t my_bins = test_bin_count(n_bins, order);
   const unsigned long long err = meta->error;
   if (err & (kErrInvalidBox | kErrWorkspace | kErrDense | kErrBinCap)) {
-    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[blockIdx.x + t * gridDim.x] = make_uint2(0u, 0u);
+    for (uint32_t t = tid; t < my_bins; t += kTestThreads) bin_seg[test_bin(t, order)] = make_uint2(0u, 0u);
     if (single) single_pass_dense_rule(total, n_bins, single, err, meta);
     return;
   }
@@ -1556,7 +1622,7 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
     if (slot == 0) {
       __syncthreads();
       for (uint32_t u = tid; u < kMaxBinsPerGroup && t + u < my_bins; u += kTestThreads) {
-        const uint32_t gb = blockIdx.x + (t + u) * gridDim.x;
+        const uint32_t gb = test_bin(t + u, order);
         const InstDesc I = inst[bin_inst[gb]];
         const uint32_t lb = gb - I.bin0;
         const uint32_t gx = I.fx0 + (lb % I.nbx) * I.k, gy = I.fy0 + (lb / I.nbx) * I.k;
@@ -1581,10 +1647,11 @@ __device__ __forceinline__ void test_narrow(unsigned char* lds_raw, const float4
         }
       }
     }
-    const uint32_t b = blockIdx.x + t * gridDim.x;
+    const uint32_t b = test_bin(t, order);
+    const uint32_t bN = t + 1 < my_bins ? test_bin(t + 1, order) : 0u;
     const uint32_t m = L.binfo[slot].y;
-    const uint64_t sN = t + 1 < my_bins ? bin_start[b + gridDim.x] : 0ull;
-    const uint32_t mN = t + 1 < my_bins ? total[b + gridDim.x] : 0u;
+    const uint64_t sN = t + 1 < my_bins ? bin_start[bN] : 0ull;
+    const uint32_t mN = t + 1 < my_bins ? total[bN] : 0u;
     const uint32_t jN = mN ? mN - 1u : 0u;
     const uint4 bg = L.bingrid[slot];
     const int gx = (int)bg.x, gy = (int)bg.y;
@@ -1837,7 +1904,8 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
                                                            uint32_t cap_wg, uint2* __restrict__ bin_seg,
                                                            uint32_t* __restrict__ overfull,
                                                            uint2* __restrict__ out, uint64_t cap, uint32_t direct,
-                                                           Meta* __restrict__ meta, uint32_t single = 0u) {
+                                                           Meta* __restrict__ meta, uint32_t single,
+                                                           const uint32_t* __restrict__ order) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   // direct output: the launch's pair counter (a single instance's own, or
   // the one its device-planned levels share)
@@ -1846,10 +1914,10 @@ __global__ __launch_bounds__(kTestThreads) void k_bin_test(const float4* __restr
   if (kDev) n_bins = meta->plan_err ? 0u : meta->dev_bins;
   if (general_schedule(meta))
     test_general(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                 overfull, out, cap, out_count, meta, single);
+                 overfull, out, cap, out_count, meta, single, order);
   else
     test_narrow(lds_raw, bbox, bid, bin_start, total, inst, bin_inst, n_bins, cap_bucket, scratch, cap_wg, bin_seg,
-                overfull, out, cap, out_count, meta, single);
+                overfull, out, cap, out_count, meta, single, order);
 }
 
 // The pair count and error word, copied to the caller's two result words
@@ -1887,7 +1955,8 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
                                                       const uint32_t* __restrict__ bin_inst, uint32_t n_inst,
                                                       uint2* __restrict__ out, uint64_t cap,
                                                       uint64_t* __restrict__ inst_out,
-                                                      const uint64_t* __restrict__ merge) {
+                                                      const uint64_t* __restrict__ merge,
+                                                      const uint32_t* __restrict__ rank) {
   if (inst_out) {
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_inst; i += gridDim.x * 256) {
       const InstDesc I = inst[i];
@@ -1901,7 +1970,7 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
   const uint2 seg = bin_seg[gb];
   const uint32_t gi = bin_inst[gb];
   const uint64_t dst0 = bin_out[gb] + inst[gi].pairs0 + (merge ? merge[n_inst + gi] : 0ull);
-  const uint2* src = scratch + (size_t)(gb % nwg) * cap_wg;
+  const uint2* src = scratch + (size_t)test_wg(gb, nwg, rank) * cap_wg;
   // a region that overflowed holds only its first cap_wg pairs (the host
   // regrows and runs again)
   const uint32_t n = seg.x < cap_wg ? (seg.y < cap_wg - seg.x ? seg.y : cap_wg - seg.x) : 0u;
@@ -2061,7 +2130,7 @@ hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins
   k_bin_hist<true><<<nc, kHistThreads, 0, s>>>(parent.bbox, ws.inst, ws.chunk_inst, 0u, ws.hist, ws.meta);
   k_bin_scan<true><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(
       ws.hist, ws.inst, ws.bin_inst, 0u, ws.chunk_inst, 0u, ws.off, ws.total, ws.meta);
-  k_bin_starts<true><<<1, 1024, 0, s>>>(ws.total, 0u, ws.bin_start, ws.cap_bucket, 0u, ws.meta);
+  k_bin_starts<true><<<1, 1024, 0, s>>>(ws.total, 0u, ws.bin_start, ws.cap_bucket, 0u, ws.meta, ws.order, ws.rank);
   const uint32_t per_xcd = (nc + 7u) / 8u < n_cu / 8u ? (nc + 7u) / 8u : (n_cu >= 8u ? n_cu / 8u : 1u);
   const size_t scatter_lds = sizeof(ScatterLds) + 8u * (caps.nb_max ? caps.nb_max : 1u);
   k_bin_scatter<true, true><<<per_xcd * 8u, kScatterThreads, scatter_lds, s>>>(
@@ -2069,7 +2138,7 @@ hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins
       ws.cap_bucket, caps.nb_max ? caps.nb_max : 1u, ws.total, 0u, 0u, ws.meta);
   k_bin_test<true><<<n_cu ? n_cu : 1u, kTestThreads, kTestLdsBytes, s>>>(
       ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, ws.bin_inst, 0u, ws.cap_bucket, ws.scratch, ws.cap_wg,
-      ws.bin_seg, ws.overfull, out, cap, direct, ws.meta);
+      ws.bin_seg, ws.overfull, out, cap, direct, ws.meta, 0u, ws.order);
   return hipGetLastError();
 }
 
@@ -2122,7 +2191,8 @@ __global__ __launch_bounds__(256) void k_level_copy(const uint2* __restrict__ sc
                                                     const InstDesc* __restrict__ inst0,
                                                     const uint64_t* __restrict__ bin_out0,
                                                     uint64_t* __restrict__ merge, uint32_t n_inst,
-                                                    uint2* __restrict__ out, uint64_t cap) {
+                                                    uint2* __restrict__ out, uint64_t cap,
+                                                    const uint32_t* __restrict__ rank) {
   __shared__ uint64_t dst0;
   const uint32_t gb = blockIdx.x;
   const uint32_t n_bins = meta->plan_err ? 0u : meta->dev_bins;
@@ -2137,7 +2207,7 @@ __global__ __launch_bounds__(256) void k_level_copy(const uint2* __restrict__ sc
            atomicAdd((unsigned long long*)&merge[2ull * n_inst + r], (unsigned long long)n);
   }
   __syncthreads();
-  const uint2* src = scratch + (size_t)(gb % nwg) * cap_wg + seg.x;
+  const uint2* src = scratch + (size_t)test_wg(gb, nwg, rank) * cap_wg + seg.x;
   for (uint32_t k = threadIdx.x; k < n; k += 256) {
     const uint64_t d = dst0 + k;
     if (d < cap) out[d] = src[k];
@@ -2167,7 +2237,7 @@ hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint3
                              uint2* out, uint64_t cap, hipStream_t s) {
   k_level_copy<<<max_bins ? max_bins : 1u, 256, 0, s>>>(lv.scratch, lv.cap_wg, lv_test_blocks ? lv_test_blocks : 1u,
                                                         lv.bin_seg, lv.inst, lv.bin_inst, lv.meta, ws0.inst, bin_out0,
-                                                        merge, n_inst, out, cap);
+                                                        merge, n_inst, out, cap, lv.rank);
   return hipGetLastError();
 }
 
@@ -2498,13 +2568,19 @@ hipError_t launch_single_tables(const BinnedWorkspace& ws, const InstDesc& d, hi
   return hipGetLastError();
 }
 
+// A single instance with at most kFoldBins bins: the scatter scans the bin
+// totals itself (no k_bin_starts), and the test walks the bins in plain
+// order.
+static bool launch_folds(const BatchPlan& bp) { return bp.n_inst == 1u && bp.nb_max <= kFoldBins; }
+
 hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, uint2* out, uint64_t cap,
                                 uint64_t* inst_out, hipStream_t s, const uint64_t* merge) {
   const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;
   k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total,
                                      merge ? merge + 3ull * bp.n_inst : nullptr, ws.meta);
   k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
-                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, merge);
+                                      ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, merge,
+                                      launch_folds(bp) ? nullptr : ws.rank);
   return hipGetLastError();
 }
 
@@ -2530,7 +2606,7 @@ hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, c
   k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(
       ws.bbox, ws.bid, op.bin_start, op.fill, ws.inst, ws.bin_inst, bp.n_bins, (uint64_t)bp.n_bins * op.bin_cap,
       ws.scratch, ws.cap_wg, ws.bin_seg, ws.overfull, out, cap, 1u, op.meta,
-      kSingleSum | (op.dense_exit ? kSingleDense : 0u));
+      kSingleSum | (op.dense_exit ? kSingleDense : 0u), nullptr);
   PBIG_MARK(2)
 #undef PBIG_MARK
   return hipGetLastError();
@@ -2553,8 +2629,10 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   k_bin_scan<false><<<(nbins + kScanBins - 1) / kScanBins + 1, 1024, 0, s>>>(ws.hist, ws.inst, ws.bin_inst, bp.n_bins, ws.chunk_inst,
                                                bp.n_chunks, ws.off, ws.total, ws.meta);
   PBIG_MARK(2)
-  const uint32_t fold = bp.n_inst == 1u && bp.nb_max <= kFoldBins ? 1u : 0u;
-  if (!fold) k_bin_starts<false><<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta);
+  const uint32_t fold = launch_folds(bp) ? 1u : 0u;
+  if (!fold)
+    k_bin_starts<false><<<1, 1024, 0, s>>>(ws.total, bp.n_bins, ws.bin_start, ws.cap_bucket, bp.dense_exit, ws.meta,
+                                           ws.order, ws.rank);
   PBIG_MARK(3)
   const uint32_t nb_cap = bp.nb_max ? bp.nb_max : 1u;
   const size_t scatter_lds = sizeof(ScatterLds) + (fold ? 12u : 8u) * nb_cap;
@@ -2575,14 +2653,16 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
-                                                                   ws.overfull, out, cap, bp.direct, ws.meta);
+                                                                   ws.overfull, out, cap, bp.direct, ws.meta, 0u,
+                                                                   fold ? nullptr : ws.order);
   PBIG_MARK(5)
   if (!bp.direct && output)
     k_bin_pair_scan<<<1, 1024, 0, s>>>(ws.bin_seg, bp.n_bins, ws.bin_out, bp.extra_total, nullptr, ws.meta);
   PBIG_MARK(6)
   if (!bp.direct && output)
     k_bin_compact<<<nbins, 256, 0, s>>>(ws.scratch, ws.cap_wg, bp.test_blocks, ws.bin_seg, ws.bin_out, bp.n_bins,
-                                        ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, nullptr);
+                                        ws.inst, ws.bin_inst, bp.n_inst, out, cap, inst_out, nullptr,
+                                        fold ? nullptr : ws.rank);
   PBIG_MARK(7)
 #undef PBIG_MARK
   return hipGetLastError();

@@ -451,17 +451,19 @@ int ensure_state_(pbig_hip_ctx* c, BinnedState& st, uint64_t bucket) {
   if ((rc = grow(c, &w.off, &w.cap_off, std::max<uint64_t>(bp.off_entries, 1)))) return rc;
   const uint64_t nb = (uint64_t)bp.n_bins + 1;
   if (w.cap_bins < nb || !w.total) {
-    for (void** q : {(void**)&w.total, (void**)&w.bin_start, (void**)&w.overfull, (void**)&w.bin_seg,
-                     (void**)&w.bin_out})
+    for (void** q : {(void**)&w.total, (void**)&w.bin_start, (void**)&w.overfull, (void**)&w.order,
+                     (void**)&w.rank, (void**)&w.bin_seg, (void**)&w.bin_out})
       if (*q) HIP_TRY(c, hipFree(*q));
-    w.total = w.bin_start = w.overfull = nullptr;
+    w.total = w.bin_start = w.overfull = w.order = w.rank = nullptr;
     w.bin_seg = nullptr;
     w.bin_out = nullptr;
     w.cap_bins = 0;
-    uint64_t d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
+    uint64_t d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0, d7 = 0;
     if ((rc = grow(c, &w.total, &d1, nb))) return rc;
     if ((rc = grow(c, &w.bin_start, &d2, nb))) return rc;
     if ((rc = grow(c, &w.overfull, &d3, nb))) return rc;
+    if ((rc = grow(c, &w.order, &d6, nb))) return rc;
+    if ((rc = grow(c, &w.rank, &d7, nb))) return rc;
     if ((rc = grow(c, &w.bin_seg, &d4, nb))) return rc;
     if ((rc = grow(c, &w.bin_out, &d5, nb))) return rc;
     w.cap_bins = nb;
@@ -522,7 +524,8 @@ void free_state(BinnedState& st) {
   st.cap_sub_out = st.cap_sub_io = st.cap_io_buf = st.cap_runs = 0;
   const BinnedWorkspace& b = st.ws;
   for (void* p : {(void*)b.inst, (void*)b.chunk_inst, (void*)b.bin_inst, (void*)b.hist, (void*)b.off,
-                  (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.bin_seg, (void*)b.bin_out,
+                  (void*)b.total, (void*)b.bin_start, (void*)b.overfull, (void*)b.order, (void*)b.rank,
+                  (void*)b.bin_seg, (void*)b.bin_out,
                   (void*)b.scratch, (void*)b.bbox, (void*)b.bid})
     if (p) (void)hipFree(p);
   if (st.own_meta && b.meta) (void)hipFree(b.meta);
@@ -1654,6 +1657,7 @@ int pbig_hip_check_device_noalloc(pbig_hip_ctx* c, const float* d_boxes, uint32_
   const uint64_t o_hist = take(4 * std::max<uint64_t>(bp.hist_entries, 1));
   const uint64_t o_off = take(4 * std::max<uint64_t>(bp.off_entries, 1));
   const uint64_t o_total = take(4 * nb1), o_start = take(4 * nb1), o_over = take(4 * nb1);
+  const uint64_t o_order = take(4 * nb1), o_rank = take(4 * nb1);
   const uint64_t o_seg = take(8 * nb1), o_out = take(8 * nb1);
   const uint64_t cap_wg = std::max<uint64_t>(8192, ((uint64_t)n + bp.test_blocks - 1) / bp.test_blocks * 2);
   const uint64_t o_reg = take(8 * cap_wg * bp.test_blocks);
@@ -1676,6 +1680,8 @@ int pbig_hip_check_device_noalloc(pbig_hip_ctx* c, const float* d_boxes, uint32_
   ws.total = (uint32_t*)(base + o_total);
   ws.bin_start = (uint32_t*)(base + o_start);
   ws.overfull = (uint32_t*)(base + o_over);
+  ws.order = (uint32_t*)(base + o_order);
+  ws.rank = (uint32_t*)(base + o_rank);
   ws.bin_seg = (uint2*)(base + o_seg);
   ws.bin_out = (uint64_t*)(base + o_out);
   ws.scratch = (uint2*)(base + o_reg);
