@@ -81,6 +81,7 @@ struct BinnedState {
   pbig::DeepCaps dcaps[pbig::kDeepLevels];
   bool deep_ok = false, deep_failed = false;
   bool to_general = false;  // the automatic choice sends this plan's instance to the general path
+  bool dense_levels = false;  // ... or, mostly dense clusters, to the device levels (level 0 without the dense exit)
   uint64_t* d_res = nullptr;  // result words of a synchronous call on them
   // Bins no level resolves, tested whole: their list, status words, and (a
   // batch) the pairs and runs they produce for the merge.
@@ -615,7 +616,7 @@ int plan_plain(pbig_hip_ctx* c, uint32_t n, double range, float cell, hipStream_
                      : 0u;
     st.onepass_failed = false;
     st.deep_ok = st.deep_failed = false;
-    st.to_general = false;
+    st.to_general = st.dense_levels = false;
   }
   st.ws.meta = c->ws.meta;
   int rc;
@@ -881,8 +882,12 @@ int enqueue_deep(pbig_hip_ctx* c, BinnedState& st, const float4* d_boxes, uint2*
                  uint64_t* d_result, hipStream_t s) {
   hipEvent_t* ev = next_events(c, kPathDeep);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
-  st.bp.direct = 1u;  // (dense_exit as the caller set it: a dense instance stops after level 0's bin totals)
-  HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, nullptr, st.bp, d_out, cap, nullptr, s, nullptr));
+  st.bp.direct = 1u;
+  // dense_exit as the caller set it (a dense instance stops after level 0's
+  // bin totals), unless the automatic choice sent a dense instance here
+  pbig::BatchPlan bp0 = st.bp;
+  if (st.dense_levels) bp0.dense_exit = 0u;
+  HIP_TRY(c, pbig::launch_binned(st.ws, d_boxes, nullptr, bp0, d_out, cap, nullptr, s, nullptr));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
   HIP_TRY(c, pbig::launch_dense_clear(st.dense_meta, s));
   const BinnedWorkspace* parent = &st.ws;
@@ -1226,18 +1231,29 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
   BinnedState& st = c->plain;
   const bool auto_choice = st.bp.dense_exit != 0;
   if (st.to_general && auto_choice) return 1;  // (known from this plan's last call)
+  // An instance of mostly dense clusters goes to the device levels when it
+  // has levels_min_boxes or more (level 0 then runs without the dense exit),
+  // else to the general path.
+  const bool dense_to_levels = auto_choice && n >= c->cfg.levels_min_boxes && !c->cfg.host_levels;
   Meta m;
   int rc = run_onepass_sync(c, d_boxes, d_out, cap, s, &m);
   if (rc < 0) return rc;
-  if (rc == 2) return 1;  // mostly dense clusters: the general path
-  if (rc == 1 && auto_choice && !st.deep_ok) {
+  if (rc == 2) {
+    if (!dense_to_levels) return 1;  // the general path
+    st.to_general = false;
+    st.dense_levels = true;
+    rc = 1;
+  }
+  if (rc == 1 && auto_choice && !st.deep_ok && !st.dense_levels) {
     // The automatic choice runs the two-pass launch alone first: its bin
-    // totals send an instance of mostly dense clusters to the general path
-    // (which stops the launch before it scatters), and an instance with no
-    // bin beyond LDS is done.  Bins beyond LDS go to the device levels in
-    // instances of levels_min_boxes or more, else to the general path.
+    // totals find an instance of mostly dense clusters (which stops the
+    // launch before it scatters), and an instance with no bin beyond LDS is
+    // done.  Bins beyond LDS go to the device levels in instances of
+    // levels_min_boxes or more, else to the general path.
     if ((rc = run_state_sync(c, st, d_boxes, nullptr, d_out, cap, nullptr, s, &m))) return rc;
-    if ((m.error & pbig::kErrDense) || (m.error && n < c->cfg.levels_min_boxes)) {
+    if ((m.error & pbig::kErrDense) && dense_to_levels) {
+      st.dense_levels = true;
+    } else if ((m.error & pbig::kErrDense) || (m.error && n < c->cfg.levels_min_boxes)) {
       st.to_general = true;
       return 1;
     }
@@ -1246,6 +1262,11 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
   if (rc == 1) rc = run_deep_sync(c, d_boxes, d_out, cap, s, &m);  // re-binning levels on the device
   if (rc < 0) return rc;
   if (rc == 2) return 1;  // mostly dense clusters: the general path
+  if (rc == 1 && st.dense_levels) {  // the levels cannot complete this plan: the general path
+    st.dense_levels = false;
+    st.to_general = true;
+    return 1;
+  }
   if (rc == 1) {  // the two-pass launch, with re-binning of bins beyond LDS on the host
     Splice sp;
     std::vector<uint8_t> failed;

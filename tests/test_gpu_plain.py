@@ -384,7 +384,7 @@ def test_binned_c3_rebins_heavy_bins(torch_cuda, binned, orc, manifest, cs):
     assert orc.pair_hash(got) == m["sha256"]
 
 
-def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors):
+def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors, levels_min=0):
     import pbig_amd
     from pbig_amd import workloads
 
@@ -394,7 +394,7 @@ def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors):
     out = torch_cuda.empty((m["pairs"] + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
     res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
     with pbig_amd.Engine(0) as e:
-        e.configure(PATHS[path], 2000)
+        e.configure(PATHS[path], 2000, levels_min)
         e.check_device_async(d, L, 64.0, out, res)
         torch_cuda.cuda.synchronize()
         n_async, err = (int(v) for v in res.cpu().tolist())
@@ -412,17 +412,19 @@ def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors):
     return settled
 
 
-def test_auto_dense_instance_goes_to_the_general_path(torch_cuda, orc, manifest):
+@pytest.mark.parametrize("levels_min", [0, 1 << 22])
+def test_auto_dense_instance_settles(torch_cuda, orc, manifest, levels_min):
     """The clustered instance under the automatic choice: the first
     asynchronous call tries the single-pass launch, whose dense bins outgrow
     their fixed share of the bucket; it stops with error bit 5 and no pairs,
     and with bit 4 too, since the bins' totals (counted to the end) put over
     a quarter of the memberships in bins beyond LDS (mostly dense clusters).
-    The synchronous call then sends the instance to the general path, and
-    the next asynchronous calls follow it, with the exact set."""
-    settled = _dense_instance_settles(torch_cuda, orc, manifest, "auto", (16, 32, 48))
-    print("settled path", settled)
-    assert settled == PATHS["general"]
+    The synchronous call then sends the instance (2^21 boxes) to the device
+    levels at the default levels_min_boxes (2^20), or to the general path
+    when levels_min_boxes is above its size, and the next asynchronous calls
+    follow it, with the exact set."""
+    settled = _dense_instance_settles(torch_cuda, orc, manifest, "auto", (16, 32, 48), levels_min)
+    assert settled == (PATHS["binned"] if levels_min == 0 else PATHS["general"])
 
 
 def test_binned_dense_instance_on_device_levels(torch_cuda, orc, manifest):
