@@ -2019,19 +2019,24 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
   // 0. the listed bins in increasing order: the test kernel appends them in
   //    the order its workgroups finish, and the plan (so every bin's
   //    workgroup and every workgroup's pair count) must be the same in each
-  //    launch of the same input.  Rank of each entry by a broadcast walk.
-  for (uint32_t i = tid; i < n_inst; i += 1024) c0[i] = poverfull[i];
+  //    launch of the same input.  Bitonic sort in LDS, padded to a power of
+  //    two with entries that sort last.
+  const uint32_t np2 = n_inst <= 1u ? 1u : 1u << (32 - __builtin_clz(n_inst - 1u));
+  for (uint32_t i = tid; i < np2; i += 1024) ord[i] = i < n_inst ? poverfull[i] : 0xFFFFFFFFu;
   __syncthreads();
-  for (uint32_t i = tid; i < n_inst; i += 1024) {
-    const uint32_t v = c0[i];
-    uint32_t r = 0;
-    for (uint32_t j = 0; j < n_inst; ++j) {
-      const uint32_t u = c0[j];
-      r += (u < v || (u == v && j < i)) ? 1u : 0u;
+  for (uint32_t size = 2; size <= np2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = tid; t < np2 / 2u; t += 1024) {
+        const uint32_t lo = 2u * t - (t & (stride - 1u)), hi = lo + stride;
+        const uint32_t a = ord[lo], b = ord[hi];
+        if ((a > b) == ((lo & size) == 0u)) {
+          ord[lo] = b;
+          ord[hi] = a;
+        }
+      }
+      __syncthreads();
     }
-    ord[r] = v;
   }
-  __syncthreads();
   // 1. every listed bin's virtual instance
   for (uint32_t i = tid; i < n_inst; i += 1024) {
     const uint32_t b = ord[i] & ~kOverfullCell;
@@ -2073,18 +2078,14 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
   __syncthreads();
   const bool fits = n_inst == n_over && chunks <= caps.chunks && bins <= caps.bins && hist <= caps.hist &&
                     off <= caps.off && nbmax <= caps.nb_max;
-  // 3. chunk -> instance and bin -> instance (binary search of the starts)
+  // 3. chunk -> instance and bin -> instance: a wave per instance writes
+  //    its ranges
   if (fits) {
-    auto owner = [&](const uint32_t* st, uint32_t x) {
-      uint32_t lo = 0, hi = n_inst;  // st[lo] <= x < st[hi]
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (st[mid] <= x) lo = mid; else hi = mid;
-      }
-      return lo;
-    };
-    for (uint32_t c = tid; c < (uint32_t)chunks; c += 1024) chunk_inst[c] = owner(c0, c);
-    for (uint32_t b = tid; b < (uint32_t)bins; b += 1024) bin_inst[b] = owner(b0, b);
+    const uint32_t wv = tid >> 6, ln = tid & 63u;
+    for (uint32_t i = wv; i < n_inst; i += 1024u / 64u) {
+      for (uint32_t c = c0[i] + ln; c < c0[i + 1u]; c += 64u) chunk_inst[c] = i;
+      for (uint32_t b = b0[i] + ln; b < b0[i + 1u]; b += 64u) bin_inst[b] = i;
+    }
   }
   if (tid == 0) {
     meta->dev_inst = n_over;
@@ -2181,8 +2182,8 @@ __global__ __launch_bounds__(1024) void k_merge_scan(uint64_t* __restrict__ merg
   if (threadIdx.x == 0) merge[3ull * n_inst] = total;
 }
 
-// One block per level bin: its run goes after its root's level-0 pairs (and
-// after the runs of the root's level bins placed before it).
+// A block per level bin in turn: the bin's run goes after its root's level-0
+// pairs (and after the runs of the root's level bins placed before it).
 __global__ __launch_bounds__(256) void k_level_copy(const uint2* __restrict__ scratch, uint32_t cap_wg, uint32_t nwg,
                                                     const uint2* __restrict__ bin_seg,
                                                     const InstDesc* __restrict__ inst,
@@ -2194,23 +2195,25 @@ __global__ __launch_bounds__(256) void k_level_copy(const uint2* __restrict__ sc
                                                     uint2* __restrict__ out, uint64_t cap,
                                                     const uint32_t* __restrict__ rank) {
   __shared__ uint64_t dst0;
-  const uint32_t gb = blockIdx.x;
   const uint32_t n_bins = meta->plan_err ? 0u : meta->dev_bins;
-  if (gb >= n_bins) return;
-  const uint2 seg = bin_seg[gb];
-  const uint32_t n = region_run(seg, cap_wg);
-  if (!n) return;
-  if (threadIdx.x == 0) {
-    const uint32_t r = inst[bin_inst[gb]].root;
-    const InstDesc R = inst0[r];
-    dst0 = bin_out0[R.bin0 + R.nb] + R.pairs0 + merge[n_inst + r] +
-           atomicAdd((unsigned long long*)&merge[2ull * n_inst + r], (unsigned long long)n);
-  }
-  __syncthreads();
-  const uint2* src = scratch + (size_t)test_wg(gb, nwg, rank) * cap_wg + seg.x;
-  for (uint32_t k = threadIdx.x; k < n; k += 256) {
-    const uint64_t d = dst0 + k;
-    if (d < cap) out[d] = src[k];
+  for (uint32_t gb = blockIdx.x; gb < n_bins; gb += gridDim.x) {
+    const uint2 seg = bin_seg[gb];
+    const uint32_t n = region_run(seg, cap_wg);
+    if (!n) continue;
+    __syncthreads();  // dst0 of the block's previous bin has been read
+    if (threadIdx.x == 0) {
+      const uint32_t r = inst[bin_inst[gb]].root;
+      const InstDesc R = inst0[r];
+      dst0 = bin_out0[R.bin0 + R.nb] + R.pairs0 + merge[n_inst + r] +
+             atomicAdd((unsigned long long*)&merge[2ull * n_inst + r], (unsigned long long)n);
+    }
+    __syncthreads();
+    const uint64_t d0 = dst0;
+    const uint2* src = scratch + (size_t)test_wg(gb, nwg, rank) * cap_wg + seg.x;
+    for (uint32_t k = threadIdx.x; k < n; k += 256) {
+      const uint64_t d = d0 + k;
+      if (d < cap) out[d] = src[k];
+    }
   }
 }
 
@@ -2235,7 +2238,9 @@ hipError_t launch_merge_scan(uint64_t* merge, uint32_t n_inst, hipStream_t s) {
 hipError_t launch_level_copy(const BinnedWorkspace& lv, uint32_t max_bins, uint32_t lv_test_blocks,
                              const BinnedWorkspace& ws0, const uint64_t* bin_out0, uint64_t* merge, uint32_t n_inst,
                              uint2* out, uint64_t cap, hipStream_t s) {
-  k_level_copy<<<max_bins ? max_bins : 1u, 256, 0, s>>>(lv.scratch, lv.cap_wg, lv_test_blocks ? lv_test_blocks : 1u,
+  // a persistent grid: each block walks level bins gb = blockIdx.x, + grid, ...
+  const uint32_t grid = max_bins < 2048u ? (max_bins ? max_bins : 1u) : 2048u;
+  k_level_copy<<<grid, 256, 0, s>>>(lv.scratch, lv.cap_wg, lv_test_blocks ? lv_test_blocks : 1u,
                                                         lv.bin_seg, lv.inst, lv.bin_inst, lv.meta, ws0.inst, bin_out0,
                                                         merge, n_inst, out, cap, lv.rank);
   return hipGetLastError();
