@@ -38,6 +38,25 @@ def alg_bytes(N, C, M, P):
     }
 
 
+def reference_grid(boxes, L, cs):
+    """Cells C and memberships M of the reference's single-level grid at cell
+    size cs (cells per row ceil(L / cs); a coordinate's cell is
+    floor(v * (1/cs)) in float32, clamped into the grid), the terms that
+    price the algorithmic bytes: counted here with numpy from the boxes, the
+    same on every path."""
+    import numpy as np
+
+    nrc = max(1, int(np.ceil(float(L) / float(cs))))
+    inv = np.float32(1.0) / np.float32(cs)
+    b = np.asarray(boxes, dtype=np.float32)
+    M = 0
+    for i0 in range(0, b.shape[0], 1 << 20):
+        q = b[i0:i0 + (1 << 20)]
+        cell = lambda v: np.clip(np.floor(v * inv), 0, nrc - 1).astype(np.int64)
+        M += int(((cell(q[:, 1]) - cell(q[:, 0]) + 1) * (cell(q[:, 3]) - cell(q[:, 2]) + 1)).sum())
+    return nrc * nrc, M
+
+
 def cpu_baseline(boxes, L, cs, n_pairs):
     """CPU baselines on the host cores (about 20 s in all):
     * value: the oracle (CPU restatement of the reference's grid algorithm)
@@ -146,9 +165,9 @@ def clustered_leg(torch, eng_cls, steps=50, reps=5):
     e = eng_cls(0)
     e.configure(pbig_amd.PATH_GENERAL)
     k_ref = e.check_device(d, L, ref_cs, pairs)
-    st = e.stats()
     e.close()
-    B = alg_bytes(n, st["n_cells"], st["memberships"], k_ref)
+    C_ref, M_ref = reference_grid(boxes, L, ref_cs)
+    B = alg_bytes(n, C_ref, M_ref, k_ref)
     for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL),
                        ("auto", pbig_amd.PATH_AUTO)):
         e = eng_cls(0)
@@ -190,7 +209,7 @@ def clustered_leg(torch, eng_cls, steps=50, reps=5):
             assert res.cpu().tolist() == [k, 0]
             avg = {kk: v / calls for kk, v in per_kernel.items() if v > 0}
             kernel_alg = {"count": B["count"], "scatter": B["scatter"], "test": B["test"],
-                          "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * st["n_cells"],
+                          "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref,
                           "bin_hist": B["count"], "bin_scan": B["scan"], "bin_starts": 0.0,
                           "bin_scatter": B["scatter"], "bin_test": B["test"],
                           "bin_place": B["count"] + B["scan"] + B["scatter"]}
@@ -591,8 +610,7 @@ def main():
     # memberships M of the single-level grid), whatever cell size runs.
     eng.configure(pbig_amd.PATH_GENERAL)
     n_ref = eng.check_device(d_boxes, L, ref_cs, d_pairs)
-    st_ref = eng.stats()
-    C_ref, M_ref = st_ref["n_cells"], st_ref["memberships"]
+    C_ref, M_ref = reference_grid(boxes, L, ref_cs)
     # The measured configuration; this call also sizes the workspace.
     eng.configure(pbig_amd.PATH_AUTO if args.path == "auto" else
                   {"general": pbig_amd.PATH_GENERAL, "binned": pbig_amd.PATH_BINNED}[args.path], args.bin_target)

@@ -71,7 +71,12 @@ TIMING_ONEPASS = 3
 # ... and the two-pass launch followed by re-binning levels planned on the
 # device as path 4 (one duration per level).
 TIMING_DEEP = 4
-PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned", TIMING_ONEPASS: "onepass", TIMING_DEEP: "deep"}
+# ... and the general path's tile test on a cell layout built from the bins
+# (the binned count, scan and scatter, then a per-bin fine-cell sort) as
+# path 5.
+TIMING_SORTED = 5
+PATH_NAMES = {PATH_GENERAL: "general", PATH_BINNED: "binned", TIMING_ONEPASS: "onepass", TIMING_DEEP: "deep",
+              TIMING_SORTED: "sorted"}
 # Kernels of each path in launch order (pbig_hip_timing_read).
 KERNELS = {
     PATH_GENERAL: ("clear", "count", "scan_reduce", "scan_apply", "scatter", "test"),
@@ -79,6 +84,8 @@ KERNELS = {
                   "bin_compact"),
     TIMING_ONEPASS: ("bin_place", "bin_test"),
     TIMING_DEEP: ("level0", "level1", "level2", "level3"),
+    TIMING_SORTED: ("clear", "bin_hist", "bin_scan", "bin_starts", "bin_scatter", "fine_count", "fine_place",
+                    "test"),
 }
 
 # numpy view of the reference's RectInfo (src/pbig/rect.h:15-18, CType=float)

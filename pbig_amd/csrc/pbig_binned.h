@@ -205,6 +205,19 @@ hipError_t launch_binned_output(const BinnedWorkspace& ws, const BatchPlan& bp, 
 // The pair count and error word of a call (meta's first two words) copied
 // to the caller's two device words by one small kernel.
 hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s);
+// The count, scan and scatter kernels of launch_binned (events 0-4).
+hipError_t launch_binned_front(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
+                               const BatchPlan& bp, hipStream_t s, hipEvent_t* ev);
+// The general path's tile-pair test over a cell layout built from the bins
+// of one instance's two-pass launch (plan bp, with no dense exit): meta
+// cleared, count, scan and scatter into ws's bucket, k_fine_count (cell
+// starts in gw.cnt, bin ends in gw.start, items, strips; counts and errors
+// in meta, the bucket's in ws.meta) and k_fine_place into gw's memberships,
+// then k_test (pairs counted in meta).  gw.cnt holds bins x k^2 entries.
+// ev: kSortedKernels + 1 events.
+constexpr int kSortedKernels = 8;  // clear, bin_hist, bin_scan, bin_starts, bin_scatter, fine_count, fine_place, test
+hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, const Workspace& gw,
+                         Meta* meta, uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev);
 
 // Single-pass launch of one instance (no histogram or scan pass): every bin
 // owns bin_cap consecutive bucket entries (bin b at b * bin_cap); each place

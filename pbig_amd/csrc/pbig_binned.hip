@@ -2533,25 +2533,6 @@ hipError_t launch_splice_copy(const SpliceRun* runs, uint32_t n_runs, const uint
 constexpr uint32_t kPlaceLdsBytes = 156u * 1024u;
 static_assert(12u * kMaxBins + 64u + 24u * 1024u <= kPlaceLdsBytes, "a stage of 1,024 entries at the most bins");
 
-hipError_t binned_init() {
-  {
-    const hipError_t e0 = hipFuncSetAttribute((const void*)k_bin_place, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)kPlaceLdsBytes);
-    if (e0 != hipSuccess) return e0;
-  }
-  hipError_t e = hipFuncSetAttribute((const void*)k_bin_test<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)kTestLdsBytes);
-  if (e != hipSuccess) return e;
-  const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
-  for (const void* f : {(const void*)k_bin_test<true>, (const void*)k_bin_scatter<false, false>,
-                        (const void*)k_bin_scatter<true, false>, (const void*)k_bin_scatter<true, true>}) {
-    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            f == (const void*)k_bin_test<true> ? (int)kTestLdsBytes : scatter_max);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
 hipError_t launch_result(const Meta* meta, uint64_t* result, hipStream_t s) {
   k_bin_result<<<1, 64, 0, s>>>(meta, result);
   return hipGetLastError();
@@ -2617,9 +2598,10 @@ hipError_t launch_binned_onepass(const BinnedWorkspace& ws, const OnePass& op, c
   return hipGetLastError();
 }
 
-hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
-                         const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
-                         hipEvent_t* ev, bool output) {
+// The count, scan and scatter kernels of the two-pass launch: every box's
+// coarse memberships in bin order in the bucket (events 0-4 when ev).
+hipError_t launch_binned_front(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
+                               const BatchPlan& bp, hipStream_t s, hipEvent_t* ev) {
   hipError_t e;
   // Every kernel tolerates an empty batch (no chunks, no bins): grids of at
   // least one block whose blocks find no work.
@@ -2655,6 +2637,19 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
         boxes, nullptr, ws.inst, ws.chunk_inst, bp.n_chunks, ws.hist, ws.off, ws.bin_start, ws.bbox, ws.bid,
         ws.cap_bucket, nb_cap, ws.total, fold, bp.dense_exit, ws.meta);
   PBIG_MARK(4)
+#undef PBIG_MARK
+  return hipGetLastError();
+}
+
+hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const uint32_t* ids_in,
+                         const BatchPlan& bp, uint2* out, uint64_t cap, uint64_t* inst_out, hipStream_t s,
+                         hipEvent_t* ev, bool output) {
+  hipError_t e;
+  if ((e = launch_binned_front(ws, boxes, ids_in, bp, s, ev)) != hipSuccess) return e;
+  const uint32_t nbins = bp.n_bins ? bp.n_bins : 1u;
+  const uint32_t fold = launch_folds(bp) ? 1u : 0u;
+#define PBIG_MARK(k) \
+  if (ev && (e = hipEventRecord(ev[k], s)) != hipSuccess) return e;
   k_bin_test<false><<<bp.test_blocks, kTestThreads, kTestLdsBytes, s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total,
                                                                    ws.inst, ws.bin_inst, bp.n_bins, ws.cap_bucket,
                                                                    ws.scratch, ws.cap_wg, ws.bin_seg,
@@ -2671,6 +2666,342 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
   PBIG_MARK(7)
 #undef PBIG_MARK
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The general path's cell layout built from the bins (after the two-pass
+// launch's count, scan and scatter, one instance), for k_test: every fine
+// membership of a bin (a box once per fine cell it covers inside the bin)
+// in fine-cell order, every cell's run inside the bin's share of the
+// membership array, with the box's start-column and start-row flags, and
+// the 64 x 64 tile-pair items of the bin's cells.  Two kernels:
+//   k_fine_count  one persistent workgroup per bin in turn: fine counts in
+//                 LDS, their scan, the bin's shares of the membership and
+//                 item arrays (two device atomics per bin, not one per
+//                 membership), the items, every cell's absolute start, and
+//                 the bin's strips (row-major runs of cells starting within
+//                 one window of kSortStage / 2 memberships) on a work list;
+//   k_fine_place  persistent workgroups over the strips of all bins: the
+//                 bin's memberships stream from the bucket, the strip's ones
+//                 go to an LDS stage in cell order and out in one run (a
+//                 strip past the stage -- a cell of more than kSortStage / 2
+//                 memberships at its end, or a bin of more than kSortBigBin
+//                 memberships, which is one strip -- is written in place).
+constexpr uint32_t kSortThreads = 1024;
+constexpr uint32_t kSortPer = kMaxCellsPerBin / kSortThreads;  // cells per thread in the scans
+static_assert(kSortPer * kSortThreads == kMaxCellsPerBin, "four cells per thread");
+constexpr uint32_t kSortStage = 5120;
+constexpr uint32_t kSortWindow = kSortStage / 2;
+constexpr uint32_t kSortBigBin = 8u * kSortWindow;
+
+// Tile-pair items of a cell of n memberships (none past the tile limit: the
+// call then fails with kErrTooDense).
+__device__ __forceinline__ uint64_t tile_items(uint32_t n) {
+  const uint64_t nt = (n + kTile - 1u) / kTile;
+  return n < 2u || nt > kMaxTilesPerCell ? 0ull : nt * (nt + 1u) / 2u;
+}
+
+// A bin's fine cells (k x k cells of the instance's grid, an owned instance
+// of the plain path) and the fine cells a membership covers inside it.
+struct SortBin {
+  uint32_t gx, gy, kx, ky;
+  __device__ SortBin(const InstDesc& I, uint32_t b) {
+    gx = I.fx0 + (b % I.nbx) * I.k;
+    gy = I.fy0 + (b / I.nbx) * I.k;
+    const uint32_t rx = I.fx0 + I.span_x - gx, ry = I.fy0 + I.span_y - gy;
+    kx = I.k < rx ? I.k : rx;
+    ky = I.k < ry ? I.k : ry;
+  }
+  // (every membership of the bin covers at least one of its cells)
+  __device__ void cells(const InstDesc& I, const float4& q, uint32_t& x0, uint32_t& x1, uint32_t& y0, uint32_t& y1,
+                        uint32_t& fx, uint32_t& fy) const {
+    fx = fine_of(q.x, I.inv_f, I.nf);
+    fy = fine_of(q.z, I.inv_f, I.nf);
+    const uint32_t ux = fine_of(q.y, I.inv_f, I.nf), uy = fine_of(q.w, I.inv_f, I.nf);
+    x0 = (fx > gx ? fx : gx) - gx;
+    y0 = (fy > gy ? fy : gy) - gy;
+    x1 = (ux < gx + kx - 1u ? ux : gx + kx - 1u) - gx;
+    y1 = (uy < gy + ky - 1u ? uy : gy + ky - 1u) - gy;
+  }
+};
+
+struct CountLds {
+  uint32_t cnt[kMaxCellsPerBin];
+  uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
+  uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
+  uint32_t ws[kSortThreads / 64];
+  unsigned long long fbase, ibase;
+  uint32_t bmax, sbase;
+};
+
+__global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __restrict__ bbox,
+                                                             const uint32_t* __restrict__ bin_start,
+                                                             const uint32_t* __restrict__ total,
+                                                             const InstDesc* __restrict__ inst, uint32_t n_bins,
+                                                             const Meta* __restrict__ bmeta,
+                                                             uint32_t* __restrict__ cell_start,
+                                                             uint32_t* __restrict__ bin_end, uint64_t cap_memb,
+                                                             uint4* __restrict__ items, uint64_t cap_items,
+                                                             uint4* __restrict__ strips, uint64_t cap_strips,
+                                                             Meta* __restrict__ meta) {
+  __shared__ CountLds L;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  // an invalid box or a bucket that did not fit: nothing to lay out
+  const unsigned long long berr = bmeta->error & (kErrInvalidBox | kErrWorkspace);
+  if (berr) {
+    if (blockIdx.x == 0 && tid == 0) atomicOr((unsigned long long*)&meta->error, berr);
+    return;
+  }
+  const InstDesc I = inst[0];
+  const uint32_t k2 = I.k * I.k;
+  for (uint32_t c = tid; c < kMaxCellsPerBin; c += kSortThreads) L.cnt[c] = 0u;
+  if (tid == 0) L.bmax = 0u;
+  __syncthreads();
+  for (uint32_t b = blockIdx.x; b < n_bins; b += gridDim.x) {
+    const uint32_t m = total[b];
+    if (m == 0u) continue;  // (uniform over the block)
+    const uint64_t s0 = bin_start[b];
+    const SortBin B(I, b);
+    const uint32_t kk = B.kx * B.ky;
+    // 1. fine counts
+    for (uint32_t j = tid; j < m; j += kSortThreads) {
+      uint32_t x0, x1, y0, y1, fx, fy;
+      B.cells(I, bbox[s0 + j], x0, x1, y0, y1, fx, fy);
+      for (uint32_t cy = y0; cy <= y1; ++cy)
+        for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cnt[cy * B.kx + cx], 1u);
+    }
+    __syncthreads();
+    // 2. exclusive scans of the cells' memberships and items (four cells
+    //    per thread, 64-bit sums: a share past 2^32 - 1 cannot fit and
+    //    fails the call with kErrWorkspace)
+    uint32_t v[kSortPer], vmax = 0;
+    uint64_t acc = 0, iacc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortPer; ++k) {
+      const uint32_t c = tid * kSortPer + k;
+      v[k] = c < kk ? L.cnt[c] : 0u;
+      acc += v[k];
+      iacc += tile_items(v[k]);
+      vmax = v[k] > vmax ? v[k] : vmax;
+    }
+    const uint64_t inc = wave_prefix_sum64(acc), iinc = wave_prefix_sum64(iacc);
+    if (lane == 63u) {
+      L.wm[w] = inc;
+      L.wi[w] = iinc;
+    }
+    if (vmax) atomicMax(&L.bmax, vmax);
+    __syncthreads();
+    uint64_t e = inc - acc, ie = iinc - iacc, F = 0, II = 0;
+    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+      if (k < w) {
+        e += L.wm[k];
+        ie += L.wi[k];
+      }
+      F += L.wm[k];
+      II += L.wi[k];
+    }
+    if (tid == 0) {
+      L.fbase = atomicAdd((unsigned long long*)&meta->memberships, (unsigned long long)F);
+      L.ibase = atomicAdd((unsigned long long*)&meta->items, (unsigned long long)II);
+      atomicMax(&meta->max_per_cell, L.bmax);
+      if (L.bmax > kTile * kMaxTilesPerCell) atomicOr((unsigned long long*)&meta->error, kErrTooDense);
+    }
+    __syncthreads();
+    const uint64_t fb = L.fbase, ib = L.ibase;
+    const bool fits = F < (1ull << 32) && II < (1ull << 32) && fb + F <= cap_memb && ib + II <= cap_items;
+    if (!fits && tid == 0) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
+    // 3. every cell's absolute start, and the items
+    {
+      uint64_t e2 = e, ie2 = ie;
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; ++k) {
+        const uint32_t c = tid * kSortPer + k;
+        if (fits && c < kk) {
+          cell_start[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
+          const uint32_t n = v[k];
+          const uint32_t nt = (n + kTile - 1u) / kTile;
+          if (n >= 2u && nt <= kMaxTilesPerCell) {
+            uint64_t at = ib + ie2;
+            const uint32_t st = (uint32_t)(fb + e2);
+            for (uint32_t r = 0; r < nt; ++r)
+              for (uint32_t q = r; q < nt; ++q) items[at++] = make_uint4(st, n, r, q);
+          }
+        }
+        e2 += v[k];
+        ie2 += tile_items(v[k]);
+      }
+    }
+    // 4. the strips: cell 0 opens one, and so does every cell whose start
+    //    lies in a later window of kSortWindow memberships than its
+    //    predecessor's (a strip then spans less than two windows unless its
+    //    last cell alone is larger)
+    {
+      uint32_t win[kSortPer];
+      uint64_t e3 = e;
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; ++k) {
+        win[k] = (uint32_t)(e3 / kSortWindow);
+        e3 += v[k];
+      }
+      L.sl[tid] = win[kSortPer - 1];  // (for now: the window of this thread's last cell)
+      __syncthreads();
+      const uint32_t before = tid ? L.sl[tid - 1u] : 0xFFFFFFFFu;
+      __syncthreads();
+      uint32_t open = 0;  // bit k: this thread's cell k opens a strip
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; ++k) {
+        const uint32_t c = tid * kSortPer + k;
+        const uint32_t pw = k ? win[k - 1] : before;
+        // (a bin of more than kSortBigBin memberships is one strip, written
+        // in place: its strips would each read the whole bin again)
+        if (c < kk && (c == 0u || (win[k] != pw && F <= kSortBigBin))) open |= 1u << k;
+      }
+      const uint32_t nopen = (uint32_t)__popc(open);
+      const uint32_t sinc = wave_prefix_sum(nopen);
+      if (lane == 63u) L.ws[w] = sinc;
+      __syncthreads();
+      uint32_t sidx = sinc - nopen, nst = 0;
+      for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+        if (k < w) sidx += L.ws[k];
+        nst += L.ws[k];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; ++k)
+        if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
+      if (tid == 0) L.sbase = fits ? atomicAdd(&meta->dev_chunks, nst) : 0u;
+      __syncthreads();
+      const uint32_t sb = L.sbase;
+      if (fits) {
+        for (uint32_t t = tid; t < nst; t += kSortThreads) {
+          const uint32_t lo = L.sl[t], hi = t + 1u < nst ? L.sl[t + 1u] : kk;
+          if ((uint64_t)sb + t < cap_strips) strips[sb + t] = make_uint4(b, lo, hi, 0u);
+        }
+        if (tid == 0) {
+          bin_end[b] = (uint32_t)(fb + F);
+          if ((uint64_t)sb + nst > cap_strips) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t c = tid; c < kk; c += kSortThreads) L.cnt[c] = 0u;
+    if (tid == 0) L.bmax = 0u;
+    __syncthreads();
+  }
+}
+
+struct PlaceLds {
+  uint32_t cur[kMaxCellsPerBin];  // per cell of the strip: cursor
+  uint32_t cst[kMaxCellsPerBin];  // per cell of the strip: start, from the strip's
+  float4 sbox[kSortStage];
+  uint32_t sid[kSortStage];
+};
+static_assert(sizeof(PlaceLds) <= 160u * 1024u, "one workgroup per CU");
+
+__global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __restrict__ bbox,
+                                                             const uint32_t* __restrict__ bid,
+                                                             const uint32_t* __restrict__ bin_start,
+                                                             const uint32_t* __restrict__ total,
+                                                             const InstDesc* __restrict__ inst,
+                                                             const uint32_t* __restrict__ cell_start,
+                                                             const uint32_t* __restrict__ bin_end,
+                                                             const uint4* __restrict__ strips, uint64_t cap_strips,
+                                                             float4* __restrict__ mbox, uint32_t* __restrict__ mid,
+                                                             const Meta* __restrict__ meta) {
+  extern __shared__ __align__(16) unsigned char place_lds[];
+  PlaceLds& L = *reinterpret_cast<PlaceLds*>(place_lds);
+  const uint32_t tid = threadIdx.x;
+  if (meta->error) return;  // (the layout is incomplete: the call fails or grows and reruns)
+  const uint32_t n_strips = (uint64_t)meta->dev_chunks < cap_strips ? meta->dev_chunks : (uint32_t)cap_strips;
+  const InstDesc I = inst[0];
+  const uint32_t k2 = I.k * I.k;
+  for (uint32_t t = blockIdx.x; t < n_strips; t += gridDim.x) {
+    const uint4 st = strips[t];
+    const uint32_t b = st.x, lo = st.y, hi = st.z;
+    const uint32_t m = total[b];
+    const uint64_t s0 = bin_start[b];
+    const SortBin B(I, b);
+    const uint32_t kk = B.kx * B.ky;
+    const uint64_t cb = (uint64_t)b * k2;
+    const uint32_t base = cell_start[cb + lo];
+    const uint32_t span = (hi < kk ? cell_start[cb + hi] : bin_end[b]) - base;
+    const bool staged = span <= kSortStage;
+    for (uint32_t c = lo + tid; c < hi; c += kSortThreads) {
+      L.cur[c - lo] = 0u;
+      L.cst[c - lo] = cell_start[cb + c] - base;
+    }
+    __syncthreads();
+    const uint32_t ylo = lo / B.kx, yhi = (hi - 1u) / B.kx;
+    for (uint32_t j = tid; j < m; j += kSortThreads) {
+      const float4 q = bbox[s0 + j];
+      uint32_t x0, x1, y0, y1, fx, fy;
+      B.cells(I, q, x0, x1, y0, y1, fx, fy);
+      if (y1 < ylo || y0 > yhi) continue;
+      const uint32_t id = bid[s0 + j];
+      for (uint32_t cy = y0 > ylo ? y0 : ylo; cy <= (y1 < yhi ? y1 : yhi); ++cy) {
+        const uint32_t flag_y = cy + B.gy == fy ? kFlagY : 0u;
+        for (uint32_t cx = x0; cx <= x1; ++cx) {
+          const uint32_t c = cy * B.kx + cx;
+          if (c < lo || c >= hi) continue;
+          const uint32_t p = L.cst[c - lo] + atomicAdd(&L.cur[c - lo], 1u);
+          const uint32_t word = id | flag_y | (cx + B.gx == fx ? kFlagX : 0u);
+          if (staged) {
+            L.sbox[p] = q;
+            L.sid[p] = word;
+          } else {
+            mbox[(uint64_t)base + p] = q;
+            mid[(uint64_t)base + p] = word;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (staged) {
+      for (uint32_t i = tid; i < span; i += kSortThreads) {
+        mbox[(uint64_t)base + i] = L.sbox[i];
+        mid[(uint64_t)base + i] = L.sid[i];
+      }
+    }
+    __syncthreads();  // the stage and the cursors are reused by the next strip
+  }
+}
+
+hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const BatchPlan& bp, const Workspace& gw,
+                         Meta* meta, uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev) {
+  hipError_t e;
+  if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
+  if ((e = launch_meta_clear(meta, s)) != hipSuccess) return e;
+  if ((e = launch_binned_front(ws, boxes, nullptr, bp, s, ev ? ev + 1 : nullptr)) != hipSuccess) return e;
+  const uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
+  k_fine_count<<<grid, kSortThreads, 0, s>>>(ws.bbox, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt,
+                                             gw.start, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
+                                             gw.cap_strips, meta);
+  if (ev && (e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
+  k_fine_place<<<grid, kSortThreads, sizeof(PlaceLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, gw.cnt,
+                                                            gw.start, gw.strips, gw.cap_strips, gw.mbox, gw.mid, meta);
+  if (ev && (e = hipEventRecord(ev[7], s)) != hipSuccess) return e;
+  if ((e = launch_tile_test(gw, meta, out, cap, s)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[8], s)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t binned_init() {
+  {
+    const hipError_t e0 = hipFuncSetAttribute((const void*)k_bin_place, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kPlaceLdsBytes);
+    if (e0 != hipSuccess) return e0;
+  }
+  hipError_t e = hipFuncSetAttribute((const void*)k_bin_test<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kTestLdsBytes);
+  if (e != hipSuccess) return e;
+  const int scatter_max = (int)(sizeof(ScatterLds) + 8u * kMaxBins);
+  for (const void* f : {(const void*)k_bin_test<true>, (const void*)k_bin_scatter<false, false>,
+                        (const void*)k_bin_scatter<true, false>, (const void*)k_bin_scatter<true, true>}) {
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            f == (const void*)k_bin_test<true> ? (int)kTestLdsBytes : scatter_max);
+    if (e != hipSuccess) return e;
+  }
+  return hipFuncSetAttribute((const void*)k_fine_place, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)sizeof(PlaceLds));
 }
 
 }  // namespace pbig

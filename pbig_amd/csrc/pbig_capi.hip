@@ -98,7 +98,10 @@ struct pbig_hip_ctx {
   hipStream_t stream = nullptr;
   int n_cu = 0;
   Workspace ws;
-  Meta* h_meta = nullptr;          // pinned mirror of ws.meta
+  Meta* h_meta = nullptr;          // pinned mirrors of ws.meta and sort_meta
+  // Result words of the general path built from the bins (launch_sorted):
+  // its fine memberships, items and pairs; the bins' own in ws.meta.
+  Meta* sort_meta = nullptr;
   // Host-path staging.
   float4* d_in = nullptr;
   uint64_t cap_in = 0;
@@ -137,13 +140,16 @@ struct pbig_hip_ctx {
 };
 
 namespace {
-constexpr int kEventsPerCall = 8;
+constexpr int kEventsPerCall = 10;
 // Timing path id of the plain path's launch with device re-binning levels:
 // one duration per level (level 0 is the two-pass launch).
 constexpr int kPathDeep = 4;
 // Timing path id of the single-pass launch (place, test), beside
 // PBIG_PATH_GENERAL and PBIG_PATH_BINNED.
 constexpr int kPathOnePass = 3;
+// Timing path id of the general path's tile test on a cell layout built
+// from the bins (launch_sorted).
+constexpr int kPathSorted = 5;
 
 // Device memory of pbig_hip_check_device_noalloc: the whole workspace of a
 // single-instance binned launch, carved per call from this array.  Each
@@ -647,6 +653,79 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
   if (c->timing_used >= c->timing_cap) return nullptr;
   c->timing_path = path;
   return &c->events[(size_t)c->timing_used++ * kEventsPerCall];
+}
+
+// The general path's tile test on a cell layout built from the bins of the
+// plain state's plan (launch_sorted), synchronously: the bucket, the
+// membership array and the item array grow until the launch fits.
+int size_sorted(pbig_hip_ctx* c, uint32_t n) {
+  const InstDesc& d = c->plain.inst[0];
+  int rc;
+  if ((rc = ensure_cells(c, std::max<uint64_t>((uint64_t)d.nb * d.k * d.k, 1)))) return rc;
+  if ((rc = grow(c, &c->ws.strips, &c->ws.cap_strips,
+                 std::max<uint64_t>(c->ws.cap_strips, 2ull * d.nb + (uint64_t)n / 1024 + 1024))))
+    return rc;
+  return ensure_work(c, std::max<uint64_t>(c->ws.cap_memb, (uint64_t)n + n / 2 + 1024),
+                     std::max<uint64_t>(c->ws.cap_items, (uint64_t)n / 2 + 1024));
+}
+
+pbig::BatchPlan sorted_plan(const BinnedState& st) {
+  pbig::BatchPlan bp = st.bp;
+  bp.dense_exit = 0u;  // every bin is laid out, dense or not
+  bp.direct = 0u;
+  return bp;
+}
+
+int run_sorted_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d_out, uint64_t cap, hipStream_t s,
+                    uint64_t* n_pairs) {
+  BinnedState& st = c->plain;
+  int rc;
+  if ((rc = size_sorted(c, n))) return rc;
+  if ((rc = after_upload(c, st, s))) return rc;
+  const pbig::BatchPlan bp = sorted_plan(st);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    HIP_TRY(c, pbig::launch_sorted(st.ws, d_boxes, bp, c->ws, c->sort_meta, d_out, cap, s,
+                                   next_events(c, kPathSorted)));
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta, st.ws.meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(c->h_meta + 1, c->sort_meta, sizeof(Meta), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const Meta mb = c->h_meta[0], m = c->h_meta[1];
+    if ((mb.error | m.error) & pbig::kErrInvalidBox)
+      return fail(c, PBIG_E_INVALID_BOX, "input has a box with xmin > xmax, ymin > ymax or a NaN");
+    if (m.error & pbig::kErrTooDense)
+      return fail(c, PBIG_E_TOO_DENSE, "a cell holds %u boxes, above the %u-tile limit", m.max_per_cell,
+                  pbig::kMaxTilesPerCell);
+    if ((mb.error | m.error) & pbig::kErrWorkspace) {
+      if (attempt == 2) return fail(c, PBIG_E_NOMEM, "workspace did not fit after regrowth");
+      if (mb.error & pbig::kErrWorkspace) {  // the bucket (the cell layout was not built)
+        uint64_t bucket = std::max<uint64_t>(st.ws.cap_bucket, mb.memberships + mb.memberships / 16 + 4096);
+        if (mb.memberships <= 0xFFFFFFFFull) bucket = std::min<uint64_t>(bucket, 0xFFFFFFFFull);
+        if ((rc = ensure_state(c, st, bucket))) return rc;
+      } else {
+        if ((rc = ensure_work(c, std::max<uint64_t>(c->ws.cap_memb, m.memberships + m.memberships / 16 + 1024),
+                              std::max<uint64_t>(c->ws.cap_items, m.items + m.items / 16 + 1024))))
+          return rc;
+        if ((rc = grow(c, &c->ws.strips, &c->ws.cap_strips,
+                       std::max<uint64_t>(c->ws.cap_strips, (uint64_t)m.dev_chunks + m.dev_chunks / 8 + 1024))))
+          return rc;
+      }
+      continue;
+    }
+    const InstDesc& d = st.inst[0];
+    c->stats.n_boxes = n;
+    c->stats.n_cells = (uint64_t)d.nf * d.nf;
+    c->stats.cells_per_row = d.nf;
+    c->stats.memberships = m.memberships;
+    c->stats.tile_pairs = m.items;
+    c->stats.n_pairs = m.n_pairs;
+    c->stats.max_per_cell = m.max_per_cell;
+    *n_pairs = m.n_pairs;
+    if (m.n_pairs > cap)
+      return fail(c, PBIG_E_OVERFLOW, "%llu pairs do not fit the capacity of %llu",
+                  (unsigned long long)m.n_pairs, (unsigned long long)cap);
+    return PBIG_OK;
+  }
+  return fail(c, PBIG_E_HIP, "unreachable");
 }
 
 // Run a binned state synchronously, regrowing the workspace when the bucket
@@ -1314,12 +1393,18 @@ int run_any_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, double rang
   GridParams gp;
   uint32_t n_cells;
   if ((rc = grid_of(c, range, cell, &gp, &n_cells))) return rc;
-  rc = run_sync(c, d_boxes, n, gp, n_cells, d_out, cap, s, n_pairs);
+  // The general path's cells laid out from the bins when the binned grid
+  // can be planned, else by one device atomic per membership over the whole
+  // grid's cells.
+  bool planned = false;
+  if ((rc = plan_plain(c, n, range, cell, s, &planned))) return rc;
+  rc = planned ? run_sorted_sync(c, d_boxes, n, d_out, cap, s, n_pairs)
+               : run_sync(c, d_boxes, n, gp, n_cells, d_out, cap, s, n_pairs);
   if (rc == PBIG_OK || rc == PBIG_E_OVERFLOW) {
     c->last_path = PBIG_PATH_GENERAL;
     c->stats.path = PBIG_PATH_GENERAL;
-    c->stats.bins_per_row = 0;
-    c->stats.cells_per_bin_side = 0;
+    c->stats.bins_per_row = planned ? c->plain.inst[0].nbx : 0u;
+    c->stats.cells_per_bin_side = planned ? c->plain.inst[0].k : 0u;
   }
   return rc;
 }
@@ -1463,12 +1548,17 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
     delete c;
     return PBIG_E_HIP;
   }
-  if (hipHostMalloc((void**)&c->h_meta, sizeof(Meta), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&c->h_meta, 2 * sizeof(Meta), hipHostMallocDefault) != hipSuccess) {
     (void)hipFree(c->ws.meta);
     delete c;
     return PBIG_E_NOMEM;
   }
-  memset(c->h_meta, 0, sizeof(Meta));
+  memset(c->h_meta, 0, 2 * sizeof(Meta));
+  if (hipMalloc((void**)&c->sort_meta, sizeof(Meta)) != hipSuccess ||
+      hipMemset(c->sort_meta, 0, sizeof(Meta)) != hipSuccess) {
+    pbig_hip_destroy(c);
+    return PBIG_E_NOMEM;
+  }
   bool ok = hipHostMalloc((void**)&c->h_ring, 16 * sizeof(InstDesc), hipHostMallocDefault) == hipSuccess;
   for (hipEvent_t& e : c->ring_ev)
     ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess &&
@@ -1487,7 +1577,8 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Workspace& w = c->ws;
   for (void* p : {(void*)w.cnt, (void*)w.start, (void*)w.partials, (void*)w.items, (void*)w.mbox,
-                  (void*)w.mid, (void*)w.meta, (void*)c->d_in, (void*)c->d_pairs})
+                  (void*)w.mid, (void*)w.strips, (void*)w.meta, (void*)c->sort_meta, (void*)c->d_in,
+                  (void*)c->d_pairs})
     if (p) (void)hipFree(p);
   free_state(c->plain);
   for (void* p : {(void*)c->uw.mxy, (void*)c->uw.mz, (void*)c->uw.mid, (void*)c->uw.mcell, (void*)c->d_in32})
@@ -1620,6 +1711,16 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
     HIP_TRY(c, pbig::launch_binned(c->plain.ws, (const float4*)d_boxes, nullptr, c->plain.bp, (uint2*)d_pairs, cap,
                                    nullptr, s, next_events(c, PBIG_PATH_BINNED)));
   } else {
+    bool planned = false;
+    if ((rc = plan_plain(c, n, range, cell_size, s, &planned))) return rc;
+    if (planned) {
+      if ((rc = size_sorted(c, n))) return rc;
+      if ((rc = after_upload(c, c->plain, s))) return rc;
+      HIP_TRY(c, pbig::launch_sorted(c->plain.ws, (const float4*)d_boxes, sorted_plan(c->plain), c->ws,
+                                     c->sort_meta, (uint2*)d_pairs, cap, s, next_events(c, kPathSorted)));
+      if (d_result) HIP_TRY(c, pbig::launch_result(c->sort_meta, d_result, s));
+      return PBIG_OK;
+    }
     if ((rc = prepare(c, n, n_cells))) return rc;
     HIP_TRY(c, pbig::launch_plain(c->ws, (const float4*)d_boxes, n, gp, n_cells, (uint2*)d_pairs, cap, s,
                                   next_events(c, PBIG_PATH_GENERAL)));
@@ -1836,6 +1937,7 @@ int pbig_hip_timing_read(pbig_hip_ctx* c, double* ms_per_kernel, int n_kernels, 
   const int nk = c->timing_path == PBIG_PATH_BINNED ? pbig::kBinnedKernels
                  : c->timing_path == kPathOnePass ? pbig::kOnePassKernels
                  : c->timing_path == kPathDeep    ? (int)pbig::kDeepLevels
+                 : c->timing_path == kPathSorted  ? pbig::kSortedKernels
                                                    : pbig::kPlainKernels;
   for (int k = 0; k < n_kernels; ++k) ms_per_kernel[k] = 0.0;
   for (int call = 0; call < c->timing_used; ++call) {

@@ -313,16 +313,26 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
     const bool afx = (aid & kFlagX) != 0, afy = (aid & kFlagY) != 0;
     aid &= kIdMask;
 
+    // the column tile: lane l holds column j0 + l, broadcast to the wave
+    // one column at a time from registers (no memory access per column)
     const uint32_t j0 = ct * kTile;
-    const uint32_t j1 = (j0 + kTile < n) ? j0 + kTile : n;
+    const uint32_t ncol = n - j0 < kTile ? n - j0 : kTile;
+    float4 cb = make_float4(0.f, -1.f, 0.f, -1.f);
+    uint32_t cw = 0;
+    if (lane < (int)ncol) {
+      cb = mbox[base + j0 + lane];
+      cw = mid[base + j0 + lane];
+    }
     const bool diag = rt == ct;
-    for (uint32_t j = j0; j < j1; ++j) {
-      const float4 b = mbox[base + j];
-      const uint32_t bw = mid[base + j];
+    for (uint32_t jj = 0; jj < ncol; ++jj) {
+      const float bx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.x), (int)jj));
+      const float by = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.y), (int)jj));
+      const float bz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.z), (int)jj));
+      const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.w), (int)jj));
+      const uint32_t bw = (uint32_t)__builtin_amdgcn_readlane((int)cw, (int)jj);
       const bool bfx = (bw & kFlagX) != 0, bfy = (bw & kFlagY) != 0;
-      bool hit = valid && (a.x <= b.y) && (b.x <= a.y) && (a.z <= b.w) && (b.z <= a.w) &&
-                 (afx || bfx) && (afy || bfy);
-      if (diag) hit = hit && (j > i);
+      bool hit = valid && (a.x <= by) && (bx <= a.y) && (a.z <= bwv) && (bz <= a.w) && (afx || bfx) && (afy || bfy);
+      if (diag) hit = hit && (jj > (uint32_t)lane);
       const uint64_t mask = __ballot(hit);
       if (mask) {
         const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -356,6 +366,22 @@ hipError_t launch_scan(const Workspace& ws, uint32_t n_cells, uint64_t memb_cap,
   k_scan_reduce<<<n_partials, 256, 0, s>>>(ws.cnt, n_cells, ws.partials);
   k_scan_apply<<<n_partials, 256, 0, s>>>(ws.cnt, n_cells, ws.partials, n_partials, ws.start, ws.items,
                                           ws.cap_items, memb_cap, ws.meta);
+  return hipGetLastError();
+}
+
+// Every word of a Meta zeroed.
+__global__ __launch_bounds__(64) void k_meta_zero(Meta* __restrict__ meta) {
+  if (threadIdx.x < sizeof(Meta) / sizeof(unsigned long long))
+    reinterpret_cast<unsigned long long*>(meta)[threadIdx.x] = 0ull;
+}
+
+hipError_t launch_meta_clear(Meta* meta, hipStream_t s) {
+  k_meta_zero<<<1, 64, 0, s>>>(meta);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_test(const Workspace& ws, Meta* meta, uint2* out, uint64_t cap, hipStream_t s) {
+  k_test<<<ws.test_blocks, 256, 0, s>>>(ws.items, ws.mbox, ws.mid, out, cap, meta);
   return hipGetLastError();
 }
 

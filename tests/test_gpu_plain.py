@@ -67,6 +67,17 @@ def run_device(torch, eng, boxes, L, cs, cap=None):
     return oracle.sort_pairs(out[:n].cpu().numpy().view(np.uint32))
 
 
+def general_grid_stats(orc, st, boxes, cs, m):
+    """The oracle's cell statistics of the grid the general path ran on: the
+    whole grid's cells (the manifest's figures), or, when its cells were laid
+    out from the bins, the binned grid, whose cells per row are rounded up to
+    whole bins (boxes past the domain edge fold into its last row and
+    column, not the domain's)."""
+    if st["bins_per_row"] == 0:
+        return m["grid_stats_at_reference_cell_size"]
+    return orc.grid_stats(boxes, st["cells_per_row"] * cs, cs)
+
+
 def test_c1_matches_golden_and_oracle(torch_cuda, eng, orc, manifest):
     from pbig_amd import workloads
 
@@ -80,8 +91,9 @@ def test_c1_matches_golden_and_oracle(torch_cuda, eng, orc, manifest):
     assert orc.pair_hash(got) == m["sha256"]
     st = eng.stats()
     if st["path"] == 1:
-        assert st["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
-        assert st["max_per_cell"] == m["grid_stats_at_reference_cell_size"]["max_per_cell"]
+        ref = general_grid_stats(orc, st, boxes, cs, m)
+        assert st["memberships"] == ref["memberships"]
+        assert st["max_per_cell"] == ref["max_per_cell"]
     # cell_size = 0 selects the reference rule on the device boxes
     got0 = run_device(torch_cuda, eng, boxes, L, 0)
     assert np.array_equal(got0, golden)
@@ -106,8 +118,10 @@ def test_c2_full_size(torch_cuda, eng, orc, manifest):
     got = run_device(torch_cuda, eng, boxes, L, m["reference_cell_size"], cap=4 * m["pairs"])
     assert len(got) == m["pairs"]
     assert orc.pair_hash(got) == m["sha256"]
-    if eng.stats()["path"] == 1:
-        assert eng.stats()["memberships"] == m["grid_stats_at_reference_cell_size"]["memberships"]
+    st = eng.stats()
+    if st["path"] == 1:
+        ref = general_grid_stats(orc, st, boxes, m["reference_cell_size"], m)
+        assert st["memberships"] == ref["memberships"]
 
 
 def test_c3_full_size(torch_cuda, eng, orc, manifest):
