@@ -655,3 +655,47 @@ def test_auto_spot_in_uniform_instance(torch_cuda, orc, levels_min):
             torch_cuda.cuda.synchronize()
             assert res.cpu().tolist() == [len(want), 0], rep
             assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want), rep
+
+
+def test_general_cells_from_bins_dense_and_large(torch_cuda, orc):
+    """The general path's cells laid out from the bins (cells per bin counted
+    and placed in LDS): a cell of 3,000 memberships (more than the placement
+    stage's window, written in place), a bin of more than eight windows
+    (placed as one strip), boxes spanning hundreds of cells (the membership
+    array grows and the call reruns), asynchronous calls after the
+    synchronous one, all against sweep-and-prune; then a cell past the
+    262,144-box tile limit, which the call refuses."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    stack = np.tile(np.array([[3000.0, 3001.0, 3000.0, 3001.0]], np.float32), (3000, 1))  # (its own bin)
+    cluster = workloads.uniform(30000, 200.0, 1.0, 4.0, 93) + np.float32(200.0)
+    background = workloads.uniform(20000, 4096.0, 1.0, 8.0, 94)
+    rng = np.random.default_rng(95)
+    c = rng.uniform(0, 4096, size=(50, 2)).astype(np.float32)
+    w = rng.uniform(100, 400, size=(50, 2)).astype(np.float32)
+    wide = np.stack([c[:, 0], c[:, 0] + w[:, 0], c[:, 1], c[:, 1] + w[:, 1]], 1).astype(np.float32)
+    boxes = np.concatenate([background[:10000], stack, wide, cluster, background[10000:]]).astype(np.float32)
+    want = orc.sort_pairs(orc.sweep_pairs(boxes))
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((len(want) + 4096, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["general"], 4096)
+        n = e.check_device(d, 4096.0, 16.0, out)
+        st = e.stats()
+        assert st["path"] == PATHS["general"] and st["bins_per_row"] > 0  # the cells came from the bins
+        assert st["max_per_cell"] >= 3000
+        assert n == len(want)
+        assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want)
+        for rep in range(2):
+            res.zero_()
+            out.fill_(-1)
+            e.check_device_async(d, 4096.0, 16.0, out, res)
+            torch_cuda.cuda.synchronize()
+            assert res.cpu().tolist() == [len(want), 0], rep
+            assert np.array_equal(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32)), want), rep
+        over = torch_cuda.from_numpy(np.tile(np.array([[5.0, 6.0, 5.0, 6.0]], np.float32), (262145, 1))).to("cuda:0")
+        with pytest.raises(pbig_amd.PbigError) as err:
+            e.check_device(over, 64.0, 16.0, out)
+        assert err.value.status == pbig_amd.PBIG_E_TOO_DENSE
