@@ -444,7 +444,7 @@ def serving_leg(torch, count, rate, budget_gb):
     total_boxes = sum(r.n for r in reqs)
     res = {"workload": f"{count} requests, n log-uniform [1000, 1000000], 50/50 uniform (L = 65536 sqrt(n/4194304), "
                        f"sides U[1,32]) and clustered (L = 65536 sqrt(n/2097152)), seed 5",
-           "boxes": total_boxes, "hbm_budget_gb": budget_gb, "arrival_rate_per_s": rate, "slots": 3}
+           "boxes": total_boxes, "hbm_budget_gb": budget_gb, "pool_share": 0.3, "arrival_rate_per_s": rate, "slots": 3}
     eng = pbig_amd.Engine(0)
     eng.configure(pbig_amd.PATH_AUTO, int(table["bin_target"]))
 
@@ -469,7 +469,7 @@ def serving_leg(torch, count, rate, budget_gb):
         if mode == "host":
             for r in reqs:
                 r.device = None
-        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, table=table, results=mode)
+        srv = Server(hbm_budget=int(budget_gb * (1 << 30)), slots=3, pool_frac=0.3, table=table, results=mode)
         server(mode, False)  # warm-up: plans, workspaces, pinned buffers, the slots' device levels
         one(mode, False)
 

@@ -253,7 +253,7 @@ struct DeepCaps {
 // The result words of every level of a launch (level 0 first), for the
 // final result kernel.
 constexpr uint32_t kDeepLevels = 4;     // level 0 and up to three re-binning levels
-constexpr uint32_t kDeepMaxInst = 2048;  // virtual instances of one level (LDS of its plan kernel)
+constexpr uint32_t kDeepMaxInst = 8192;  // virtual instances of one level (LDS of its plan kernel)
 struct DeepMetas {
   const Meta* meta[kDeepLevels];
   uint32_t levels;

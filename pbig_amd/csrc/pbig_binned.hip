@@ -1993,6 +1993,8 @@ __global__ __launch_bounds__(256) void k_bin_compact(const uint2* __restrict__ s
 // levels: the grids are upper bounds and every count comes from the level's
 // result words, written by its plan kernel (one workgroup).
 
+constexpr uint32_t kDeepPlanLds = (3u * kDeepMaxInst + 2u) * sizeof(uint32_t);
+
 __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__ pinst,
                                                     const uint32_t* __restrict__ pbin_inst,
                                                     const uint32_t* __restrict__ ptotal,
@@ -2006,9 +2008,13 @@ __global__ __launch_bounds__(1024) void k_deep_plan(const InstDesc* __restrict__
                                                     uint32_t parent_level, uint32_t* __restrict__ dense_list,
                                                     Meta* __restrict__ dense) {
   __shared__ ScanLds L;
-  __shared__ uint32_t c0[kDeepMaxInst + 1], b0[kDeepMaxInst + 1];
-  __shared__ uint32_t ord[kDeepMaxInst];
   __shared__ uint32_t nbmax;
+  // dynamic LDS: first chunk and first bin of every instance (and the
+  // totals after them), and the sorted list
+  extern __shared__ __align__(16) unsigned char plan_lds[];
+  uint32_t* const c0 = reinterpret_cast<uint32_t*>(plan_lds);
+  uint32_t* const b0 = c0 + (kDeepMaxInst + 1);
+  uint32_t* const ord = b0 + (kDeepMaxInst + 1);
   const uint32_t tid = threadIdx.x;
   // (a level planned on the host has no plan error word of its own)
   uint32_t n_over = p_dev && pmeta->plan_err ? 0u : pmeta->max_per_cell;
@@ -2123,7 +2129,7 @@ hipError_t launch_deep_level(const BinnedWorkspace& parent, uint32_t parent_bins
                              const InstDesc* root_inst, uint32_t max_refine, uint32_t n_cu, uint32_t direct,
                              uint2* out, uint64_t cap, uint32_t parent_level, uint32_t* dense_list, Meta* dense,
                              hipStream_t s) {
-  k_deep_plan<<<1, 1024, 0, s>>>(parent.inst, parent.bin_inst, parent.total, parent.bin_start, parent.overfull,
+  k_deep_plan<<<1, 1024, kDeepPlanLds, s>>>(parent.inst, parent.bin_inst, parent.total, parent.bin_start, parent.overfull,
                                  parent_bins, parent_dev ? 1u : 0u, parent.meta, target, caps, root_inv_f, root_inst,
                                  max_refine, ws.inst, ws.chunk_inst, ws.bin_inst, ws.meta, parent_level, dense_list,
                                  dense);
@@ -3000,6 +3006,8 @@ hipError_t binned_init() {
                             f == (const void*)k_bin_test<true> ? (int)kTestLdsBytes : scatter_max);
     if (e != hipSuccess) return e;
   }
+  e = hipFuncSetAttribute((const void*)k_deep_plan, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDeepPlanLds);
+  if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_fine_place, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)sizeof(PlaceLds));
 }
