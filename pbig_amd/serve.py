@@ -324,6 +324,9 @@ class Server:
                 s.src = self.d_pool[r._off:r._off + r.n]
             s.eng.check_device_async(s.src, r.L, r.cell_size, s.pairs, s.res, stream=s.stream)
         else:
+            if s.general:  # the slot's last request ended on the general path: batches choose again
+                self._configure(s.eng, pbig_amd.PATH_AUTO)
+                s.general = False
             if s.batch is None:
                 s.batch = pbig_amd.Batch.from_instances(s.eng, starts, counts, Ls, cells)
             else:
