@@ -436,7 +436,8 @@ def serving_leg(torch, count, rate, budget_gb):
     for q in spec:
         b = workloads.request_boxes(q)
         r = Request(rid=q["id"], boxes=b, L=q["L"], arrival=q["t"],
-                    cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table, b)[0]))
+                    cell_size=float(autotune.cell_sizes([0, b.shape[0]], [q["L"]], table, b)[0]),
+                    narrow=autotune.sides_bounded(b))
         r.host = torch.from_numpy(b).pin_memory()
         r.device = r.host.to(dev)
         reqs.append(r)

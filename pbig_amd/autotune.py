@@ -85,6 +85,13 @@ def side_floor(boxes) -> float:
     return HEADROOM * mx if mx <= BOUNDED_RATIO * med else 0.0
 
 
+def sides_bounded(boxes) -> bool:
+    """The first SAMPLE boxes' sides look bounded (see side_floor): at the
+    floored cell every box spans at most two cells per axis, and a batch of
+    such instances runs the narrow-box test schedule."""
+    return side_floor(boxes) > 0.0
+
+
 def cell_side(n: int, L: float, q: float, floor: float = 0.0) -> float:
     """Cell side holding q box origins on average, sqrt(q L^2 / n), and at
     least `floor`, rounded to a float32 value; at least L / 4096 and at most

@@ -75,6 +75,7 @@ class Request:
     L: float
     arrival: float = 0.0
     cell_size: float = 0.0           # 0 = the autotuner's table
+    narrow: bool | None = None       # sides bounded (autotune.sides_bounded); None = from `boxes` at submission
     host: object = None
     device: object = None
     # filled by the server
@@ -182,7 +183,7 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 8 << 30, slots: int = 2, pool_frac: float = 0.5,
                  table: dict | None = None, bytes_per_box: int = 96, results: str = "host", consumer=None,
-                 max_batch_boxes: int | None = None, host_levels: bool = False):
+                 max_batch_boxes: int | None = None, host_levels: bool = False, group_by_schedule: bool = True):
         import torch
 
         if results not in ("host", "device"):
@@ -192,6 +193,7 @@ class Server:
         torch.zeros(1, device=self.dev)  # torch's HIP runtime before the engine's context
         self.table = table or autotune.load_table()
         self.host_levels = host_levels
+        self.group_by_schedule = group_by_schedule
         self.results = results
         self.consumer = consumer or keep_copy
         self.eng = pbig_amd.Engine(device)
@@ -225,6 +227,8 @@ class Server:
         if req.cell_size <= 0:
             req.cell_size = float(autotune.cell_sizes([0, req.n], [req.L], self.table,
                                                       req.boxes if req.boxes is not None else None)[0])
+        if req.narrow is None and req.boxes is not None:
+            req.narrow = autotune.sides_bounded(req.boxes)
         req._alone = False
         self.queue.append(req)
 
@@ -297,7 +301,18 @@ class Server:
         # alone runs on the plain path, several are finished as a whole, and
         # none is split off again.
         redo = bool(self.admitted) and self.admitted[0]._alone
+        # Like requests together: the test schedule is chosen per launch, and
+        # one instance with wide boxes sends a whole batch to the general
+        # schedule, so a batch takes the oldest request's kind (sides
+        # bounded or not) and leaves the others for the next free slot.
+        kind = None
         for r in self.admitted:
+            if self.group_by_schedule and r._alone == redo:
+                if kind is None:
+                    kind = bool(r.narrow)
+                if bool(r.narrow) != kind:
+                    rest.append(r)
+                    continue
             if r._alone != redo:
                 rest.append(r)
             elif not reqs or total + r.n <= self.max_batch_boxes:

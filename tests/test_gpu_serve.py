@@ -144,7 +144,8 @@ def test_serve_batch_with_overfull_bins(torch_cuda, host_levels):
     from pbig_amd.serve import Server
 
     reqs, table = overfull_requests()
-    srv = Server(hbm_budget=1 << 30, slots=3, table=table, host_levels=host_levels)
+    # (mixed batches: grouped by kind, the clustered request would not share one)
+    srv = Server(hbm_budget=1 << 30, slots=3, table=table, host_levels=host_levels, group_by_schedule=False)
     try:
         srv.run(reqs, realtime=False)  # sizes the slots' workspaces
         for r in reqs:
