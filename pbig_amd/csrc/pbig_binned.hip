@@ -2698,6 +2698,9 @@ constexpr uint32_t kSortPer = kMaxCellsPerBin / kSortThreads;  // cells per thre
 static_assert(kSortPer * kSortThreads == kMaxCellsPerBin, "four cells per thread");
 constexpr uint32_t kSortStage = 5120;
 constexpr uint32_t kSortWindow = kSortStage / 2;
+// Memberships a thread loads before it uses any of them (a dense bin's tens
+// of thousands of memberships are otherwise one load round trip each).
+constexpr uint32_t kSortLoads = 8;
 constexpr uint32_t kSortBigBin = 8u * kSortWindow;
 
 // Tile-pair items of a cell of n memberships (none past the tile limit: the
@@ -2770,11 +2773,21 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
     const SortBin B(I, b);
     const uint32_t kk = B.kx * B.ky;
     // 1. fine counts
-    for (uint32_t j = tid; j < m; j += kSortThreads) {
-      uint32_t x0, x1, y0, y1, fx, fy;
-      B.cells(I, bbox[s0 + j], x0, x1, y0, y1, fx, fy);
-      for (uint32_t cy = y0; cy <= y1; ++cy)
-        for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cnt[cy * B.kx + cx], 1u);
+    for (uint32_t j0 = tid; j0 < m; j0 += kSortThreads * kSortLoads) {
+      float4 q[kSortLoads];
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        const uint32_t j = j0 + u * kSortThreads;
+        q[u] = bbox[s0 + (j < m ? j : m - 1u)];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        if (j0 + u * kSortThreads >= m) break;
+        uint32_t x0, x1, y0, y1, fx, fy;
+        B.cells(I, q[u], x0, x1, y0, y1, fx, fy);
+        for (uint32_t cy = y0; cy <= y1; ++cy)
+          for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cnt[cy * B.kx + cx], 1u);
+      }
     }
     __syncthreads();
     // 2. exclusive scans of the cells' memberships and items (four cells
@@ -2937,25 +2950,38 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __res
     }
     __syncthreads();
     const uint32_t ylo = lo / B.kx, yhi = (hi - 1u) / B.kx;
-    for (uint32_t j = tid; j < m; j += kSortThreads) {
-      const float4 q = bbox[s0 + j];
-      uint32_t x0, x1, y0, y1, fx, fy;
-      B.cells(I, q, x0, x1, y0, y1, fx, fy);
-      if (y1 < ylo || y0 > yhi) continue;
-      const uint32_t id = bid[s0 + j];
-      for (uint32_t cy = y0 > ylo ? y0 : ylo; cy <= (y1 < yhi ? y1 : yhi); ++cy) {
-        const uint32_t flag_y = cy + B.gy == fy ? kFlagY : 0u;
-        for (uint32_t cx = x0; cx <= x1; ++cx) {
-          const uint32_t c = cy * B.kx + cx;
-          if (c < lo || c >= hi) continue;
-          const uint32_t p = L.cst[c - lo] + atomicAdd(&L.cur[c - lo], 1u);
-          const uint32_t word = id | flag_y | (cx + B.gx == fx ? kFlagX : 0u);
-          if (staged) {
-            L.sbox[p] = q;
-            L.sid[p] = word;
-          } else {
-            mbox[(uint64_t)base + p] = q;
-            mid[(uint64_t)base + p] = word;
+    for (uint32_t j0 = tid; j0 < m; j0 += kSortThreads * kSortLoads) {
+      float4 qs[kSortLoads];
+      uint32_t ids[kSortLoads];
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        const uint32_t j = j0 + u * kSortThreads;
+        const uint64_t at = s0 + (j < m ? j : m - 1u);
+        qs[u] = bbox[at];
+        ids[u] = bid[at];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        if (j0 + u * kSortThreads >= m) break;
+        const float4 q = qs[u];
+        uint32_t x0, x1, y0, y1, fx, fy;
+        B.cells(I, q, x0, x1, y0, y1, fx, fy);
+        if (y1 < ylo || y0 > yhi) continue;
+        const uint32_t id = ids[u];
+        for (uint32_t cy = y0 > ylo ? y0 : ylo; cy <= (y1 < yhi ? y1 : yhi); ++cy) {
+          const uint32_t flag_y = cy + B.gy == fy ? kFlagY : 0u;
+          for (uint32_t cx = x0; cx <= x1; ++cx) {
+            const uint32_t c = cy * B.kx + cx;
+            if (c < lo || c >= hi) continue;
+            const uint32_t p = L.cst[c - lo] + atomicAdd(&L.cur[c - lo], 1u);
+            const uint32_t word = id | flag_y | (cx + B.gx == fx ? kFlagX : 0u);
+            if (staged) {
+              L.sbox[p] = q;
+              L.sid[p] = word;
+            } else {
+              mbox[(uint64_t)base + p] = q;
+              mid[(uint64_t)base + p] = word;
+            }
           }
         }
       }
