@@ -2690,9 +2690,10 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 //   k_fine_place  persistent workgroups over the strips of all bins: the
 //                 bin's memberships stream from the bucket, the strip's ones
 //                 go to an LDS stage in cell order and out in one run (a
-//                 strip past the stage -- a cell of more than kSortStage / 2
-//                 memberships at its end, or a bin of more than kSortBigBin
-//                 memberships, which is one strip -- is written in place).
+//                 strip past the stage, i.e. a cell of more than kSortStage
+//                 / 2 memberships at its end, is written in place); a bin of
+//                 more than kSortBigBin memberships is placed in slices
+//                 instead, spread over the chip like the strips.
 constexpr uint32_t kSortThreads = 1024;
 constexpr uint32_t kSortPer = kMaxCellsPerBin / kSortThreads;  // cells per thread in the scans
 static_assert(kSortPer * kSortThreads == kMaxCellsPerBin, "four cells per thread");
@@ -2702,6 +2703,13 @@ constexpr uint32_t kSortWindow = kSortStage / 2;
 // of thousands of memberships are otherwise one load round trip each).
 constexpr uint32_t kSortLoads = 8;
 constexpr uint32_t kSortBigBin = 8u * kSortWindow;
+// A bin of more than kSortBigBin fine memberships is placed in slices of
+// kSortSlice coarse memberships, each by any workgroup: the slice's cells
+// counted in LDS, every cell's range reserved with one device atomic on that
+// cell's cursor, the memberships written there (strip items carry
+// kSortSliceTag in their last word, the slice index in the second).
+constexpr uint32_t kSortSlice = 8192;
+constexpr uint32_t kSortSliceTag = 0xFFFFFFFFu;
 
 // Tile-pair items of a cell of n memberships (none past the tile limit: the
 // call then fails with kErrTooDense).
@@ -2749,7 +2757,8 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
                                                              const InstDesc* __restrict__ inst, uint32_t n_bins,
                                                              const Meta* __restrict__ bmeta,
                                                              uint32_t* __restrict__ cell_start,
-                                                             uint32_t* __restrict__ bin_end, uint64_t cap_memb,
+                                                             uint32_t* __restrict__ bin_end,
+                                                             uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
                                                              uint4* __restrict__ items, uint64_t cap_items,
                                                              uint4* __restrict__ strips, uint64_t cap_strips,
                                                              Meta* __restrict__ meta) {
@@ -2837,6 +2846,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
         const uint32_t c = tid * kSortPer + k;
         if (fits && c < kk) {
           cell_start[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
+          if (F > kSortBigBin) cell_cur[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
           const uint32_t n = v[k];
           const uint32_t nt = (n + kTile - 1u) / kTile;
           if (n >= 2u && nt <= kMaxTilesPerCell) {
@@ -2871,9 +2881,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
       for (uint32_t k = 0; k < kSortPer; ++k) {
         const uint32_t c = tid * kSortPer + k;
         const uint32_t pw = k ? win[k - 1] : before;
-        // (a bin of more than kSortBigBin memberships is one strip, written
-        // in place: its strips would each read the whole bin again)
-        if (c < kk && (c == 0u || (win[k] != pw && F <= kSortBigBin))) open |= 1u << k;
+        if (c < kk && (c == 0u || win[k] != pw)) open |= 1u << k;
       }
       const uint32_t nopen = (uint32_t)__popc(open);
       const uint32_t sinc = wave_prefix_sum(nopen);
@@ -2887,13 +2895,16 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
 #pragma unroll
       for (uint32_t k = 0; k < kSortPer; ++k)
         if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
+      const bool sliced = F > kSortBigBin;
+      if (sliced) nst = (m + kSortSlice - 1u) / kSortSlice;
       if (tid == 0) L.sbase = fits ? atomicAdd(&meta->dev_chunks, nst) : 0u;
       __syncthreads();
       const uint32_t sb = L.sbase;
       if (fits) {
         for (uint32_t t = tid; t < nst; t += kSortThreads) {
-          const uint32_t lo = L.sl[t], hi = t + 1u < nst ? L.sl[t + 1u] : kk;
-          if ((uint64_t)sb + t < cap_strips) strips[sb + t] = make_uint4(b, lo, hi, 0u);
+          const uint4 item = sliced ? make_uint4(b, t, 0u, kSortSliceTag)
+                                    : make_uint4(b, L.sl[t], t + 1u < nst ? L.sl[t + 1u] : kk, 0u);
+          if ((uint64_t)sb + t < cap_strips) strips[sb + t] = item;
         }
         if (tid == 0) {
           bin_end[b] = (uint32_t)(fb + F);
@@ -2923,6 +2934,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __res
                                                              const InstDesc* __restrict__ inst,
                                                              const uint32_t* __restrict__ cell_start,
                                                              const uint32_t* __restrict__ bin_end,
+                                                             uint32_t* __restrict__ cell_cur,
                                                              const uint4* __restrict__ strips, uint64_t cap_strips,
                                                              float4* __restrict__ mbox, uint32_t* __restrict__ mid,
                                                              const Meta* __restrict__ meta) {
@@ -2941,6 +2953,63 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __res
     const SortBin B(I, b);
     const uint32_t kk = B.kx * B.ky;
     const uint64_t cb = (uint64_t)b * k2;
+    if (st.w == kSortSliceTag) {
+      // a slice of a large bin: its memberships j0 .. j1 - 1, counted per
+      // cell, each cell's range reserved on the cell's cursor, then written
+      const uint32_t j0 = lo * kSortSlice, j1 = j0 + kSortSlice < m ? j0 + kSortSlice : m;
+      for (uint32_t c = tid; c < kk; c += kSortThreads) L.cur[c] = 0u;
+      __syncthreads();
+      for (uint32_t i0 = j0 + tid; i0 < j1; i0 += kSortThreads * kSortLoads) {
+        float4 qs[kSortLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kSortLoads; ++u) {
+          const uint32_t j = i0 + u * kSortThreads;
+          qs[u] = bbox[s0 + (j < j1 ? j : j1 - 1u)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kSortLoads; ++u) {
+          if (i0 + u * kSortThreads >= j1) break;
+          uint32_t x0, x1, y0, y1, fx, fy;
+          B.cells(I, qs[u], x0, x1, y0, y1, fx, fy);
+          for (uint32_t cy = y0; cy <= y1; ++cy)
+            for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cur[cy * B.kx + cx], 1u);
+        }
+      }
+      __syncthreads();
+      for (uint32_t c = tid; c < kk; c += kSortThreads) {
+        const uint32_t n = L.cur[c];
+        L.cst[c] = n ? atomicAdd(&cell_cur[cb + c], n) : 0u;  // the slice's first slot in cell c
+      }
+      __syncthreads();
+      for (uint32_t i0 = j0 + tid; i0 < j1; i0 += kSortThreads * kSortLoads) {
+        float4 qs[kSortLoads];
+        uint32_t ids[kSortLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kSortLoads; ++u) {
+          const uint32_t j = i0 + u * kSortThreads;
+          const uint64_t at = s0 + (j < j1 ? j : j1 - 1u);
+          qs[u] = bbox[at];
+          ids[u] = bid[at];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kSortLoads; ++u) {
+          if (i0 + u * kSortThreads >= j1) break;
+          uint32_t x0, x1, y0, y1, fx, fy;
+          B.cells(I, qs[u], x0, x1, y0, y1, fx, fy);
+          for (uint32_t cy = y0; cy <= y1; ++cy) {
+            const uint32_t flag_y = cy + B.gy == fy ? kFlagY : 0u;
+            for (uint32_t cx = x0; cx <= x1; ++cx) {
+              const uint32_t c = cy * B.kx + cx;
+              const uint64_t pos = (uint64_t)atomicAdd(&L.cst[c], 1u);
+              mbox[pos] = qs[u];
+              mid[pos] = ids[u] | flag_y | (cx + B.gx == fx ? kFlagX : 0u);
+            }
+          }
+        }
+      }
+      __syncthreads();  // the counters are reused by the next item
+      continue;
+    }
     const uint32_t base = cell_start[cb + lo];
     const uint32_t span = (hi < kk ? cell_start[cb + hi] : bin_end[b]) - base;
     const bool staged = span <= kSortStage;
@@ -3004,12 +3073,15 @@ hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const B
   if ((e = launch_meta_clear(meta, s)) != hipSuccess) return e;
   if ((e = launch_binned_front(ws, boxes, nullptr, bp, s, ev ? ev + 1 : nullptr)) != hipSuccess) return e;
   const uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
+  // (cell starts in gw.cnt; bin ends and then the large bins' cell cursors in gw.start)
+  uint32_t* const cell_cur = gw.start + bp.n_bins;
   k_fine_count<<<grid, kSortThreads, 0, s>>>(ws.bbox, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt,
-                                             gw.start, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
+                                             gw.start, cell_cur, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
                                              gw.cap_strips, meta);
   if (ev && (e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
   k_fine_place<<<grid, kSortThreads, sizeof(PlaceLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, gw.cnt,
-                                                            gw.start, gw.strips, gw.cap_strips, gw.mbox, gw.mid, meta);
+                                                            gw.start, cell_cur, gw.strips, gw.cap_strips, gw.mbox,
+                                                            gw.mid, meta);
   if (ev && (e = hipEventRecord(ev[7], s)) != hipSuccess) return e;
   if ((e = launch_tile_test(gw, meta, out, cap, s)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[8], s)) != hipSuccess) return e;

@@ -661,7 +661,8 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
 int size_sorted(pbig_hip_ctx* c, uint32_t n) {
   const InstDesc& d = c->plain.inst[0];
   int rc;
-  if ((rc = ensure_cells(c, std::max<uint64_t>((uint64_t)d.nb * d.k * d.k, 1)))) return rc;
+  // cell starts (bins x k^2), and bin ends then the large bins' cell cursors (bins + bins x k^2)
+  if ((rc = ensure_cells(c, (uint64_t)d.nb + (uint64_t)d.nb * d.k * d.k + 1))) return rc;
   if ((rc = grow(c, &c->ws.strips, &c->ws.cap_strips,
                  std::max<uint64_t>(c->ws.cap_strips, 2ull * d.nb + (uint64_t)n / 1024 + 1024))))
     return rc;
