@@ -279,12 +279,46 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t& stage_n, uin
   stage_n = 0;
 }
 
+// One tile-pair item: rows r (lane l holds row box rt*64+l) against the
+// column tile broadcast one column at a time.  Every condition is a lane mask:
+// the four `<=` tests, the reference-cell rule as (row flags | column flags) ==
+// both, the valid rows, and on a diagonal item the rows below the column.
+template <bool kDiag>
+__device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint32_t rf, uint64_t vm,
+                                             const float4& cb, uint32_t cw, uint32_t ncol, uint2* stage,
+                                             uint32_t& stage_n, uint2* __restrict__ out, uint64_t cap,
+                                             Meta* __restrict__ meta) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t jj = 0; jj < ncol; ++jj) {
+    const float bx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.x), (int)jj));
+    const float by = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.y), (int)jj));
+    const float bz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.z), (int)jj));
+    const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.w), (int)jj));
+    const uint32_t bw = (uint32_t)__builtin_amdgcn_readlane((int)cw, (int)jj);
+    uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
+                 __ballot((rf | (bw >> 30)) == 3u) & vm;
+    if (kDiag) m &= (1ull << jj) - 1ull;  // rows above the diagonal only
+    if (m) {
+      if ((m >> lane) & 1ull) {
+        const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t bid = bw & kIdMask;
+        stage[pos] = make_uint2(aid < bid ? aid : bid, aid < bid ? bid : aid);
+      }
+      stage_n += (uint32_t)__popcll(m);
+      if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
                                                const float4* __restrict__ mbox,
                                                const uint32_t* __restrict__ mid,
                                                uint2* __restrict__ out, uint64_t cap,
                                                Meta* __restrict__ meta) {
   __shared__ uint2 stage_all[4][kStage];
+  __shared__ uint32_t tail_n[4];
+  __shared__ uint64_t tail_base;
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   uint2* stage = stage_all[wib];
@@ -295,12 +329,16 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
   const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const bool ok = meta->error == 0ull;
 
-  for (uint64_t it = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib; ok && it < n_items; it += nwaves) {
-    const uint4 d = items[it];
+  // The next item's descriptor is loaded while the current one is tested.
+  uint64_t it = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+  uint4 d = make_uint4(0u, 0u, 0u, 0u);
+  if (ok && it < n_items) d = items[it];
+  for (; ok && it < n_items; it += nwaves) {
     const uint32_t base = __builtin_amdgcn_readfirstlane(d.x);
     const uint32_t n = __builtin_amdgcn_readfirstlane(d.y);
     const uint32_t rt = __builtin_amdgcn_readfirstlane(d.z);
     const uint32_t ct = __builtin_amdgcn_readfirstlane(d.w);
+    if (it + nwaves < n_items) d = items[it + nwaves];
 
     const uint32_t i = rt * kTile + lane;
     const bool valid = i < n;
@@ -310,11 +348,7 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
       a = mbox[base + i];
       aid = mid[base + i];
     }
-    const bool afx = (aid & kFlagX) != 0, afy = (aid & kFlagY) != 0;
-    aid &= kIdMask;
-
-    // the column tile: lane l holds column j0 + l, broadcast to the wave
-    // one column at a time from registers (no memory access per column)
+    // the column tile: lane l holds column j0 + l
     const uint32_t j0 = ct * kTile;
     const uint32_t ncol = n - j0 < kTile ? n - j0 : kTile;
     float4 cb = make_float4(0.f, -1.f, 0.f, -1.f);
@@ -323,30 +357,28 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
       cb = mbox[base + j0 + lane];
       cw = mid[base + j0 + lane];
     }
-    const bool diag = rt == ct;
-    for (uint32_t jj = 0; jj < ncol; ++jj) {
-      const float bx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.x), (int)jj));
-      const float by = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.y), (int)jj));
-      const float bz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.z), (int)jj));
-      const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.w), (int)jj));
-      const uint32_t bw = (uint32_t)__builtin_amdgcn_readlane((int)cw, (int)jj);
-      const bool bfx = (bw & kFlagX) != 0, bfy = (bw & kFlagY) != 0;
-      bool hit = valid && (a.x <= by) && (bx <= a.y) && (a.z <= bwv) && (bz <= a.w) && (afx || bfx) && (afy || bfy);
-      if (diag) hit = hit && (jj > (uint32_t)lane);
-      const uint64_t mask = __ballot(hit);
-      if (mask) {
-        const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        if (hit) {
-          const uint32_t bid = bw & kIdMask;
-          stage[pos] = make_uint2(aid < bid ? aid : bid, aid < bid ? bid : aid);
-        }
-        stage_n += (uint32_t)__popcll(mask);
-        if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
-      }
-    }
+    const uint64_t vm = __ballot(valid);
+    const uint32_t rf = aid >> 30;  // (start column, start row) flags of the row box
+    aid &= kIdMask;
+    if (rt == ct)
+      test_columns<true>(a, aid, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
+    else
+      test_columns<false>(a, aid, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
   }
-  if (stage_n) flush_stage(stage, stage_n, out, cap, meta);
+  // what is left in the four stages goes out under one reservation
+  if (lane == 0) tail_n[wib] = stage_n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t t = (uint64_t)tail_n[0] + tail_n[1] + tail_n[2] + tail_n[3];
+    tail_base = t ? atomicAdd((unsigned long long*)&meta->n_pairs, (unsigned long long)t) : 0ull;
+  }
+  __syncthreads();
+  uint64_t g = tail_base;
+  for (int k = 0; k < wib; ++k) g += tail_n[k];
+  for (uint32_t k = lane; k < stage_n; k += 64) {
+    const uint64_t dst = g + k;
+    if (dst < cap) out[dst] = stage[k];
+  }
 }
 
 }  // namespace pbig
