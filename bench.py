@@ -140,7 +140,7 @@ def read_traffic(path_name, cs, bin_target):
 
 def clustered_leg(torch, eng_cls, steps=50, reps=5):
     """The clustered 2,097,152-box instance (seed 3), boxes resident, at the
-    autotuner's cell size.
+    autotuner's cell size and bin target.
     * One synchronous call per step, per path: the binned path forced (its
       heavy coarse bins are re-binned, which needs the host between levels),
       the general path forced, and the automatic choice.
@@ -168,10 +168,11 @@ def clustered_leg(torch, eng_cls, steps=50, reps=5):
     e.close()
     C_ref, M_ref = reference_grid(boxes, L, ref_cs)
     B = alg_bytes(n, C_ref, M_ref, k_ref)
+    bin_target = int(autotune.load_table()["bin_target"])
     for name, path in (("binned", pbig_amd.PATH_BINNED), ("general", pbig_amd.PATH_GENERAL),
                        ("auto", pbig_amd.PATH_AUTO)):
         e = eng_cls(0)
-        e.configure(path)
+        e.configure(path, bin_target)
         k = e.check_device(d, L, cell, pairs)
         assert k == k_ref, (name, k, k_ref)
         torch.cuda.synchronize()
@@ -212,7 +213,8 @@ def clustered_leg(torch, eng_cls, steps=50, reps=5):
                           "scan_reduce": B["scan"] / 2, "scan_apply": B["scan"] / 2, "clear": 4 * C_ref,
                           "bin_hist": B["count"], "bin_scan": B["scan"], "bin_starts": 0.0,
                           "bin_scatter": B["scatter"], "bin_test": B["test"],
-                          "bin_place": B["count"] + B["scan"] + B["scatter"]}
+                          "bin_place": B["count"] + B["scan"] + B["scatter"],
+                          "fine_count": B["count"], "fine_place": B["scatter"]}
             dom = max(avg, key=lambda kk: avg[kk])
             achieved = kernel_alg.get(dom, 0.0) / (avg[dom] / 1e3) / 1e9
             dom_bytes, dom_ms = kernel_alg.get(dom, 0.0), avg[dom]

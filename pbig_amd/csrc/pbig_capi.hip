@@ -150,6 +150,10 @@ constexpr int kPathOnePass = 3;
 // Timing path id of the general path's tile test on a cell layout built
 // from the bins (launch_sorted).
 constexpr int kPathSorted = 5;
+// Instances of mostly dense clusters take the device levels from this many
+// times levels_min_boxes (other instances with bins beyond LDS from
+// levels_min_boxes).
+constexpr uint64_t kDenseLevelsFactor = 4;
 
 // Device memory of pbig_hip_check_device_noalloc: the whole workspace of a
 // single-instance binned launch, carved per call from this array.  Each
@@ -1312,9 +1316,13 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
   const bool auto_choice = st.bp.dense_exit != 0;
   if (st.to_general && auto_choice) return 1;  // (known from this plan's last call)
   // An instance of mostly dense clusters goes to the device levels when it
-  // has levels_min_boxes or more (level 0 then runs without the dense exit),
-  // else to the general path.
-  const bool dense_to_levels = auto_choice && n >= c->cfg.levels_min_boxes && !c->cfg.host_levels;
+  // has kDenseLevelsFactor x levels_min_boxes or more (level 0 then runs
+  // without the dense exit), else to the general path, whose per-bin cell
+  // layout is the faster of the two below that size (clustered instances of
+  // 2^20 and 2^21 boxes: 0.26 vs 0.46 ms and 0.40 vs 0.53 ms; of 2^23: 1.33
+  // vs 1.19 ms).
+  const bool dense_to_levels =
+      auto_choice && (uint64_t)n >= kDenseLevelsFactor * c->cfg.levels_min_boxes && !c->cfg.host_levels;
   Meta m;
   int rc = run_onepass_sync(c, d_boxes, d_out, cap, s, &m);
   if (rc < 0) return rc;

@@ -426,19 +426,20 @@ def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors, level
     return settled
 
 
-@pytest.mark.parametrize("levels_min", [0, 1 << 22])
+@pytest.mark.parametrize("levels_min", [0, 1 << 19])
 def test_auto_dense_instance_settles(torch_cuda, orc, manifest, levels_min):
     """The clustered instance under the automatic choice: the first
     asynchronous call tries the single-pass launch, whose dense bins outgrow
     their fixed share of the bucket; it stops with error bit 5 and no pairs,
     and with bit 4 too, since the bins' totals (counted to the end) put over
     a quarter of the memberships in bins beyond LDS (mostly dense clusters).
-    The synchronous call then sends the instance (2^21 boxes) to the device
-    levels at the default levels_min_boxes (2^20), or to the general path
-    when levels_min_boxes is above its size, and the next asynchronous calls
-    follow it, with the exact set."""
+    The synchronous call then sends the instance (2^21 boxes) to the general
+    path at the default levels_min_boxes (2^20: mostly dense instances take
+    the device levels from four times that), or to the device levels when
+    four times levels_min_boxes is within its size, and the next
+    asynchronous calls follow it, with the exact set."""
     settled = _dense_instance_settles(torch_cuda, orc, manifest, "auto", (16, 32, 48), levels_min)
-    assert settled == (PATHS["binned"] if levels_min == 0 else PATHS["general"])
+    assert settled == (PATHS["general"] if levels_min == 0 else PATHS["binned"])
 
 
 def test_binned_dense_instance_on_device_levels(torch_cuda, orc, manifest):
