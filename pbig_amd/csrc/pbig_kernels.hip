@@ -14,7 +14,8 @@
 //                  id so the test reads each cell contiguously (reference
 //                  d_mapping, pbig.cu:321-427, stores only ids)
 //   k_test         persistent waves over the tile-pair items: all-pairs test
-//                  with the column box broadcast from scalar registers, the
+//                  with the column box broadcast from scalar registers (a
+//                  tile against itself: lanes rotated across the wave), the
 //                  reference-cell rule, and ballot/mbcnt compaction into a
 //                  per-wave LDS stage flushed with one atomic per flush
 //                  (reference d_matching + prefix_scan, pbig.cu:476-543,
@@ -279,11 +280,10 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t& stage_n, uin
   stage_n = 0;
 }
 
-// One tile-pair item: rows r (lane l holds row box rt*64+l) against the
-// column tile broadcast one column at a time.  Every condition is a lane mask:
-// the four `<=` tests, the reference-cell rule as (row flags | column flags) ==
-// both, the valid rows, and on a diagonal item the rows below the column.
-template <bool kDiag>
+// An item of two different tiles: rows (lane l holds row box rt*64+l)
+// against the column tile broadcast one column at a time.  Every condition is
+// a lane mask: the four `<=` tests, the reference-cell rule as (row flags |
+// column flags) == both, and the valid rows.
 __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint32_t rf, uint64_t vm,
                                              const float4& cb, uint32_t cw, uint32_t ncol, uint2* stage,
                                              uint32_t& stage_n, uint2* __restrict__ out, uint64_t cap,
@@ -295,9 +295,43 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
     const float bz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.z), (int)jj));
     const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.w), (int)jj));
     const uint32_t bw = (uint32_t)__builtin_amdgcn_readlane((int)cw, (int)jj);
+    const uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
+                       __ballot((rf | (bw >> 30)) == 3u) & vm;
+    if (m) {
+      if ((m >> lane) & 1ull) {
+        const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t bid = bw & kIdMask;
+        stage[pos] = make_uint2(aid < bid ? aid : bid, aid < bid ? bid : aid);
+      }
+      stage_n += (uint32_t)__popcll(m);
+      if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
+    }
+  }
+}
+
+// A diagonal item (a tile against itself, w boxes on lanes 0 .. w-1): at step
+// k = 1 .. w/2 lane l tests the box of lane (l + k) mod w, fetched across the
+// wave, so every unordered pair is tested once (at k = w/2 of an even w only
+// by the lanes below w/2) in half the steps of a column broadcast.
+__device__ __forceinline__ void test_diagonal(const float4& a, uint32_t aw, uint64_t vm, uint32_t w,
+                                              uint2* stage, uint32_t& stage_n, uint2* __restrict__ out,
+                                              uint64_t cap, Meta* __restrict__ meta) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t aid = aw & kIdMask, rf = aw >> 30;
+  const uint32_t half = w >> 1;
+  uint32_t src = (uint32_t)lane;  // stays in 0 .. 63 on every lane
+  for (uint32_t k = 1; k <= half; ++k) {
+    src = src + 1u >= w ? src + 1u - w : src + 1u;
+    const int addr = (int)(src << 2);
+    const float bx = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.x)));
+    const float by = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.y)));
+    const float bz = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.z)));
+    const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.w)));
+    const uint32_t bw = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)aw);
     uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
                  __ballot((rf | (bw >> 30)) == 3u) & vm;
-    if (kDiag) m &= (1ull << jj) - 1ull;  // rows above the diagonal only
+    if (2u * k == w) m &= (1ull << half) - 1ull;
     if (m) {
       if ((m >> lane) & 1ull) {
         const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -348,22 +382,22 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
       a = mbox[base + i];
       aid = mid[base + i];
     }
-    // the column tile: lane l holds column j0 + l
+    const uint64_t vm = __ballot(valid);
     const uint32_t j0 = ct * kTile;
     const uint32_t ncol = n - j0 < kTile ? n - j0 : kTile;
+    if (rt == ct) {
+      test_diagonal(a, aid, vm, ncol, stage, stage_n, out, cap, meta);
+      continue;
+    }
+    // the column tile: lane l holds column j0 + l
     float4 cb = make_float4(0.f, -1.f, 0.f, -1.f);
     uint32_t cw = 0;
     if (lane < (int)ncol) {
       cb = mbox[base + j0 + lane];
       cw = mid[base + j0 + lane];
     }
-    const uint64_t vm = __ballot(valid);
     const uint32_t rf = aid >> 30;  // (start column, start row) flags of the row box
-    aid &= kIdMask;
-    if (rt == ct)
-      test_columns<true>(a, aid, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
-    else
-      test_columns<false>(a, aid, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
+    test_columns(a, aid & kIdMask, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
   }
   // what is left in the four stages goes out under one reservation
   if (lane == 0) tail_n[wib] = stage_n;
