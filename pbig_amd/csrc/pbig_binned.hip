@@ -2681,12 +2681,15 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 // in fine-cell order, every cell's run inside the bin's share of the
 // membership array, with the box's start-column and start-row flags, and
 // the 64 x 64 tile-pair items of the bin's cells.  Two kernels:
-//   k_fine_count  one persistent workgroup per bin in turn: fine counts in
-//                 LDS, their scan, the bin's shares of the membership and
-//                 item arrays (two device atomics per bin, not one per
-//                 membership), the items, every cell's absolute start, and
-//                 the bin's strips (row-major runs of cells starting within
-//                 one window of kSortStage / 2 memberships) on a work list;
+//   k_fine_count  persistent workgroups over the bins: fine counts in LDS
+//                 (a bin of more than kCountSlice memberships counted in
+//                 slices spread over the chip, then summed by the workgroup
+//                 that counts its last slice), their scan, the bin's shares
+//                 of the membership, item and strip arrays (three device
+//                 atomics per bin, not one per membership), the items, every
+//                 cell's absolute start, and the bin's strips (row-major runs
+//                 of cells starting within one window of kSortStage / 2
+//                 memberships) on a work list;
 //   k_fine_place  persistent workgroups over the strips of all bins: the
 //                 bin's memberships stream from the bucket, the strip's ones
 //                 go to an LDS stage in cell order and out in one run (a
@@ -2745,12 +2748,175 @@ struct SortBin {
 struct CountLds {
   uint32_t cnt[kMaxCellsPerBin];
   uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
+  uint32_t soff[kSortThreads + 1];  // slices of a chunk of bins: inclusive scan
   uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
   uint32_t ws[kSortThreads / 64];
   unsigned long long fbase, ibase;
-  uint32_t bmax, sbase;
+  uint32_t bmax, sbase, last;
 };
 
+// A bin's fine counts of memberships j0 .. j1 - 1 of the bucket into L.cnt.
+__device__ __forceinline__ void count_fine(CountLds& L, const InstDesc& I, const SortBin& B,
+                                           const float4* __restrict__ bbox, uint64_t s0, uint32_t j0,
+                                           uint32_t j1) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i0 = j0 + tid; i0 < j1; i0 += kSortThreads * kSortLoads) {
+    float4 q[kSortLoads];
+#pragma unroll
+    for (uint32_t u = 0; u < kSortLoads; ++u) {
+      const uint32_t j = i0 + u * kSortThreads;
+      q[u] = bbox[s0 + (j < j1 ? j : j1 - 1u)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kSortLoads; ++u) {
+      if (i0 + u * kSortThreads >= j1) break;
+      uint32_t x0, x1, y0, y1, fx, fy;
+      B.cells(I, q[u], x0, x1, y0, y1, fx, fy);
+      for (uint32_t cy = y0; cy <= y1; ++cy)
+        for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cnt[cy * B.kx + cx], 1u);
+    }
+  }
+}
+
+// Bin b's layout from its fine counts in L.cnt (cleared again on return).
+__device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint32_t b, uint32_t m,
+                                        uint32_t* __restrict__ cell_start, uint32_t* __restrict__ bin_end,
+                                        uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
+                                        uint4* __restrict__ items, uint64_t cap_items,
+                                        uint4* __restrict__ strips, uint64_t cap_strips, Meta* __restrict__ meta) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t k2 = I.k * I.k;
+  const SortBin B(I, b);
+  const uint32_t kk = B.kx * B.ky;
+  // 1. exclusive scans of the cells' memberships and items (four cells
+  //    per thread, 64-bit sums: a share past 2^32 - 1 cannot fit and
+  //    fails the call with kErrWorkspace)
+  uint32_t v[kSortPer], vmax = 0;
+  uint64_t acc = 0, iacc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kSortPer; ++k) {
+    const uint32_t c = tid * kSortPer + k;
+    v[k] = c < kk ? L.cnt[c] : 0u;
+    acc += v[k];
+    iacc += tile_items(v[k]);
+    vmax = v[k] > vmax ? v[k] : vmax;
+  }
+  const uint64_t inc = wave_prefix_sum64(acc), iinc = wave_prefix_sum64(iacc);
+  if (lane == 63u) {
+    L.wm[w] = inc;
+    L.wi[w] = iinc;
+  }
+  if (vmax) atomicMax(&L.bmax, vmax);
+  __syncthreads();  // (the counts are in registers from here: L.cnt is free until it is cleared)
+  uint64_t e = inc - acc, ie = iinc - iacc, F = 0, II = 0;
+  for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+    if (k < w) {
+      e += L.wm[k];
+      ie += L.wi[k];
+    }
+    F += L.wm[k];
+    II += L.wi[k];
+  }
+  // 2. the strips: cell 0 opens one, and so does every cell whose start
+  //    lies in a later window of kSortWindow memberships than its
+  //    predecessor's (a strip then spans less than two windows unless its
+  //    last cell alone is larger); a bin of more than kSortBigBin
+  //    memberships is placed in slices instead
+  uint32_t win[kSortPer];
+  {
+    uint64_t e3 = e;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortPer; ++k) {
+      win[k] = (uint32_t)(e3 / kSortWindow);
+      e3 += v[k];
+    }
+  }
+  L.cnt[tid] = win[kSortPer - 1];  // the window of this thread's last cell
+  __syncthreads();
+  const uint32_t before = tid ? L.cnt[tid - 1u] : 0xFFFFFFFFu;
+  uint32_t open = 0;  // bit k: this thread's cell k opens a strip
+#pragma unroll
+  for (uint32_t k = 0; k < kSortPer; ++k) {
+    const uint32_t c = tid * kSortPer + k;
+    const uint32_t pw = k ? win[k - 1] : before;
+    if (c < kk && (c == 0u || win[k] != pw)) open |= 1u << k;
+  }
+  const uint32_t nopen = (uint32_t)__popc(open);
+  const uint32_t sinc = wave_prefix_sum(nopen);
+  if (lane == 63u) L.ws[w] = sinc;
+  __syncthreads();
+  uint32_t sidx = sinc - nopen, nst = 0;
+  for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+    if (k < w) sidx += L.ws[k];
+    nst += L.ws[k];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kSortPer; ++k)
+    if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
+  const bool sliced = F > kSortBigBin;
+  if (sliced) nst = (m + kSortSlice - 1u) / kSortSlice;
+  // 3. the bin's shares of the membership, item and strip arrays: three
+  //    device atomics from three waves, in flight together (a share that
+  //    does not fit fails the call; the kernels after this one then stop
+  //    at the error word before they read a strip)
+  if (tid == 0u) L.fbase = atomicAdd((unsigned long long*)&meta->memberships, (unsigned long long)F);
+  if (tid == 64u) L.ibase = atomicAdd((unsigned long long*)&meta->items, (unsigned long long)II);
+  if (tid == 128u) L.sbase = atomicAdd(&meta->dev_chunks, nst);
+  if (tid == 192u) {
+    atomicMax(&meta->max_per_cell, L.bmax);
+    if (L.bmax > kTile * kMaxTilesPerCell) atomicOr((unsigned long long*)&meta->error, kErrTooDense);
+  }
+  __syncthreads();
+  const uint64_t fb = L.fbase, ib = L.ibase;
+  const uint32_t sb = L.sbase;
+  const bool fits = F < (1ull << 32) && II < (1ull << 32) && fb + F <= cap_memb && ib + II <= cap_items;
+  if (!fits && tid == 0) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
+  // 4. every cell's absolute start, the items, and the strips
+  if (fits) {
+    uint64_t e2 = e, ie2 = ie;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortPer; ++k) {
+      const uint32_t c = tid * kSortPer + k;
+      if (c < kk) {
+        cell_start[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
+        if (sliced) cell_cur[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
+        const uint32_t n = v[k];
+        const uint32_t nt = (n + kTile - 1u) / kTile;
+        if (n >= 2u && nt <= kMaxTilesPerCell) {
+          uint64_t at = ib + ie2;
+          const uint32_t st = (uint32_t)(fb + e2);
+          for (uint32_t r = 0; r < nt; ++r)
+            for (uint32_t q = r; q < nt; ++q) items[at++] = make_uint4(st, n, r, q);
+        }
+      }
+      e2 += v[k];
+      ie2 += tile_items(v[k]);
+    }
+    for (uint32_t t = tid; t < nst; t += kSortThreads) {
+      const uint4 item = sliced ? make_uint4(b, t, 0u, kSortSliceTag)
+                                : make_uint4(b, L.sl[t], t + 1u < nst ? L.sl[t + 1u] : kk, 0u);
+      if ((uint64_t)sb + t < cap_strips) strips[sb + t] = item;
+    }
+    if (tid == 0) {
+      bin_end[b] = (uint32_t)(fb + F);
+      if ((uint64_t)sb + nst > cap_strips) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
+    }
+  }
+  __syncthreads();
+  L.cnt[tid] = 0u;  // (step 2's window word, also where kk is below kSortThreads)
+  for (uint32_t c = tid + kSortThreads; c < kk; c += kSortThreads) L.cnt[c] = 0u;
+  if (tid == 0) L.bmax = 0u;
+  __syncthreads();
+}
+
+// Bins of more than kCountSlice memberships are counted in slices of
+// kCountSlice by any workgroup: a slice adds its fine counts to the bin's
+// row of slc (after the bins' completion counters) with device atomics, and
+// the workgroup that counts a bin's last slice takes the sums with atomic
+// exchanges (the row is zero again for the next launch) and lays the bin
+// out.  Slices are numbered over the bins in order, chunk by chunk, and
+// workgroup g takes the slices g, g + grid, ...; the other bins go to
+// workgroup b mod grid.
 __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __restrict__ bbox,
                                                              const uint32_t* __restrict__ bin_start,
                                                              const uint32_t* __restrict__ total,
@@ -2761,6 +2927,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
                                                              uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
                                                              uint4* __restrict__ items, uint64_t cap_items,
                                                              uint4* __restrict__ strips, uint64_t cap_strips,
+                                                             uint32_t* __restrict__ slc, uint64_t cap_slc,
                                                              Meta* __restrict__ meta) {
   __shared__ CountLds L;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -2772,150 +2939,71 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
   }
   const InstDesc I = inst[0];
   const uint32_t k2 = I.k * I.k;
+  uint32_t* const done = slc;           // per bin: slices counted (zeroed before the launch)
+  uint32_t* const rows = slc + n_bins;  // per bin: k2 fine counts (zero between launches)
   for (uint32_t c = tid; c < kMaxCellsPerBin; c += kSortThreads) L.cnt[c] = 0u;
   if (tid == 0) L.bmax = 0u;
   __syncthreads();
+  // 1. slices of the large bins
+  uint32_t gbase = 0;
+  for (uint32_t base = 0; base < n_bins; base += kSortThreads) {
+    const uint32_t bb = base + tid;
+    const uint32_t mm = bb < n_bins ? total[bb] : 0u;
+    const uint32_t ns = mm > kCountSlice ? (mm + kCountSlice - 1u) / kCountSlice : 0u;
+    const uint32_t inc = wave_prefix_sum(ns);
+    if (lane == 63u) L.ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = inc, T = 0;
+    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+      if (k < w) pre += L.ws[k];
+      T += L.ws[k];
+    }
+    L.soff[tid + 1u] = pre;
+    if (tid == 0) L.soff[0] = 0u;
+    __syncthreads();
+    const uint32_t skew = gbase % gridDim.x;
+    for (uint32_t r = (blockIdx.x + gridDim.x - skew) % gridDim.x; r < T; r += gridDim.x) {
+      // the bin of chunk slice r: soff[x] <= r < soff[x + 1]
+      uint32_t lo = 0, hi = kSortThreads;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.soff[mid] <= r) lo = mid; else hi = mid;
+      }
+      const uint32_t b = base + lo, j = r - L.soff[lo];
+      const uint32_t m = total[b], nsb = L.soff[lo + 1u] - L.soff[lo];
+      const SortBin B(I, b);
+      const uint32_t kk = B.kx * B.ky;
+      const uint32_t j0 = j * kCountSlice, j1 = j0 + kCountSlice < m ? j0 + kCountSlice : m;
+      count_fine(L, I, B, bbox, bin_start[b], j0, j1);
+      __syncthreads();
+      uint32_t seen = 0;
+      for (uint32_t c = tid; c < kk; c += kSortThreads) {
+        if (L.cnt[c]) seen |= atomicAdd(&rows[(uint64_t)b * k2 + c], L.cnt[c]);
+        L.cnt[c] = 0u;
+      }
+      // every add has returned, so it is performed at device scope, before
+      // this workgroup counts the slice as done
+      asm volatile("" ::"v"(seen));
+      __syncthreads();
+      if (tid == 0) L.last = atomicAdd(&done[b], 1u) == nsb - 1u;
+      __syncthreads();
+      if (L.last) {  // every slice of bin b is counted: take the sums
+        for (uint32_t c = tid; c < kk; c += kSortThreads) L.cnt[c] = atomicExch(&rows[(uint64_t)b * k2 + c], 0u);
+        __syncthreads();
+        layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips, meta);
+      }
+    }
+    gbase += T;
+    __syncthreads();  // (soff and ws are reused by the next chunk)
+  }
+  // 2. the other bins, one workgroup each
   for (uint32_t b = blockIdx.x; b < n_bins; b += gridDim.x) {
     const uint32_t m = total[b];
-    if (m == 0u) continue;  // (uniform over the block)
-    const uint64_t s0 = bin_start[b];
+    if (m == 0u || m > kCountSlice) continue;  // (uniform over the block)
     const SortBin B(I, b);
-    const uint32_t kk = B.kx * B.ky;
-    // 1. fine counts
-    for (uint32_t j0 = tid; j0 < m; j0 += kSortThreads * kSortLoads) {
-      float4 q[kSortLoads];
-#pragma unroll
-      for (uint32_t u = 0; u < kSortLoads; ++u) {
-        const uint32_t j = j0 + u * kSortThreads;
-        q[u] = bbox[s0 + (j < m ? j : m - 1u)];
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < kSortLoads; ++u) {
-        if (j0 + u * kSortThreads >= m) break;
-        uint32_t x0, x1, y0, y1, fx, fy;
-        B.cells(I, q[u], x0, x1, y0, y1, fx, fy);
-        for (uint32_t cy = y0; cy <= y1; ++cy)
-          for (uint32_t cx = x0; cx <= x1; ++cx) atomicAdd(&L.cnt[cy * B.kx + cx], 1u);
-      }
-    }
+    count_fine(L, I, B, bbox, bin_start[b], 0u, m);
     __syncthreads();
-    // 2. exclusive scans of the cells' memberships and items (four cells
-    //    per thread, 64-bit sums: a share past 2^32 - 1 cannot fit and
-    //    fails the call with kErrWorkspace)
-    uint32_t v[kSortPer], vmax = 0;
-    uint64_t acc = 0, iacc = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kSortPer; ++k) {
-      const uint32_t c = tid * kSortPer + k;
-      v[k] = c < kk ? L.cnt[c] : 0u;
-      acc += v[k];
-      iacc += tile_items(v[k]);
-      vmax = v[k] > vmax ? v[k] : vmax;
-    }
-    const uint64_t inc = wave_prefix_sum64(acc), iinc = wave_prefix_sum64(iacc);
-    if (lane == 63u) {
-      L.wm[w] = inc;
-      L.wi[w] = iinc;
-    }
-    if (vmax) atomicMax(&L.bmax, vmax);
-    __syncthreads();
-    uint64_t e = inc - acc, ie = iinc - iacc, F = 0, II = 0;
-    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-      if (k < w) {
-        e += L.wm[k];
-        ie += L.wi[k];
-      }
-      F += L.wm[k];
-      II += L.wi[k];
-    }
-    if (tid == 0) {
-      L.fbase = atomicAdd((unsigned long long*)&meta->memberships, (unsigned long long)F);
-      L.ibase = atomicAdd((unsigned long long*)&meta->items, (unsigned long long)II);
-      atomicMax(&meta->max_per_cell, L.bmax);
-      if (L.bmax > kTile * kMaxTilesPerCell) atomicOr((unsigned long long*)&meta->error, kErrTooDense);
-    }
-    __syncthreads();
-    const uint64_t fb = L.fbase, ib = L.ibase;
-    const bool fits = F < (1ull << 32) && II < (1ull << 32) && fb + F <= cap_memb && ib + II <= cap_items;
-    if (!fits && tid == 0) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
-    // 3. every cell's absolute start, and the items
-    {
-      uint64_t e2 = e, ie2 = ie;
-#pragma unroll
-      for (uint32_t k = 0; k < kSortPer; ++k) {
-        const uint32_t c = tid * kSortPer + k;
-        if (fits && c < kk) {
-          cell_start[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
-          if (F > kSortBigBin) cell_cur[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
-          const uint32_t n = v[k];
-          const uint32_t nt = (n + kTile - 1u) / kTile;
-          if (n >= 2u && nt <= kMaxTilesPerCell) {
-            uint64_t at = ib + ie2;
-            const uint32_t st = (uint32_t)(fb + e2);
-            for (uint32_t r = 0; r < nt; ++r)
-              for (uint32_t q = r; q < nt; ++q) items[at++] = make_uint4(st, n, r, q);
-          }
-        }
-        e2 += v[k];
-        ie2 += tile_items(v[k]);
-      }
-    }
-    // 4. the strips: cell 0 opens one, and so does every cell whose start
-    //    lies in a later window of kSortWindow memberships than its
-    //    predecessor's (a strip then spans less than two windows unless its
-    //    last cell alone is larger)
-    {
-      uint32_t win[kSortPer];
-      uint64_t e3 = e;
-#pragma unroll
-      for (uint32_t k = 0; k < kSortPer; ++k) {
-        win[k] = (uint32_t)(e3 / kSortWindow);
-        e3 += v[k];
-      }
-      L.sl[tid] = win[kSortPer - 1];  // (for now: the window of this thread's last cell)
-      __syncthreads();
-      const uint32_t before = tid ? L.sl[tid - 1u] : 0xFFFFFFFFu;
-      __syncthreads();
-      uint32_t open = 0;  // bit k: this thread's cell k opens a strip
-#pragma unroll
-      for (uint32_t k = 0; k < kSortPer; ++k) {
-        const uint32_t c = tid * kSortPer + k;
-        const uint32_t pw = k ? win[k - 1] : before;
-        if (c < kk && (c == 0u || win[k] != pw)) open |= 1u << k;
-      }
-      const uint32_t nopen = (uint32_t)__popc(open);
-      const uint32_t sinc = wave_prefix_sum(nopen);
-      if (lane == 63u) L.ws[w] = sinc;
-      __syncthreads();
-      uint32_t sidx = sinc - nopen, nst = 0;
-      for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-        if (k < w) sidx += L.ws[k];
-        nst += L.ws[k];
-      }
-#pragma unroll
-      for (uint32_t k = 0; k < kSortPer; ++k)
-        if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
-      const bool sliced = F > kSortBigBin;
-      if (sliced) nst = (m + kSortSlice - 1u) / kSortSlice;
-      if (tid == 0) L.sbase = fits ? atomicAdd(&meta->dev_chunks, nst) : 0u;
-      __syncthreads();
-      const uint32_t sb = L.sbase;
-      if (fits) {
-        for (uint32_t t = tid; t < nst; t += kSortThreads) {
-          const uint4 item = sliced ? make_uint4(b, t, 0u, kSortSliceTag)
-                                    : make_uint4(b, L.sl[t], t + 1u < nst ? L.sl[t + 1u] : kk, 0u);
-          if ((uint64_t)sb + t < cap_strips) strips[sb + t] = item;
-        }
-        if (tid == 0) {
-          bin_end[b] = (uint32_t)(fb + F);
-          if ((uint64_t)sb + nst > cap_strips) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
-        }
-      }
-    }
-    __syncthreads();
-    for (uint32_t c = tid; c < kk; c += kSortThreads) L.cnt[c] = 0u;
-    if (tid == 0) L.bmax = 0u;
-    __syncthreads();
+    layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips, meta);
   }
 }
 
@@ -3070,14 +3158,15 @@ hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const B
                          Meta* meta, uint2* out, uint64_t cap, hipStream_t s, hipEvent_t* ev) {
   hipError_t e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
-  if ((e = launch_meta_clear(meta, s)) != hipSuccess) return e;
+  // (the result words, and the bins' slice counters of k_fine_count)
+  if ((e = launch_meta_clear(meta, s, gw.slc, bp.n_bins)) != hipSuccess) return e;
   if ((e = launch_binned_front(ws, boxes, nullptr, bp, s, ev ? ev + 1 : nullptr)) != hipSuccess) return e;
   const uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
   // (cell starts in gw.cnt; bin ends and then the large bins' cell cursors in gw.start)
   uint32_t* const cell_cur = gw.start + bp.n_bins;
   k_fine_count<<<grid, kSortThreads, 0, s>>>(ws.bbox, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt,
                                              gw.start, cell_cur, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
-                                             gw.cap_strips, meta);
+                                             gw.cap_strips, gw.slc, gw.cap_slc, meta);
   if (ev && (e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
   k_fine_place<<<grid, kSortThreads, sizeof(PlaceLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, gw.cnt,
                                                             gw.start, cell_cur, gw.strips, gw.cap_strips, gw.mbox,

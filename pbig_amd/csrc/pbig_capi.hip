@@ -662,7 +662,7 @@ hipEvent_t* next_events(pbig_hip_ctx* c, int path) {
 // The general path's tile test on a cell layout built from the bins of the
 // plain state's plan (launch_sorted), synchronously: the bucket, the
 // membership array and the item array grow until the launch fits.
-int size_sorted(pbig_hip_ctx* c, uint32_t n) {
+int size_sorted(pbig_hip_ctx* c, uint32_t n, hipStream_t s) {
   const InstDesc& d = c->plain.inst[0];
   int rc;
   // cell starts (bins x k^2), and bin ends then the large bins' cell cursors (bins + bins x k^2)
@@ -670,6 +670,14 @@ int size_sorted(pbig_hip_ctx* c, uint32_t n) {
   if ((rc = grow(c, &c->ws.strips, &c->ws.cap_strips,
                  std::max<uint64_t>(c->ws.cap_strips, 2ull * d.nb + (uint64_t)n / 1024 + 1024))))
     return rc;
+  // the bins' slice counters, then per bin a row of fine counts that the
+  // counting slices of a large bin add to (zero between launches: the
+  // kernel takes each sum with an exchange against zero)
+  const uint64_t want_slc = (uint64_t)d.nb + (uint64_t)d.nb * d.k * d.k;
+  if (c->ws.cap_slc < want_slc || !c->ws.slc) {
+    if ((rc = grow(c, &c->ws.slc, &c->ws.cap_slc, want_slc))) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->ws.slc, 0, c->ws.cap_slc * sizeof(uint32_t), s));
+  }
   return ensure_work(c, std::max<uint64_t>(c->ws.cap_memb, (uint64_t)n + n / 2 + 1024),
                      std::max<uint64_t>(c->ws.cap_items, (uint64_t)n / 2 + 1024));
 }
@@ -685,7 +693,7 @@ int run_sorted_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
                     uint64_t* n_pairs) {
   BinnedState& st = c->plain;
   int rc;
-  if ((rc = size_sorted(c, n))) return rc;
+  if ((rc = size_sorted(c, n, s))) return rc;
   if ((rc = after_upload(c, st, s))) return rc;
   const pbig::BatchPlan bp = sorted_plan(st);
   for (int attempt = 0; attempt < 3; ++attempt) {
@@ -1586,7 +1594,7 @@ int pbig_hip_destroy(pbig_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Workspace& w = c->ws;
   for (void* p : {(void*)w.cnt, (void*)w.start, (void*)w.partials, (void*)w.items, (void*)w.mbox,
-                  (void*)w.mid, (void*)w.strips, (void*)w.meta, (void*)c->sort_meta, (void*)c->d_in,
+                  (void*)w.mid, (void*)w.strips, (void*)w.slc, (void*)w.meta, (void*)c->sort_meta, (void*)c->d_in,
                   (void*)c->d_pairs})
     if (p) (void)hipFree(p);
   free_state(c->plain);
@@ -1723,7 +1731,7 @@ int pbig_hip_check_device_async(pbig_hip_ctx* c, const float* d_boxes, uint32_t 
     bool planned = false;
     if ((rc = plan_plain(c, n, range, cell_size, s, &planned))) return rc;
     if (planned) {
-      if ((rc = size_sorted(c, n))) return rc;
+      if ((rc = size_sorted(c, n, s))) return rc;
       if ((rc = after_upload(c, c->plain, s))) return rc;
       HIP_TRY(c, pbig::launch_sorted(c->plain.ws, (const float4*)d_boxes, sorted_plan(c->plain), c->ws,
                                      c->sort_meta, (uint2*)d_pairs, cap, s, next_events(c, kPathSorted)));

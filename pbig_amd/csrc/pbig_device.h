@@ -68,8 +68,10 @@ struct Workspace {
   float4* mbox = nullptr;         // cell-ordered box copies
   uint32_t* mid = nullptr;        // cell-ordered ids | start-cell flags
   uint4* strips = nullptr;        // cells laid out from the bins: {bin, first cell, end cell, 0} per strip
+  uint32_t* slc = nullptr;        // cells laid out from the bins: per bin a slice counter, then per
+                                  // counting slice of a large bin its fine counts
   Meta* meta = nullptr;
-  uint64_t cap_cells = 0, cap_partials = 0, cap_items = 0, cap_memb = 0, cap_strips = 0;
+  uint64_t cap_cells = 0, cap_partials = 0, cap_items = 0, cap_memb = 0, cap_strips = 0, cap_slc = 0;
   uint32_t test_blocks = 0;
 };
 
@@ -91,8 +93,9 @@ hipError_t launch_scan(const Workspace& ws, uint32_t n_cells, uint64_t memb_cap,
 hipError_t launch_plain(const Workspace& ws, const float4* boxes, uint32_t n, GridParams gp,
                         uint32_t n_cells, uint2* out, uint64_t cap, hipStream_t s,
                         hipEvent_t* events = nullptr);
-// The result words of a call zeroed (the whole Meta).
-hipError_t launch_meta_clear(Meta* meta, hipStream_t s);
+// The result words of a call zeroed (the whole Meta), and n_words words at
+// `words` (optional).
+hipError_t launch_meta_clear(Meta* meta, hipStream_t s, uint32_t* words = nullptr, uint32_t n_words = 0);
 // k_test over ws's tile-pair items, with the counts and pair counter in meta.
 hipError_t launch_tile_test(const Workspace& ws, Meta* meta, uint2* out, uint64_t cap, hipStream_t s);
 }  // namespace pbig

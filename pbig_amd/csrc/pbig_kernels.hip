@@ -436,13 +436,16 @@ hipError_t launch_scan(const Workspace& ws, uint32_t n_cells, uint64_t memb_cap,
 }
 
 // Every word of a Meta zeroed.
-__global__ __launch_bounds__(64) void k_meta_zero(Meta* __restrict__ meta) {
-  if (threadIdx.x < sizeof(Meta) / sizeof(unsigned long long))
+__global__ __launch_bounds__(256) void k_meta_zero(Meta* __restrict__ meta, uint32_t* __restrict__ words,
+                                                    uint32_t n_words) {
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(Meta) / sizeof(unsigned long long))
     reinterpret_cast<unsigned long long*>(meta)[threadIdx.x] = 0ull;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += gridDim.x * blockDim.x) words[i] = 0u;
 }
 
-hipError_t launch_meta_clear(Meta* meta, hipStream_t s) {
-  k_meta_zero<<<1, 64, 0, s>>>(meta);
+hipError_t launch_meta_clear(Meta* meta, hipStream_t s, uint32_t* words, uint32_t n_words) {
+  const uint32_t blocks = words && n_words ? (n_words + 255u) / 256u : 1u;
+  k_meta_zero<<<blocks < 64u ? blocks : 64u, 256, 0, s>>>(meta, words, words ? n_words : 0u);
   return hipGetLastError();
 }
 
