@@ -214,8 +214,9 @@ hipError_t launch_binned_front(const BinnedWorkspace& ws, const float4* boxes, c
 // starts in gw.cnt, bin ends in gw.start, items, strips; counts and errors
 // in meta, the bucket's in ws.meta) and k_fine_place into gw's memberships,
 // then k_test (pairs counted in meta).  gw.cnt holds bins x k^2 entries;
-// gw.slc bins + bins x k^2, zero before the first launch (a bin of more
-// than kCountSlice memberships is counted in slices of kCountSlice).
+// gw.slc bins + bins / 1024 + 1 + bins x k^2, zero before the first
+// launch (a bin of more than kCountSlice memberships is counted in slices of
+// kCountSlice).
 // ev: kSortedKernels + 1 events.
 constexpr uint32_t kCountSlice = 8192;
 constexpr int kSortedKernels = 8;  // clear, bin_hist, bin_scan, bin_starts, bin_scatter, fine_count, fine_place, test

@@ -2681,16 +2681,16 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 // in fine-cell order, every cell's run inside the bin's share of the
 // membership array, with the box's start-column and start-row flags, and
 // the 64 x 64 tile-pair items of the bin's cells.  Two kernels:
-//   k_fine_count  persistent workgroups over the bins: fine counts in LDS
-//                 (a bin of more than kCountSlice memberships counted in
-//                 slices spread over the chip, then summed by the workgroup
-//                 that counts its last slice), their scan, the bin's shares
+//   k_fine_count  persistent workgroups taking the bins by ticket: fine
+//                 counts in LDS (a bin of more than kCountSlice memberships
+//                 counted in slices spread over the chip, then summed by the
+//                 workgroup that counts its last slice), their scan, the bin's shares
 //                 of the membership, item and strip arrays (three device
 //                 atomics per bin, not one per membership), the items, every
 //                 cell's absolute start, and the bin's strips (row-major runs
 //                 of cells starting within one window of kSortStage / 2
 //                 memberships) on a work list;
-//   k_fine_place  persistent workgroups over the strips of all bins: the
+//   k_fine_place  persistent workgroups taking the strips of all bins by ticket: the
 //                 bin's memberships stream from the bucket, the strip's ones
 //                 go to an LDS stage in cell order and out in one run (a
 //                 strip past the stage, i.e. a cell of more than kSortStage
@@ -2748,11 +2748,12 @@ struct SortBin {
 struct CountLds {
   uint32_t cnt[kMaxCellsPerBin];
   uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
-  uint32_t soff[kSortThreads + 1];  // slices of a chunk of bins: inclusive scan
+  uint32_t soff[kSortThreads + 1];  // a chunk of bins: inclusive scan of their slices
+  uint32_t boff[kSortThreads + 1];  // and of their other bins
   uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
-  uint32_t ws[kSortThreads / 64];
+  uint32_t ws[kSortThreads / 64], wb[kSortThreads / 64];
   unsigned long long fbase, ibase;
-  uint32_t bmax, sbase, last;
+  uint32_t bmax, sbase, last, tick;
 };
 
 // A bin's fine counts of memberships j0 .. j1 - 1 of the bucket into L.cnt.
@@ -2911,12 +2912,13 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
 
 // Bins of more than kCountSlice memberships are counted in slices of
 // kCountSlice by any workgroup: a slice adds its fine counts to the bin's
-// row of slc (after the bins' completion counters) with device atomics, and
-// the workgroup that counts a bin's last slice takes the sums with atomic
-// exchanges (the row is zero again for the next launch) and lays the bin
-// out.  Slices are numbered over the bins in order, chunk by chunk, and
-// workgroup g takes the slices g, g + grid, ...; the other bins go to
-// workgroup b mod grid.
+// row of slc with device atomics, and the workgroup that counts a bin's last
+// slice takes the sums with atomic exchanges (the row is zero again for the
+// next launch) and lays the bin out.  Work is taken by ticket, chunk of
+// kSortThreads bins by chunk: the chunk's slices first (heaviest work), then
+// its other bins, one ticket each.  slc: per bin a slice counter, per chunk
+// a ticket counter (both zeroed before the launch), per bin a row of k^2
+// counts.
 __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __restrict__ bbox,
                                                              const uint32_t* __restrict__ bin_start,
                                                              const uint32_t* __restrict__ total,
@@ -2927,8 +2929,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
                                                              uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
                                                              uint4* __restrict__ items, uint64_t cap_items,
                                                              uint4* __restrict__ strips, uint64_t cap_strips,
-                                                             uint32_t* __restrict__ slc, uint64_t cap_slc,
-                                                             Meta* __restrict__ meta) {
+                                                             uint32_t* __restrict__ slc, Meta* __restrict__ meta) {
   __shared__ CountLds L;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   // an invalid box or a bucket that did not fit: nothing to lay out
@@ -2939,71 +2940,89 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
   }
   const InstDesc I = inst[0];
   const uint32_t k2 = I.k * I.k;
-  uint32_t* const done = slc;           // per bin: slices counted (zeroed before the launch)
-  uint32_t* const rows = slc + n_bins;  // per bin: k2 fine counts (zero between launches)
+  const uint32_t n_chunks = (n_bins + kSortThreads - 1u) / kSortThreads;
+  uint32_t* const done = slc;
+  uint32_t* const tick = slc + n_bins;
+  uint32_t* const rows = slc + n_bins + n_chunks;
   for (uint32_t c = tid; c < kMaxCellsPerBin; c += kSortThreads) L.cnt[c] = 0u;
   if (tid == 0) L.bmax = 0u;
   __syncthreads();
-  // 1. slices of the large bins
-  uint32_t gbase = 0;
-  for (uint32_t base = 0; base < n_bins; base += kSortThreads) {
+  for (uint32_t chunk = 0; chunk < n_chunks; ++chunk) {
+    const uint32_t base = chunk * kSortThreads;
     const uint32_t bb = base + tid;
     const uint32_t mm = bb < n_bins ? total[bb] : 0u;
     const uint32_t ns = mm > kCountSlice ? (mm + kCountSlice - 1u) / kCountSlice : 0u;
-    const uint32_t inc = wave_prefix_sum(ns);
-    if (lane == 63u) L.ws[w] = inc;
-    __syncthreads();
-    uint32_t pre = inc, T = 0;
-    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-      if (k < w) pre += L.ws[k];
-      T += L.ws[k];
+    const uint32_t sm = mm > 0u && mm <= kCountSlice ? 1u : 0u;
+    const uint32_t inc = wave_prefix_sum(ns), incb = wave_prefix_sum(sm);
+    if (lane == 63u) {
+      L.ws[w] = inc;
+      L.wb[w] = incb;
     }
-    L.soff[tid + 1u] = pre;
-    if (tid == 0) L.soff[0] = 0u;
+    if (tid == 0) L.tick = atomicAdd(&tick[chunk], 1u);
     __syncthreads();
-    const uint32_t skew = gbase % gridDim.x;
-    for (uint32_t r = (blockIdx.x + gridDim.x - skew) % gridDim.x; r < T; r += gridDim.x) {
-      // the bin of chunk slice r: soff[x] <= r < soff[x + 1]
-      uint32_t lo = 0, hi = kSortThreads;
+    uint32_t pre = inc, preb = incb, T = 0, TB = 0;
+    for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
+      if (k < w) {
+        pre += L.ws[k];
+        preb += L.wb[k];
+      }
+      T += L.ws[k];
+      TB += L.wb[k];
+    }
+    L.soff[tid + 1u] = pre;  // inclusive scans: slices, other bins
+    L.boff[tid + 1u] = preb;
+    if (tid == 0) {
+      L.soff[0] = 0u;
+      L.boff[0] = 0u;
+    }
+    __syncthreads();
+    for (;;) {
+      const uint32_t u = L.tick;
+      if (u >= T + TB) break;  // (uniform)
+      __syncthreads();
+      if (tid == 0) L.tick = atomicAdd(&tick[chunk], 1u);  // the next ticket, in flight meanwhile
+      const bool slice = u < T;
+      const uint32_t r = slice ? u : u - T;
+      const uint32_t* const off = slice ? L.soff : L.boff;
+      uint32_t lo = 0, hi = kSortThreads;  // off[lo] <= r < off[lo + 1]
       while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (L.soff[mid] <= r) lo = mid; else hi = mid;
+        if (off[mid] <= r) lo = mid; else hi = mid;
       }
-      const uint32_t b = base + lo, j = r - L.soff[lo];
-      const uint32_t m = total[b], nsb = L.soff[lo + 1u] - L.soff[lo];
+      const uint32_t b = base + lo, m = total[b];
       const SortBin B(I, b);
-      const uint32_t kk = B.kx * B.ky;
-      const uint32_t j0 = j * kCountSlice, j1 = j0 + kCountSlice < m ? j0 + kCountSlice : m;
-      count_fine(L, I, B, bbox, bin_start[b], j0, j1);
-      __syncthreads();
-      uint32_t seen = 0;
-      for (uint32_t c = tid; c < kk; c += kSortThreads) {
-        if (L.cnt[c]) seen |= atomicAdd(&rows[(uint64_t)b * k2 + c], L.cnt[c]);
-        L.cnt[c] = 0u;
-      }
-      // every add has returned, so it is performed at device scope, before
-      // this workgroup counts the slice as done
-      asm volatile("" ::"v"(seen));
-      __syncthreads();
-      if (tid == 0) L.last = atomicAdd(&done[b], 1u) == nsb - 1u;
-      __syncthreads();
-      if (L.last) {  // every slice of bin b is counted: take the sums
-        for (uint32_t c = tid; c < kk; c += kSortThreads) L.cnt[c] = atomicExch(&rows[(uint64_t)b * k2 + c], 0u);
+      if (slice) {
+        const uint32_t kk = B.kx * B.ky;
+        const uint32_t j = r - L.soff[lo], nsb = L.soff[lo + 1u] - L.soff[lo];
+        const uint32_t j0 = j * kCountSlice, j1 = j0 + kCountSlice < m ? j0 + kCountSlice : m;
+        count_fine(L, I, B, bbox, bin_start[b], j0, j1);
+        __syncthreads();
+        uint32_t seen = 0;
+        for (uint32_t c = tid; c < kk; c += kSortThreads) {
+          if (L.cnt[c]) seen |= atomicAdd(&rows[(uint64_t)b * k2 + c], L.cnt[c]);
+          L.cnt[c] = 0u;
+        }
+        // every add has returned, so it is performed at device scope, before
+        // this workgroup counts the slice as done
+        asm volatile("" ::"v"(seen));
+        __syncthreads();
+        if (tid == 0) L.last = atomicAdd(&done[b], 1u) == nsb - 1u;
+        __syncthreads();
+        if (L.last) {  // every slice of bin b is counted: take the sums
+          for (uint32_t c = tid; c < kk; c += kSortThreads)
+            L.cnt[c] = atomicExch(&rows[(uint64_t)b * k2 + c], 0u);
+          __syncthreads();
+          layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips,
+                     meta);
+        }
+      } else {
+        count_fine(L, I, B, bbox, bin_start[b], 0u, m);
         __syncthreads();
         layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips, meta);
       }
+      __syncthreads();  // (the next ticket is in L.tick)
     }
-    gbase += T;
-    __syncthreads();  // (soff and ws are reused by the next chunk)
-  }
-  // 2. the other bins, one workgroup each
-  for (uint32_t b = blockIdx.x; b < n_bins; b += gridDim.x) {
-    const uint32_t m = total[b];
-    if (m == 0u || m > kCountSlice) continue;  // (uniform over the block)
-    const SortBin B(I, b);
-    count_fine(L, I, B, bbox, bin_start[b], 0u, m);
-    __syncthreads();
-    layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips, meta);
+    __syncthreads();  // (soff, boff and ws are reused by the next chunk)
   }
 }
 
@@ -3012,6 +3031,7 @@ struct PlaceLds {
   uint32_t cst[kMaxCellsPerBin];  // per cell of the strip: start, from the strip's
   float4 sbox[kSortStage];
   uint32_t sid[kSortStage];
+  uint32_t tick;  // the workgroup's next strip
 };
 static_assert(sizeof(PlaceLds) <= 160u * 1024u, "one workgroup per CU");
 
@@ -3025,7 +3045,7 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __res
                                                              uint32_t* __restrict__ cell_cur,
                                                              const uint4* __restrict__ strips, uint64_t cap_strips,
                                                              float4* __restrict__ mbox, uint32_t* __restrict__ mid,
-                                                             const Meta* __restrict__ meta) {
+                                                             Meta* __restrict__ meta) {
   extern __shared__ __align__(16) unsigned char place_lds[];
   PlaceLds& L = *reinterpret_cast<PlaceLds*>(place_lds);
   const uint32_t tid = threadIdx.x;
@@ -3033,7 +3053,14 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_place(const float4* __res
   const uint32_t n_strips = (uint64_t)meta->dev_chunks < cap_strips ? meta->dev_chunks : (uint32_t)cap_strips;
   const InstDesc I = inst[0];
   const uint32_t k2 = I.k * I.k;
-  for (uint32_t t = blockIdx.x; t < n_strips; t += gridDim.x) {
+  // strips taken by ticket (meta->dev_bins, zeroed with the result words)
+  if (tid == 0) L.tick = atomicAdd(&meta->dev_bins, 1u);
+  __syncthreads();
+  for (;;) {
+    const uint32_t t = L.tick;
+    if (t >= n_strips) break;  // (uniform)
+    __syncthreads();
+    if (tid == 0) L.tick = atomicAdd(&meta->dev_bins, 1u);  // the next one, in flight meanwhile
     const uint4 st = strips[t];
     const uint32_t b = st.x, lo = st.y, hi = st.z;
     const uint32_t m = total[b];
@@ -3159,14 +3186,15 @@ hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const B
   hipError_t e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
   // (the result words, and the bins' slice counters of k_fine_count)
-  if ((e = launch_meta_clear(meta, s, gw.slc, bp.n_bins)) != hipSuccess) return e;
+  const uint32_t n_chunks = (bp.n_bins + kSortThreads - 1u) / kSortThreads;
+  if ((e = launch_meta_clear(meta, s, gw.slc, bp.n_bins + n_chunks)) != hipSuccess) return e;
   if ((e = launch_binned_front(ws, boxes, nullptr, bp, s, ev ? ev + 1 : nullptr)) != hipSuccess) return e;
   const uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
   // (cell starts in gw.cnt; bin ends and then the large bins' cell cursors in gw.start)
   uint32_t* const cell_cur = gw.start + bp.n_bins;
   k_fine_count<<<grid, kSortThreads, 0, s>>>(ws.bbox, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt,
                                              gw.start, cell_cur, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
-                                             gw.cap_strips, gw.slc, gw.cap_slc, meta);
+                                             gw.cap_strips, gw.slc, meta);
   if (ev && (e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
   k_fine_place<<<grid, kSortThreads, sizeof(PlaceLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, gw.cnt,
                                                             gw.start, cell_cur, gw.strips, gw.cap_strips, gw.mbox,

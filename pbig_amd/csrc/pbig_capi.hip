@@ -670,10 +670,11 @@ int size_sorted(pbig_hip_ctx* c, uint32_t n, hipStream_t s) {
   if ((rc = grow(c, &c->ws.strips, &c->ws.cap_strips,
                  std::max<uint64_t>(c->ws.cap_strips, 2ull * d.nb + (uint64_t)n / 1024 + 1024))))
     return rc;
-  // the bins' slice counters, then per bin a row of fine counts that the
-  // counting slices of a large bin add to (zero between launches: the
-  // kernel takes each sum with an exchange against zero)
-  const uint64_t want_slc = (uint64_t)d.nb + (uint64_t)d.nb * d.k * d.k;
+  // the bins' slice counters, a ticket counter per 1,024 bins, then per bin
+  // a row of fine counts that the counting slices of a large bin add to
+  // (zero between launches: the kernel takes each sum with an exchange
+  // against zero)
+  const uint64_t want_slc = (uint64_t)d.nb + (d.nb + 1023u) / 1024u + (uint64_t)d.nb * d.k * d.k;
   if (c->ws.cap_slc < want_slc || !c->ws.slc) {
     if ((rc = grow(c, &c->ws.slc, &c->ws.cap_slc, want_slc))) return rc;
     HIP_TRY(c, hipMemsetAsync(c->ws.slc, 0, c->ws.cap_slc * sizeof(uint32_t), s));
