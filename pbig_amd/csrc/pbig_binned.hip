@@ -2711,7 +2711,7 @@ constexpr uint32_t kSortBigBin = 8u * kSortWindow;
 // counted in LDS, every cell's range reserved with one device atomic on that
 // cell's cursor, the memberships written there (strip items carry
 // kSortSliceTag in their last word, the slice index in the second).
-constexpr uint32_t kSortSlice = 8192;
+constexpr uint32_t kSortSlice = 4096;
 constexpr uint32_t kSortSliceTag = 0xFFFFFFFFu;
 
 // Tile-pair items of a cell of n memberships (none past the tile limit: the
