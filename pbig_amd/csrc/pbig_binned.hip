@@ -2748,6 +2748,8 @@ struct SortBin {
 struct CountLds {
   uint32_t cnt[kMaxCellsPerBin];
   uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
+  uint32_t eoff[kMaxCellsPerBin + 1];  // per cell: first membership and first item in the bin
+  uint32_t ioff[kMaxCellsPerBin];
   uint32_t soff[kSortThreads + 1];  // a chunk of bins: inclusive scan of their slices
   uint32_t boff[kSortThreads + 1];  // and of their other bins
   uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
@@ -2818,6 +2820,20 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
     F += L.wm[k];
     II += L.wi[k];
   }
+  {
+    uint64_t e2 = e, ie2 = ie;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortPer; ++k) {
+      const uint32_t c = tid * kSortPer + k;
+      if (c < kk) {
+        L.eoff[c] = (uint32_t)e2;  // (a bin past 2^32 - 1 memberships or items fails below)
+        L.ioff[c] = (uint32_t)ie2;
+      }
+      e2 += v[k];
+      ie2 += tile_items(v[k]);
+    }
+    if (tid == 0) L.eoff[kk] = (uint32_t)F;
+  }
   // 2. the strips: cell 0 opens one, and so does every cell whose start
   //    lies in a later window of kSortWindow memberships than its
   //    predecessor's (a strip then spans less than two windows unless its
@@ -2872,26 +2888,37 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
   const uint32_t sb = L.sbase;
   const bool fits = F < (1ull << 32) && II < (1ull << 32) && fb + F <= cap_memb && ib + II <= cap_items;
   if (!fits && tid == 0) atomicOr((unsigned long long*)&meta->error, kErrWorkspace);
-  // 4. every cell's absolute start, the items, and the strips
+  // 4. every cell's absolute start, the items (spread over the threads:
+  //    item t of the bin is item x of the cell c whose items it falls in, the
+  //    tile pair (r, q) at x in row-major order of r <= q < nt), and the
+  //    strips
   if (fits) {
-    uint64_t e2 = e, ie2 = ie;
+    uint64_t e2 = e;
 #pragma unroll
     for (uint32_t k = 0; k < kSortPer; ++k) {
       const uint32_t c = tid * kSortPer + k;
       if (c < kk) {
         cell_start[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
         if (sliced) cell_cur[(uint64_t)b * k2 + c] = (uint32_t)(fb + e2);
-        const uint32_t n = v[k];
-        const uint32_t nt = (n + kTile - 1u) / kTile;
-        if (n >= 2u && nt <= kMaxTilesPerCell) {
-          uint64_t at = ib + ie2;
-          const uint32_t st = (uint32_t)(fb + e2);
-          for (uint32_t r = 0; r < nt; ++r)
-            for (uint32_t q = r; q < nt; ++q) items[at++] = make_uint4(st, n, r, q);
-        }
       }
       e2 += v[k];
-      ie2 += tile_items(v[k]);
+    }
+    for (uint32_t t = tid; t < (uint32_t)II; t += kSortThreads) {
+      uint32_t lo = 0, hi = kk;  // the last cell with ioff <= t
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.ioff[mid] <= t) lo = mid; else hi = mid;
+      }
+      const uint32_t n = L.eoff[lo + 1u] - L.eoff[lo];
+      const uint64_t nt = (n + kTile - 1u) / kTile;
+      const uint64_t x = t - L.ioff[lo];
+      // rows before row r hold r nt - r (r - 1) / 2 items
+      const double d2 = 2.0 * (double)nt + 1.0;
+      uint64_t r = (uint64_t)((d2 - sqrt(d2 * d2 - 8.0 * (double)x)) * 0.5);
+      while (r > 0u && r * nt - r * (r - 1u) / 2u > x) --r;
+      while ((r + 1u) * nt - (r + 1u) * r / 2u <= x) ++r;
+      const uint64_t q = r + (x - (r * nt - r * (r - 1u) / 2u));
+      items[ib + t] = make_uint4((uint32_t)(fb + L.eoff[lo]), n, (uint32_t)r, (uint32_t)q);
     }
     for (uint32_t t = tid; t < nst; t += kSortThreads) {
       const uint4 item = sliced ? make_uint4(b, t, 0u, kSortSliceTag)
