@@ -2812,13 +2812,13 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
   if (vmax) atomicMax(&L.bmax, vmax);
   __syncthreads();  // (the counts are in registers from here: L.cnt is free until it is cleared)
   uint64_t e = inc - acc, ie = iinc - iacc, F = 0, II = 0;
-  for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-    if (k < w) {
-      e += L.wm[k];
-      ie += L.wi[k];
-    }
-    F += L.wm[k];
-    II += L.wi[k];
+#pragma unroll
+  for (uint32_t k = 0; k < kSortThreads / 64; ++k) {  // (all loads issued before the sums wait)
+    const uint64_t a = L.wm[k], c = L.wi[k];
+    e += k < w ? a : 0ull;
+    ie += k < w ? c : 0ull;
+    F += a;
+    II += c;
   }
   {
     uint64_t e2 = e, ie2 = ie;
@@ -2863,9 +2863,11 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
   if (lane == 63u) L.ws[w] = sinc;
   __syncthreads();
   uint32_t sidx = sinc - nopen, nst = 0;
+#pragma unroll
   for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-    if (k < w) sidx += L.ws[k];
-    nst += L.ws[k];
+    const uint32_t a = L.ws[k];
+    sidx += k < w ? a : 0u;
+    nst += a;
   }
 #pragma unroll
   for (uint32_t k = 0; k < kSortPer; ++k)
@@ -2988,13 +2990,13 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
     if (tid == 0) L.tick = atomicAdd(&tick[chunk], 1u);
     __syncthreads();
     uint32_t pre = inc, preb = incb, T = 0, TB = 0;
+#pragma unroll
     for (uint32_t k = 0; k < kSortThreads / 64; ++k) {
-      if (k < w) {
-        pre += L.ws[k];
-        preb += L.wb[k];
-      }
-      T += L.ws[k];
-      TB += L.wb[k];
+      const uint32_t a = L.ws[k], c = L.wb[k];
+      pre += k < w ? a : 0u;
+      preb += k < w ? c : 0u;
+      T += a;
+      TB += c;
     }
     L.soff[tid + 1u] = pre;  // inclusive scans: slices, other bins
     L.boff[tid + 1u] = preb;
