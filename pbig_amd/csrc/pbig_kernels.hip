@@ -267,6 +267,13 @@ __global__ __launch_bounds__(256) void k_scatter(const float4* __restrict__ boxe
 // compacted with mbcnt into a per-wave LDS stage of kStage pairs; a full
 // stage is flushed with one 64-bit atomic reservation on the pair counter.
 
+// A staged pair as it goes out: both ids without their flag bits, the
+// smaller first (the test loops stage the raw words).
+__device__ __forceinline__ uint2 pair_out(uint2 v) {
+  const uint32_t a = v.x & kIdMask, b = v.y & kIdMask;
+  return make_uint2(a < b ? a : b, a < b ? b : a);
+}
+
 __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t& stage_n, uint2* __restrict__ out,
                                             uint64_t cap, Meta* __restrict__ meta) {
   const int lane = threadIdx.x & 63;
@@ -275,7 +282,7 @@ __device__ __forceinline__ void flush_stage(uint2* stage, uint32_t& stage_n, uin
   g = __shfl(g, 0, 64);
   for (uint32_t k = lane; k < stage_n; k += 64) {
     uint64_t dst = g + k;
-    if (dst < cap) out[dst] = stage[k];
+    if (dst < cap) out[dst] = pair_out(stage[k]);
   }
   stage_n = 0;
 }
@@ -288,7 +295,6 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
                                              const float4& cb, uint32_t cw, uint32_t ncol, uint2* stage,
                                              uint32_t& stage_n, uint2* __restrict__ out, uint64_t cap,
                                              Meta* __restrict__ meta) {
-  const int lane = threadIdx.x & 63;
   for (uint32_t jj = 0; jj < ncol; ++jj) {
     const float bx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.x), (int)jj));
     const float by = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cb.y), (int)jj));
@@ -298,11 +304,10 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
     const uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
                        __ballot((rf | (bw >> 30)) == 3u) & vm;
     if (m) {
-      if ((m >> lane) & 1ull) {
+      if (__builtin_amdgcn_inverse_ballot_w64(m)) {  // (this lane's bit of m, as the exec mask)
         const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        const uint32_t bid = bw & kIdMask;
-        stage[pos] = make_uint2(aid < bid ? aid : bid, aid < bid ? bid : aid);
+        stage[pos] = make_uint2(aid, bw);
       }
       stage_n += (uint32_t)__popcll(m);
       if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
@@ -311,33 +316,37 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
 }
 
 // A diagonal item (a tile against itself, w boxes on lanes 0 .. w-1): at step
-// k = 1 .. w/2 lane l tests the box of lane (l + k) mod w, fetched across the
-// wave, so every unordered pair is tested once (at k = w/2 of an even w only
-// by the lanes below w/2) in half the steps of a column broadcast.
+// k = 1 .. w/2 lane l tests the box of lane (l + k) mod w, so every
+// unordered pair is tested once (at k = w/2 of an even w only by the lanes
+// below w/2) in half the steps of a column broadcast.  The partner boxes
+// move one lane per step (each lane fetches lane (l + 1) mod w's partner
+// across the wave), so a step needs no index arithmetic.
 __device__ __forceinline__ void test_diagonal(const float4& a, uint32_t aw, uint64_t vm, uint32_t w,
                                               uint2* stage, uint32_t& stage_n, uint2* __restrict__ out,
                                               uint64_t cap, Meta* __restrict__ meta) {
   const int lane = threadIdx.x & 63;
-  const uint32_t aid = aw & kIdMask, rf = aw >> 30;
+  const uint32_t rf = aw >> 30;
   const uint32_t half = w >> 1;
-  uint32_t src = (uint32_t)lane;  // stays in 0 .. 63 on every lane
+  const int addr = (int)(((uint32_t)lane + 1u >= w ? (uint32_t)lane + 1u - w : (uint32_t)lane + 1u) << 2);
+  int px = __builtin_bit_cast(int, a.x), py = __builtin_bit_cast(int, a.y);
+  int pz = __builtin_bit_cast(int, a.z), pw = __builtin_bit_cast(int, a.w), pid = (int)aw;
   for (uint32_t k = 1; k <= half; ++k) {
-    src = src + 1u >= w ? src + 1u - w : src + 1u;
-    const int addr = (int)(src << 2);
-    const float bx = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.x)));
-    const float by = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.y)));
-    const float bz = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.z)));
-    const float bwv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, a.w)));
-    const uint32_t bw = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)aw);
+    px = __builtin_amdgcn_ds_bpermute(addr, px);
+    py = __builtin_amdgcn_ds_bpermute(addr, py);
+    pz = __builtin_amdgcn_ds_bpermute(addr, pz);
+    pw = __builtin_amdgcn_ds_bpermute(addr, pw);
+    pid = __builtin_amdgcn_ds_bpermute(addr, pid);
+    const float bx = __builtin_bit_cast(float, px), by = __builtin_bit_cast(float, py);
+    const float bz = __builtin_bit_cast(float, pz), bwv = __builtin_bit_cast(float, pw);
+    const uint32_t bw = (uint32_t)pid;
     uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
                  __ballot((rf | (bw >> 30)) == 3u) & vm;
     if (2u * k == w) m &= (1ull << half) - 1ull;
     if (m) {
-      if ((m >> lane) & 1ull) {
+      if (__builtin_amdgcn_inverse_ballot_w64(m)) {  // (this lane's bit of m, as the exec mask)
         const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        const uint32_t bid = bw & kIdMask;
-        stage[pos] = make_uint2(aid < bid ? aid : bid, aid < bid ? bid : aid);
+        stage[pos] = make_uint2(aw, bw);
       }
       stage_n += (uint32_t)__popcll(m);
       if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
@@ -397,7 +406,7 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
       cw = mid[base + j0 + lane];
     }
     const uint32_t rf = aid >> 30;  // (start column, start row) flags of the row box
-    test_columns(a, aid & kIdMask, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
+    test_columns(a, aid, rf, vm, cb, cw, ncol, stage, stage_n, out, cap, meta);
   }
   // what is left in the four stages goes out under one reservation
   if (lane == 0) tail_n[wib] = stage_n;
@@ -411,7 +420,7 @@ __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
   for (int k = 0; k < wib; ++k) g += tail_n[k];
   for (uint32_t k = lane; k < stage_n; k += 64) {
     const uint64_t dst = g + k;
-    if (dst < cap) out[dst] = stage[k];
+    if (dst < cap) out[dst] = pair_out(stage[k]);
   }
 }
 
