@@ -2712,6 +2712,9 @@ constexpr uint32_t kSortBigBin = 8u * kSortWindow;
 // cell's cursor, the memberships written there (strip items carry
 // kSortSliceTag in their last word, the slice index in the second).
 constexpr uint32_t kSortSlice = 4096;
+// A bin of at most kCountStage fine memberships is placed by k_fine_count
+// itself, right after its layout (no strips, no second read by k_fine_place).
+constexpr uint32_t kCountStage = 4096;
 constexpr uint32_t kSortSliceTag = 0xFFFFFFFFu;
 
 // Tile-pair items of a cell of n memberships (none past the tile limit: the
@@ -2750,6 +2753,8 @@ struct CountLds {
   uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
   uint32_t eoff[kMaxCellsPerBin + 1];  // per cell: first membership and first item in the bin
   uint32_t ioff[kMaxCellsPerBin];
+  float4 sbox[kCountStage];  // a bin of at most kCountStage memberships, laid out in place
+  uint32_t sid[kCountStage];
   uint32_t soff[kSortThreads + 1];  // a chunk of bins: inclusive scan of their slices
   uint32_t boff[kSortThreads + 1];  // and of their other bins
   uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
@@ -2757,6 +2762,7 @@ struct CountLds {
   unsigned long long fbase, ibase;
   uint32_t bmax, sbase, last, tick;
 };
+static_assert(sizeof(CountLds) <= 160u * 1024u, "one workgroup per CU");
 
 // A bin's fine counts of memberships j0 .. j1 - 1 of the bucket into L.cnt.
 __device__ __forceinline__ void count_fine(CountLds& L, const InstDesc& I, const SortBin& B,
@@ -2782,11 +2788,15 @@ __device__ __forceinline__ void count_fine(CountLds& L, const InstDesc& I, const
 }
 
 // Bin b's layout from its fine counts in L.cnt (cleared again on return).
-__device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint32_t b, uint32_t m,
-                                        uint32_t* __restrict__ cell_start, uint32_t* __restrict__ bin_end,
-                                        uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
-                                        uint4* __restrict__ items, uint64_t cap_items,
-                                        uint4* __restrict__ strips, uint64_t cap_strips, Meta* __restrict__ meta) {
+// A bin of at most kCountStage memberships (and not counted in slices) is
+// also placed here, through an LDS stage, instead of by k_fine_place.
+__device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint32_t b, uint32_t m, bool whole,
+                                        const float4* __restrict__ bbox, const uint32_t* __restrict__ bid,
+                                        uint64_t s0, uint32_t* __restrict__ cell_start,
+                                        uint32_t* __restrict__ bin_end, uint32_t* __restrict__ cell_cur,
+                                        uint64_t cap_memb, uint4* __restrict__ items, uint64_t cap_items,
+                                        uint4* __restrict__ strips, uint64_t cap_strips, float4* __restrict__ mbox,
+                                        uint32_t* __restrict__ mid, Meta* __restrict__ meta) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t k2 = I.k * I.k;
   const SortBin B(I, b);
@@ -2874,6 +2884,8 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
     if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
   const bool sliced = F > kSortBigBin;
   if (sliced) nst = (m + kSortSlice - 1u) / kSortSlice;
+  const bool here = whole && F <= kCountStage;  // placed by this workgroup
+  if (here) nst = 0u;
   // 3. the bin's shares of the membership, item and strip arrays: three
   //    device atomics from three waves, in flight together (a share that
   //    does not fit fails the call; the kernels after this one then stop
@@ -2937,6 +2949,43 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
   for (uint32_t c = tid + kSortThreads; c < kk; c += kSortThreads) L.cnt[c] = 0u;
   if (tid == 0) L.bmax = 0u;
   __syncthreads();
+  if (here && fits) {
+    // 5. the memberships ranked per cell into the stage (L.cnt as the cells'
+    //    cursors), then written out in one run
+    for (uint32_t j0 = tid; j0 < m; j0 += kSortThreads * kSortLoads) {
+      float4 qs[kSortLoads];
+      uint32_t ids[kSortLoads];
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        const uint32_t j = j0 + u * kSortThreads;
+        const uint64_t at = s0 + (j < m ? j : m - 1u);
+        qs[u] = bbox[at];
+        ids[u] = bid[at];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kSortLoads; ++u) {
+        if (j0 + u * kSortThreads >= m) break;
+        uint32_t x0, x1, y0, y1, fx, fy;
+        B.cells(I, qs[u], x0, x1, y0, y1, fx, fy);
+        for (uint32_t cy = y0; cy <= y1; ++cy) {
+          const uint32_t flag_y = cy + B.gy == fy ? kFlagY : 0u;
+          for (uint32_t cx = x0; cx <= x1; ++cx) {
+            const uint32_t c = cy * B.kx + cx;
+            const uint32_t p = L.eoff[c] + atomicAdd(&L.cnt[c], 1u);
+            L.sbox[p] = qs[u];
+            L.sid[p] = ids[u] | flag_y | (cx + B.gx == fx ? kFlagX : 0u);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < (uint32_t)F; i += kSortThreads) {
+      mbox[fb + i] = L.sbox[i];
+      mid[fb + i] = L.sid[i];
+    }
+    for (uint32_t c = tid; c < kk; c += kSortThreads) L.cnt[c] = 0u;
+    __syncthreads();
+  }
 }
 
 // Bins of more than kCountSlice memberships are counted in slices of
@@ -2949,6 +2998,7 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
 // a ticket counter (both zeroed before the launch), per bin a row of k^2
 // counts.
 __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __restrict__ bbox,
+                                                             const uint32_t* __restrict__ bid,
                                                              const uint32_t* __restrict__ bin_start,
                                                              const uint32_t* __restrict__ total,
                                                              const InstDesc* __restrict__ inst, uint32_t n_bins,
@@ -2958,8 +3008,10 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
                                                              uint32_t* __restrict__ cell_cur, uint64_t cap_memb,
                                                              uint4* __restrict__ items, uint64_t cap_items,
                                                              uint4* __restrict__ strips, uint64_t cap_strips,
-                                                             uint32_t* __restrict__ slc, Meta* __restrict__ meta) {
-  __shared__ CountLds L;
+                                                             uint32_t* __restrict__ slc, float4* __restrict__ mbox,
+                                                             uint32_t* __restrict__ mid, Meta* __restrict__ meta) {
+  extern __shared__ __align__(16) unsigned char count_lds[];
+  CountLds& L = *reinterpret_cast<CountLds*>(count_lds);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   // an invalid box or a bucket that did not fit: nothing to lay out
   const unsigned long long berr = bmeta->error & (kErrInvalidBox | kErrWorkspace);
@@ -3041,13 +3093,14 @@ __global__ __launch_bounds__(kSortThreads) void k_fine_count(const float4* __res
           for (uint32_t c = tid; c < kk; c += kSortThreads)
             L.cnt[c] = atomicExch(&rows[(uint64_t)b * k2 + c], 0u);
           __syncthreads();
-          layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips,
-                     meta);
+          layout_bin(L, I, b, m, false, bbox, bid, bin_start[b], cell_start, bin_end, cell_cur, cap_memb, items,
+                     cap_items, strips, cap_strips, mbox, mid, meta);
         }
       } else {
         count_fine(L, I, B, bbox, bin_start[b], 0u, m);
         __syncthreads();
-        layout_bin(L, I, b, m, cell_start, bin_end, cell_cur, cap_memb, items, cap_items, strips, cap_strips, meta);
+        layout_bin(L, I, b, m, true, bbox, bid, bin_start[b], cell_start, bin_end, cell_cur, cap_memb, items,
+                   cap_items, strips, cap_strips, mbox, mid, meta);
       }
       __syncthreads();  // (the next ticket is in L.tick)
     }
@@ -3221,9 +3274,9 @@ hipError_t launch_sorted(const BinnedWorkspace& ws, const float4* boxes, const B
   const uint32_t grid = bp.n_cu ? bp.n_cu : 1u;
   // (cell starts in gw.cnt; bin ends and then the large bins' cell cursors in gw.start)
   uint32_t* const cell_cur = gw.start + bp.n_bins;
-  k_fine_count<<<grid, kSortThreads, 0, s>>>(ws.bbox, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt,
-                                             gw.start, cell_cur, gw.cap_memb, gw.items, gw.cap_items, gw.strips,
-                                             gw.cap_strips, gw.slc, meta);
+  k_fine_count<<<grid, kSortThreads, sizeof(CountLds), s>>>(
+      ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, bp.n_bins, ws.meta, gw.cnt, gw.start, cell_cur, gw.cap_memb,
+      gw.items, gw.cap_items, gw.strips, gw.cap_strips, gw.slc, gw.mbox, gw.mid, meta);
   if (ev && (e = hipEventRecord(ev[6], s)) != hipSuccess) return e;
   k_fine_place<<<grid, kSortThreads, sizeof(PlaceLds), s>>>(ws.bbox, ws.bid, ws.bin_start, ws.total, ws.inst, gw.cnt,
                                                             gw.start, cell_cur, gw.strips, gw.cap_strips, gw.mbox,
@@ -3251,6 +3304,9 @@ hipError_t binned_init() {
     if (e != hipSuccess) return e;
   }
   e = hipFuncSetAttribute((const void*)k_deep_plan, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDeepPlanLds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_fine_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)sizeof(CountLds));
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_fine_place, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)sizeof(PlaceLds));
