@@ -660,12 +660,12 @@ def test_auto_spot_in_uniform_instance(torch_cuda, orc, levels_min):
 
 def test_general_cells_from_bins_dense_and_large(torch_cuda, orc):
     """The general path's cells laid out from the bins (cells per bin counted
-    and placed in LDS): a cell of 3,000 memberships (more than the placement
-    stage's window, written in place), a bin of more than eight windows
-    (placed as one strip), boxes spanning hundreds of cells (the membership
-    array grows and the call reruns), asynchronous calls after the
-    synchronous one, all against sweep-and-prune; then a cell past the
-    262,144-box tile limit, which the call refuses."""
+    and placed in LDS): a bin holding one cell of 3,000 memberships (placed
+    by the count kernel's own stage), the cluster's bins of more than eight
+    placement windows (counted and placed in slices), boxes spanning hundreds
+    of cells (the membership array grows and the call reruns), asynchronous
+    calls after the synchronous one, all against sweep-and-prune; then a
+    cell past the 262,144-box tile limit, which the call refuses."""
     import pbig_amd
     from pbig_amd import workloads
 
@@ -700,3 +700,37 @@ def test_general_cells_from_bins_dense_and_large(torch_cuda, orc):
         with pytest.raises(pbig_amd.PbigError) as err:
             e.check_device(over, 64.0, 16.0, out)
         assert err.value.status == pbig_amd.PBIG_E_TOO_DENSE
+
+
+@pytest.mark.parametrize("cell", [45.254833, 22.627417])
+def test_general_clustered_sliced_layout(torch_cuda, orc, manifest, cell):
+    """The clustered 2M instance on the general path at bin target 4,096: its
+    cluster cores' bins (up to ~58,000 coarse memberships) are counted in
+    slices by many workgroups and placed in slices; at cell 22.6 the grid has
+    2,116 bins, so the count kernel's tickets run over three chunks of bins.
+    The sorted set's hash matches the manifest, synchronously and in two
+    asynchronous calls after it."""
+    import pbig_amd
+    from pbig_amd import workloads
+
+    boxes, L = workloads.c3_clustered()
+    m = manifest["c3_clustered_2m"]
+    d = torch_cuda.from_numpy(boxes).to("cuda:0")
+    out = torch_cuda.empty((m["pairs"] + 1024, 2), dtype=torch_cuda.int32, device="cuda:0")
+    res = torch_cuda.zeros(2, dtype=torch_cuda.int64, device="cuda:0")
+    with pbig_amd.Engine(0) as e:
+        e.configure(PATHS["general"], 4096)
+        n = e.check_device(d, L, cell, out)
+        st = e.stats()
+        assert st["path"] == PATHS["general"] and st["bins_per_row"] > 0
+        if cell < 30.0:
+            assert st["bins_per_row"] ** 2 > 2048  # more than two chunks of 1,024 bins
+        assert n == m["pairs"]
+        assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"]
+        for rep in range(2):
+            res.zero_()
+            out.fill_(-1)
+            e.check_device_async(d, L, cell, out, res)
+            torch_cuda.cuda.synchronize()
+            assert res.cpu().tolist() == [m["pairs"], 0], rep
+            assert orc.pair_hash(orc.sort_pairs(out[:n].cpu().numpy().view(np.uint32))) == m["sha256"], rep
