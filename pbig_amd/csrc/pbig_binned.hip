@@ -2684,19 +2684,21 @@ hipError_t launch_binned(const BinnedWorkspace& ws, const float4* boxes, const u
 //   k_fine_count  persistent workgroups taking the bins by ticket: fine
 //                 counts in LDS (a bin of more than kCountSlice memberships
 //                 counted in slices spread over the chip, then summed by the
-//                 workgroup that counts its last slice), their scan, the bin's shares
-//                 of the membership, item and strip arrays (three device
-//                 atomics per bin, not one per membership), the items, every
-//                 cell's absolute start, and the bin's strips (row-major runs
-//                 of cells starting within one window of kSortStage / 2
-//                 memberships) on a work list;
-//   k_fine_place  persistent workgroups taking the strips of all bins by ticket: the
-//                 bin's memberships stream from the bucket, the strip's ones
-//                 go to an LDS stage in cell order and out in one run (a
-//                 strip past the stage, i.e. a cell of more than kSortStage
-//                 / 2 memberships at its end, is written in place); a bin of
-//                 more than kSortBigBin memberships is placed in slices
-//                 instead, spread over the chip like the strips.
+//                 workgroup that counts its last slice), their scan, the
+//                 bin's shares of the membership, item and strip arrays
+//                 (three device atomics per bin, not one per membership),
+//                 the items, every cell's absolute start, and the bin's
+//                 strips (row-major runs of cells starting within one window
+//                 of kSortStage / 2 memberships) on a work list; a bin of at
+//                 most kCountStage memberships is placed right there instead;
+//   k_fine_place  persistent workgroups taking the strips of all bins by
+//                 ticket: the bin's memberships stream from the bucket, the
+//                 strip's ones go to an LDS stage in cell order and out in
+//                 one run (a strip past the stage, i.e. a cell of more than
+//                 kSortStage / 2 memberships at its end, is written in
+//                 place); a bin of more than kSortBigBin memberships is
+//                 placed in slices instead, spread over the chip like the
+//                 strips.
 constexpr uint32_t kSortThreads = 1024;
 constexpr uint32_t kSortPer = kMaxCellsPerBin / kSortThreads;  // cells per thread in the scans
 static_assert(kSortPer * kSortThreads == kMaxCellsPerBin, "four cells per thread");
