@@ -315,6 +315,31 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
   }
 }
 
+// One step of a diagonal item: the partner boxes moved one lane on.
+__device__ __forceinline__ void diag_step(const float4& a, uint32_t aw, uint32_t rf, uint64_t vm, int addr, int& px,
+                                          int& py, int& pz, int& pw, int& pid, uint2* stage, uint32_t& stage_n,
+                                          uint2* __restrict__ out, uint64_t cap, Meta* __restrict__ meta) {
+  px = __builtin_amdgcn_ds_bpermute(addr, px);
+  py = __builtin_amdgcn_ds_bpermute(addr, py);
+  pz = __builtin_amdgcn_ds_bpermute(addr, pz);
+  pw = __builtin_amdgcn_ds_bpermute(addr, pw);
+  pid = __builtin_amdgcn_ds_bpermute(addr, pid);
+  const float bx = __builtin_bit_cast(float, px), by = __builtin_bit_cast(float, py);
+  const float bz = __builtin_bit_cast(float, pz), bwv = __builtin_bit_cast(float, pw);
+  const uint32_t bw = (uint32_t)pid;
+  const uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
+                     __ballot((rf | (bw >> 30)) == 3u) & vm;
+  if (m) {
+    if (__builtin_amdgcn_inverse_ballot_w64(m)) {  // (this lane's bit of m, as the exec mask)
+      const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      stage[pos] = make_uint2(aw, bw);
+    }
+    stage_n += (uint32_t)__popcll(m);
+    if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
+  }
+}
+
 // A diagonal item (a tile against itself, w boxes on lanes 0 .. w-1): at step
 // k = 1 .. w/2 lane l tests the box of lane (l + k) mod w, so every
 // unordered pair is tested once (at k = w/2 of an even w only by the lanes
@@ -324,34 +349,16 @@ __device__ __forceinline__ void test_columns(const float4& a, uint32_t aid, uint
 __device__ __forceinline__ void test_diagonal(const float4& a, uint32_t aw, uint64_t vm, uint32_t w,
                                               uint2* stage, uint32_t& stage_n, uint2* __restrict__ out,
                                               uint64_t cap, Meta* __restrict__ meta) {
-  const int lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t rf = aw >> 30;
-  const uint32_t half = w >> 1;
-  const int addr = (int)(((uint32_t)lane + 1u >= w ? (uint32_t)lane + 1u - w : (uint32_t)lane + 1u) << 2);
+  const int addr = (int)((lane + 1u >= w ? lane + 1u - w : lane + 1u) << 2);
   int px = __builtin_bit_cast(int, a.x), py = __builtin_bit_cast(int, a.y);
   int pz = __builtin_bit_cast(int, a.z), pw = __builtin_bit_cast(int, a.w), pid = (int)aw;
-  for (uint32_t k = 1; k <= half; ++k) {
-    px = __builtin_amdgcn_ds_bpermute(addr, px);
-    py = __builtin_amdgcn_ds_bpermute(addr, py);
-    pz = __builtin_amdgcn_ds_bpermute(addr, pz);
-    pw = __builtin_amdgcn_ds_bpermute(addr, pw);
-    pid = __builtin_amdgcn_ds_bpermute(addr, pid);
-    const float bx = __builtin_bit_cast(float, px), by = __builtin_bit_cast(float, py);
-    const float bz = __builtin_bit_cast(float, pz), bwv = __builtin_bit_cast(float, pw);
-    const uint32_t bw = (uint32_t)pid;
-    uint64_t m = __ballot(a.x <= by) & __ballot(bx <= a.y) & __ballot(a.z <= bwv) & __ballot(bz <= a.w) &
-                 __ballot((rf | (bw >> 30)) == 3u) & vm;
-    if (2u * k == w) m &= (1ull << half) - 1ull;
-    if (m) {
-      if (__builtin_amdgcn_inverse_ballot_w64(m)) {  // (this lane's bit of m, as the exec mask)
-        const uint32_t pos = stage_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        stage[pos] = make_uint2(aw, bw);
-      }
-      stage_n += (uint32_t)__popcll(m);
-      if (stage_n > kStage - 64) flush_stage(stage, stage_n, out, cap, meta);
-    }
-  }
+  for (uint32_t k = 1; 2u * k < w; ++k)
+    diag_step(a, aw, rf, vm, addr, px, py, pz, pw, pid, stage, stage_n, out, cap, meta);
+  if (w >= 2u && (w & 1u) == 0u)  // k = w/2: the lanes below w/2 only
+    diag_step(a, aw, rf, vm & ((1ull << (w >> 1)) - 1ull), addr, px, py, pz, pw, pid, stage, stage_n, out, cap,
+              meta);
 }
 
 __global__ __launch_bounds__(256) void k_test(const uint4* __restrict__ items,
