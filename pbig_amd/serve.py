@@ -170,6 +170,7 @@ class _Slot:
     total: int = 0
     src: object = None               # a plain launch's boxes (the request's own device array, or its pool run)
     general: bool = False            # a plain launch relaunched on the general path
+    bin_target: int = 0              # the slot's configured bin target (0: the table's)
 
 
 class Server:
@@ -179,11 +180,15 @@ class Server:
     (pool_frac of the budget) plus the slots' workspaces and pair buffers
     (each slot's batches are capped at slot budget / bytes_per_box boxes; a
     single larger request still runs, alone).
+    unbounded_bin_target: the bin target of batches of requests whose box
+    sides are unbounded (Request.narrow False; their dense bins are re-binned
+    by levels, which do better on smaller bins); 0 = the table's.
     """
 
     def __init__(self, device: int = 0, hbm_budget: int = 8 << 30, slots: int = 2, pool_frac: float = 0.5,
                  table: dict | None = None, bytes_per_box: int = 96, results: str = "host", consumer=None,
-                 max_batch_boxes: int | None = None, host_levels: bool = False, group_by_schedule: bool = True):
+                 max_batch_boxes: int | None = None, host_levels: bool = False, group_by_schedule: bool = True,
+                 unbounded_bin_target: int = 3000):
         import torch
 
         if results not in ("host", "device"):
@@ -194,6 +199,7 @@ class Server:
         self.table = table or autotune.load_table()
         self.host_levels = host_levels
         self.group_by_schedule = group_by_schedule
+        self.unbounded_bin_target = int(unbounded_bin_target)
         self.results = results
         self.consumer = consumer or keep_copy
         self.eng = pbig_amd.Engine(device)
@@ -217,8 +223,8 @@ class Server:
         self.batch_sizes: list[int] = []
         self.n_requeued = 0                  # requests a batch left incomplete, rerun alone
 
-    def _configure(self, eng, path: int):
-        eng.configure(path, int(self.table["bin_target"]), host_levels=self.host_levels)
+    def _configure(self, eng, path: int, bin_target: int = 0):
+        eng.configure(path, int(bin_target or self.table["bin_target"]), host_levels=self.host_levels)
 
     # -- client side ---------------------------------------------------------
     def submit(self, req: Request):
@@ -329,6 +335,7 @@ class Server:
         s.mode = "plain" if len(reqs) == 1 and reqs[0].n > 0 else "batch"
         if s.mode == "plain":
             self._configure(s.eng, pbig_amd.PATH_AUTO)  # no memory of earlier requests
+            s.bin_target = 0
             r = reqs[0]
             s.general = False
             if r.device is not None:  # the request's own device array: no wait for its pool copy
@@ -339,9 +346,13 @@ class Server:
                 s.src = self.d_pool[r._off:r._off + r.n]
             s.eng.check_device_async(s.src, r.L, r.cell_size, s.pairs, s.res, stream=s.stream)
         else:
-            if s.general:  # the slot's last request ended on the general path: batches choose again
-                self._configure(s.eng, pbig_amd.PATH_AUTO)
+            # batches choose their paths automatically (also after a request
+            # that ended on the general path), at the bin target of their kind
+            bt = self.unbounded_bin_target if self.group_by_schedule and kind is False else 0
+            if s.general or bt != s.bin_target:
+                self._configure(s.eng, pbig_amd.PATH_AUTO, bt)
                 s.general = False
+                s.bin_target = bt
             if s.batch is None:
                 s.batch = pbig_amd.Batch.from_instances(s.eng, starts, counts, Ls, cells)
             else:
@@ -403,6 +414,7 @@ class Server:
             finally:
                 # the slot's later batches choose their paths automatically again
                 self._configure(s.eng, pbig_amd.PATH_AUTO)
+                s.bin_target = 0
             s.h_io[0, 0] = 0
             s.h_io[0, 1] = total
             return total
@@ -418,6 +430,7 @@ class Server:
         r = s.reqs[0]
         s.general = True
         self._configure(s.eng, pbig_amd.PATH_GENERAL)
+        s.bin_target = 0
         s.eng.check_device_async(s.src, r.L, r.cell_size, s.pairs, s.res, stream=s.stream)
         with torch.cuda.stream(s.stream):
             s.h_res.copy_(s.res, non_blocking=True)
