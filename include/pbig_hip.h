@@ -73,8 +73,8 @@ typedef struct pbig_hip_config {
    * re-binned by levels on the device; smaller ones run on the general path
    * (the levels' fixed cost, about 0.3 ms, exceeds the general path's whole
    * cost below about a million boxes).  Instances of mostly dense clusters
-   * take the levels from four times this size (the general path's per-bin
-   * cell layout is faster on them below that).  0 = 2^20. */
+   * take the levels only from 64 times this size (the general path's per-bin
+   * cell layout is faster on them at every size measured).  0 = 2^20. */
   uint32_t levels_min_boxes;
   /* Nonzero: coarse bins beyond LDS are always re-binned by the host
    * between levels (synchronous calls; asynchronous launches report them,

@@ -333,7 +333,7 @@ class Engine:
                   host_levels: bool = False):
         """Path choice and tuning (pbig_hip_configure): bin_target 0 = 2000;
         levels_min_boxes 0 = 2^20 (the automatic choice's smallest instance
-        whose bins beyond LDS are re-binned on the device; four times that for
+        whose bins beyond LDS are re-binned on the device; 64 times that for
         an instance of mostly dense clusters); host_levels: bins beyond LDS
         always re-binned by the host, never by device levels."""
         cfg = Config(int(path), int(bin_target), int(levels_min_boxes), int(bool(host_levels)))

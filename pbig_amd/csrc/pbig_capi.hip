@@ -151,9 +151,9 @@ constexpr int kPathOnePass = 3;
 // from the bins (launch_sorted).
 constexpr int kPathSorted = 5;
 // Instances of mostly dense clusters take the device levels from this many
-// times levels_min_boxes (other instances with bins beyond LDS from
-// levels_min_boxes).
-constexpr uint64_t kDenseLevelsFactor = 4;
+// times levels_min_boxes (2^26 boxes by default, past every size measured;
+// other instances with bins beyond LDS from levels_min_boxes).
+constexpr uint64_t kDenseLevelsFactor = 64;
 
 // Device memory of pbig_hip_check_device_noalloc: the whole workspace of a
 // single-instance binned launch, carved per call from this array.  Each
@@ -1327,9 +1327,9 @@ int run_binned_sync(pbig_hip_ctx* c, const float4* d_boxes, uint32_t n, uint2* d
   // An instance of mostly dense clusters goes to the device levels when it
   // has kDenseLevelsFactor x levels_min_boxes or more (level 0 then runs
   // without the dense exit), else to the general path, whose per-bin cell
-  // layout is the faster of the two below that size (clustered instances of
-  // 2^20 and 2^21 boxes: 0.26 vs 0.46 ms and 0.40 vs 0.53 ms; of 2^23: 1.33
-  // vs 1.19 ms).
+  // layout is the faster of the two on every size measured (clustered
+  // instances of 2^20, 2^21 and 2^23 boxes: 0.20 vs 0.46 ms, 0.26 vs 0.53 ms
+  // and 0.96 vs 1.23 ms).
   const bool dense_to_levels =
       auto_choice && (uint64_t)n >= kDenseLevelsFactor * c->cfg.levels_min_boxes && !c->cfg.host_levels;
   Meta m;

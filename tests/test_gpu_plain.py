@@ -426,7 +426,7 @@ def _dense_instance_settles(torch_cuda, orc, manifest, path, first_errors, level
     return settled
 
 
-@pytest.mark.parametrize("levels_min", [0, 1 << 19])
+@pytest.mark.parametrize("levels_min", [0, 1 << 15])
 def test_auto_dense_instance_settles(torch_cuda, orc, manifest, levels_min):
     """The clustered instance under the automatic choice: the first
     asynchronous call tries the single-pass launch, whose dense bins outgrow
@@ -435,9 +435,9 @@ def test_auto_dense_instance_settles(torch_cuda, orc, manifest, levels_min):
     a quarter of the memberships in bins beyond LDS (mostly dense clusters).
     The synchronous call then sends the instance (2^21 boxes) to the general
     path at the default levels_min_boxes (2^20: mostly dense instances take
-    the device levels from four times that), or to the device levels when
-    four times levels_min_boxes is within its size, and the next
-    asynchronous calls follow it, with the exact set."""
+    the device levels from 64 times that), or to the device levels when 64
+    times levels_min_boxes is within its size, and the next asynchronous
+    calls follow it, with the exact set."""
     settled = _dense_instance_settles(torch_cuda, orc, manifest, "auto", (16, 32, 48), levels_min)
     assert settled == (PATHS["general"] if levels_min == 0 else PATHS["binned"])
 
