@@ -2716,7 +2716,8 @@ constexpr uint32_t kSortBigBin = 8u * kSortWindow;
 constexpr uint32_t kSortSlice = 4096;
 // A bin of at most kCountStage fine memberships is placed by k_fine_count
 // itself, right after its layout (no strips, no second read by k_fine_place).
-constexpr uint32_t kCountStage = 4096;
+constexpr uint32_t kCountStage = 5120;
+static_assert(kCountStage >= kMaxCellsPerBin, "the stage's id words hold a strip list");
 constexpr uint32_t kSortSliceTag = 0xFFFFFFFFu;
 
 // Tile-pair items of a cell of n memberships (none past the tile limit: the
@@ -2752,11 +2753,10 @@ struct SortBin {
 
 struct CountLds {
   uint32_t cnt[kMaxCellsPerBin];
-  uint32_t sl[kMaxCellsPerBin];  // first cell of each strip
   uint32_t eoff[kMaxCellsPerBin + 1];  // per cell: first membership and first item in the bin
   uint32_t ioff[kMaxCellsPerBin];
   float4 sbox[kCountStage];  // a bin of at most kCountStage memberships, laid out in place
-  uint32_t sid[kCountStage];
+  uint32_t sid[kCountStage];  // (for a bin placed by strips: the first cell of each strip)
   uint32_t soff[kSortThreads + 1];  // a chunk of bins: inclusive scan of their slices
   uint32_t boff[kSortThreads + 1];  // and of their other bins
   uint64_t wm[kSortThreads / 64], wi[kSortThreads / 64];
@@ -2883,7 +2883,7 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
   }
 #pragma unroll
   for (uint32_t k = 0; k < kSortPer; ++k)
-    if (open & (1u << k)) L.sl[sidx++] = tid * kSortPer + k;
+    if (open & (1u << k)) L.sid[sidx++] = tid * kSortPer + k;
   const bool sliced = F > kSortBigBin;
   if (sliced) nst = (m + kSortSlice - 1u) / kSortSlice;
   const bool here = whole && F <= kCountStage;  // placed by this workgroup
@@ -2938,7 +2938,7 @@ __device__ __forceinline__ void layout_bin(CountLds& L, const InstDesc& I, uint3
     }
     for (uint32_t t = tid; t < nst; t += kSortThreads) {
       const uint4 item = sliced ? make_uint4(b, t, 0u, kSortSliceTag)
-                                : make_uint4(b, L.sl[t], t + 1u < nst ? L.sl[t + 1u] : kk, 0u);
+                                : make_uint4(b, L.sid[t], t + 1u < nst ? L.sid[t + 1u] : kk, 0u);
       if ((uint64_t)sb + t < cap_strips) strips[sb + t] = item;
     }
     if (tid == 0) {
