@@ -1558,10 +1558,10 @@ int pbig_hip_create(pbig_hip_ctx** out, int device_id) {
   c->n_cu = prop.multiProcessorCount;
   if (pbig::binned_init() != hipSuccess) { delete c; return PBIG_E_HIP; }
   if (noalloc_setup(device_id, &c->noalloc) != hipSuccess) { delete c; return PBIG_E_HIP; }
-  // tile test: twice the resident workgroups (eight of four waves per CU),
-  // so that the second half starts as workgroups of the first run out of
+  // tile test: three times the resident workgroups (eight of four waves per
+  // CU), so that later workgroups start as those before them run out of
   // items, which evens out items of very different sizes
-  c->ws.test_blocks = (uint32_t)c->n_cu * 16;
+  c->ws.test_blocks = (uint32_t)c->n_cu * 24;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return PBIG_E_HIP; }
   if (hipMalloc((void**)&c->ws.meta, sizeof(Meta)) != hipSuccess) { delete c; return PBIG_E_NOMEM; }
   if (hipMemset(c->ws.meta, 0, sizeof(Meta)) != hipSuccess) {
